@@ -1,0 +1,137 @@
+"""Host-side graph description: numpy arrays in the layout of include/mgo.h's mgo_graph_desc.
+
+A GraphSpec is the structure-of-arrays form of a
+``Graph<SimulationNode<osecir::Simulation>, MobilityEdge>`` (cpp/memilio/mobility/graph.h:144-303):
+per-node osecir parameters (cpp/models/ode_secir/parameters.h:46-394), one contact matrix per node
+with its dampings (cpp/memilio/epidemiology/contact_matrix.h, damping.h), initial populations and
+the mobility edges with their coefficient vectors.
+"""
+import numpy as np
+
+from . import capi
+
+NC = capi.NUM_COMPARTMENTS
+
+# osecir::InfectionState order (cpp/models/ode_secir/infection_state.h:33-47)
+S, E, INS, INSC, ISY, ISYC, ISEV, ICR, R, D = range(NC)
+COMPARTMENT_NAMES = ["S", "E", "INS", "INSC", "ISy", "ISyC", "ISev", "ICr", "R", "D"]
+
+
+def default_scalars():
+    """Reference defaults of the scalar parameters (parameters.h:46-394, 764-767)."""
+    s = np.zeros(capi.NUM_SCALARS)
+    s[capi.P_START_DAY] = 0.0
+    s[capi.P_SEASONALITY] = 0.0
+    s[capi.P_ICU_CAPACITY] = capi.DBL_MAX
+    s[capi.P_TEST_AND_TRACE_CAPACITY] = capi.DBL_MAX
+    s[capi.P_TEST_AND_TRACE_MAX_RISK] = 5.0
+    return s
+
+
+def default_age_params(G):
+    """Reference defaults of the per-age parameters (parameters.h:96-324)."""
+    a = np.zeros((capi.NUM_AGE_PARAMS, G))
+    for k in (capi.A_TIME_EXPOSED, capi.A_TIME_INFECTED_NO_SYMPTOMS, capi.A_TIME_INFECTED_SYMPTOMS,
+              capi.A_TIME_INFECTED_SEVERE, capi.A_TIME_INFECTED_CRITICAL,
+              capi.A_TRANSMISSION_PROBABILITY_ON_CONTACT, capi.A_RELATIVE_TRANSMISSION_NO_SYMPTOMS,
+              capi.A_RISK_OF_INFECTION_FROM_SYMPTOMATIC, capi.A_MAX_RISK_OF_INFECTION_FROM_SYMPTOMATIC):
+        a[k, :] = 1.0
+    return a
+
+
+class GraphSpec:
+    """Arrays of one graph; ``desc()`` returns a capi.GraphDesc pointing into them."""
+
+    def __init__(self, num_nodes, G):
+        self.num_nodes = int(num_nodes)
+        self.G = int(G)
+        n = self.num_nodes
+        self.node_scalars = np.tile(default_scalars(), (n, 1))
+        self.age_params = np.tile(default_age_params(G)[None], (n, 1, 1))
+        self.contact_baseline = np.zeros((n, G, G))
+        self.contact_minimum = None
+        self.damping_offsets = None
+        self.damping_times = None
+        self.damping_levels = None
+        self.damping_types = None
+        self.damping_coeffs = None
+        self.populations = np.zeros((n, G, NC))
+        self.edge_start = np.zeros(0, np.int32)
+        self.edge_end = np.zeros(0, np.int32)
+        self.edge_coeffs = np.zeros((0, NC * G))
+        self._desc = None
+
+    # ---------------------------------------------------------------- building
+    def set_dampings(self, per_node):
+        """per_node: list (len num_nodes) of lists of (time, level, type, coeffs[G][G] or scalar)."""
+        G = self.G
+        offs = [0]
+        times, levels, types, coeffs = [], [], [], []
+        for lst in per_node:
+            for (t, lev, typ, c) in lst:
+                times.append(float(t))
+                levels.append(int(lev))
+                types.append(int(typ))
+                coeffs.append(np.broadcast_to(np.asarray(c, dtype=np.float64), (G, G)))
+            offs.append(len(times))
+        self.damping_offsets = np.asarray(offs, np.int32)
+        self.damping_times = np.asarray(times, np.float64)
+        self.damping_levels = np.asarray(levels, np.int32)
+        self.damping_types = np.asarray(types, np.int32)
+        self.damping_coeffs = (np.stack(coeffs) if coeffs else np.zeros((0, G, G))).astype(np.float64)
+        self._desc = None
+
+    def set_edges(self, start, end, coeffs):
+        self.edge_start = np.ascontiguousarray(start, dtype=np.int32)
+        self.edge_end = np.ascontiguousarray(end, dtype=np.int32)
+        c = np.asarray(coeffs, dtype=np.float64)
+        if c.ndim == 1:
+            c = np.repeat(c[:, None], NC * self.G, axis=1)
+        self.edge_coeffs = np.ascontiguousarray(c)
+        self._desc = None
+
+    @property
+    def num_edges(self):
+        return int(self.edge_start.shape[0])
+
+    # ---------------------------------------------------------------- C view
+    def _finalize(self):
+        for name in ("node_scalars", "age_params", "contact_baseline", "contact_minimum", "damping_times",
+                     "damping_coeffs", "populations", "edge_coeffs"):
+            v = getattr(self, name)
+            if v is not None:
+                setattr(self, name, np.ascontiguousarray(v, dtype=np.float64))
+        for name in ("damping_offsets", "damping_levels", "damping_types", "edge_start", "edge_end"):
+            v = getattr(self, name)
+            if v is not None:
+                setattr(self, name, np.ascontiguousarray(v, dtype=np.int32))
+
+    def desc(self):
+        if self._desc is None:
+            self._finalize()
+            d = capi.GraphDesc()
+            d.num_nodes = self.num_nodes
+            d.num_groups = self.G
+            d.node_scalars = capi.dptr(self.node_scalars)
+            d.age_params = capi.dptr(self.age_params)
+            d.contact_baseline = capi.dptr(self.contact_baseline)
+            d.contact_minimum = capi.dptr(self.contact_minimum)
+            d.damping_offsets = capi.iptr(self.damping_offsets)
+            d.damping_times = capi.dptr(self.damping_times)
+            d.damping_levels = capi.iptr(self.damping_levels)
+            d.damping_types = capi.iptr(self.damping_types)
+            d.damping_coeffs = capi.dptr(self.damping_coeffs)
+            d.populations = capi.dptr(self.populations)
+            d.num_edges = self.num_edges
+            d.edge_start = capi.iptr(self.edge_start) if self.num_edges else None
+            d.edge_end = capi.iptr(self.edge_end) if self.num_edges else None
+            d.edge_coeffs = capi.dptr(self.edge_coeffs) if self.num_edges else None
+            self._desc = d
+        return self._desc
+
+    def state_size(self):
+        return self.G * NC
+
+    def num_days(self, t0, tmax):
+        """Days of interpolate_simulation_result: ceil(t0 - 1e-10) .. floor(tmax + 1e-10)."""
+        return int(np.floor(tmax + 1e-10)) - int(np.ceil(t0 - 1e-10)) + 1

@@ -1,0 +1,161 @@
+"""ctypes loader of the oracle (oracle/liboracle.so).
+
+The oracle is the parity checker: only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg
+import this module.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from memilio_amd import capi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+
+class OrcStats(C.Structure):
+    _fields_ = [
+        ("steps", C.c_int64),
+        ("attempts", C.c_int64),
+        ("rhs_evals", C.c_int64),
+        ("negative_fixups", C.c_int64),
+        ("returns", C.c_int64),
+    ]
+
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle library {LIB_PATH} missing; run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        L.orc_simulate.restype = C.c_int32
+        L.orc_simulate.argtypes = [C.POINTER(capi.GraphDesc), C.POINTER(capi.SimCfg), C.c_double, C.c_int32, _dp, _dp]
+        L.orc_graph_run.restype = C.c_int32
+        L.orc_graph_run.argtypes = [C.POINTER(capi.GraphDesc), C.POINTER(capi.SimCfg), C.c_double, C.c_int32, _dp,
+                                    _ip, C.POINTER(OrcStats)]
+        L.orc_graph_run_node_series.restype = C.c_int32
+        L.orc_graph_run_node_series.argtypes = [C.POINTER(capi.GraphDesc), C.POINTER(capi.SimCfg), C.c_double,
+                                                C.c_int32, C.c_int32, _dp, _dp]
+        L.orc_ensemble_run.restype = C.c_int32
+        L.orc_ensemble_run.argtypes = [C.POINTER(capi.EnsembleDesc), C.c_int32, _dp, C.POINTER(OrcStats)]
+        L.orc_oseir_graph_series.restype = C.c_int32
+        L.orc_oseir_graph_series.argtypes = [C.c_int32, C.c_int32, _dp, _dp, _dp, C.c_int32, _ip, _ip, _dp, C.c_double,
+                                             C.c_double, C.c_double, C.c_double, C.c_int32, C.c_int32, _dp, _dp]
+        L.orc_edge_scenario.restype = C.c_int32
+        L.orc_edge_scenario.argtypes = [C.POINTER(capi.GraphDesc), C.c_double, _dp, _dp]
+        L.orc_integrate_test_problem.restype = C.c_int32
+        L.orc_integrate_test_problem.argtypes = [C.c_int32, C.POINTER(capi.IntegratorCfg), C.c_double, C.c_double,
+                                                 C.c_double, C.c_int32, _dp, _dp]
+        L.orc_contact_at.restype = C.c_double
+        L.orc_contact_at.argtypes = [C.POINTER(capi.GraphDesc), C.c_double, C.c_int32, C.c_int32]
+        _lib = L
+    return _lib
+
+
+def simulate(spec, cfg, tmax, capacity=200000):
+    """Full TimeSeries (times[n], values[n][10G]) of node 0 of spec simulated alone."""
+    n = spec.state_size()
+    times = np.zeros(capacity)
+    vals = np.zeros((capacity, n))
+    np_ = lib().orc_simulate(C.byref(spec.desc()), C.byref(cfg), tmax, capacity, capi.dptr(times), capi.dptr(vals))
+    if np_ < 0:
+        raise RuntimeError(f"orc_simulate failed: {np_}")
+    if np_ > capacity:
+        return simulate(spec, cfg, tmax, np_)
+    return times[:np_].copy(), vals[:np_].copy()
+
+
+def graph_run(spec, cfg, tmax):
+    """Daily values [day][node][10G] and stats of make_mobility_sim(...).advance(tmax)."""
+    nd = spec.num_days(cfg.t0, tmax)
+    out = np.zeros((nd, spec.num_nodes, spec.state_size()))
+    got = C.c_int32(0)
+    st = OrcStats()
+    s = lib().orc_graph_run(C.byref(spec.desc()), C.byref(cfg), tmax, nd, capi.dptr(out), C.byref(got), C.byref(st))
+    if s != 0:
+        raise RuntimeError(f"orc_graph_run failed: {s}")
+    return out[:got.value], st
+
+
+def graph_node_series(spec, cfg, tmax, node, capacity=400000):
+    n = spec.state_size()
+    times = np.zeros(capacity)
+    vals = np.zeros((capacity, n))
+    np_ = lib().orc_graph_run_node_series(C.byref(spec.desc()), C.byref(cfg), tmax, node, capacity,
+                                          capi.dptr(times), capi.dptr(vals))
+    if np_ < 0:
+        raise RuntimeError(f"orc_graph_run_node_series failed: {np_}")
+    if np_ > capacity:
+        return graph_node_series(spec, cfg, tmax, node, np_)
+    return times[:np_].copy(), vals[:np_].copy()
+
+
+def ensemble_run(specs, cfgs, tmaxs, threads=0, want_output=True):
+    """ParameterStudy over pre-sampled graphs; returns (list of [day][node][10G] arrays or None, stats)."""
+    nr = len(specs)
+    graphs = (capi.GraphDesc * nr)(*[s.desc() for s in specs])
+    cfg_arr = (capi.SimCfg * nr)(*cfgs)
+    tm = np.ascontiguousarray(tmaxs, dtype=np.float64)
+    desc = capi.EnsembleDesc(nr, graphs, cfg_arr, capi.dptr(tm))
+    sizes = [s.num_days(c.t0, t) * s.num_nodes * s.state_size() for s, c, t in zip(specs, cfgs, tm)]
+    out = np.zeros(sum(sizes)) if want_output else None
+    st = OrcStats()
+    s = lib().orc_ensemble_run(C.byref(desc), threads, capi.dptr(out) if want_output else None, C.byref(st))
+    if s != 0:
+        raise RuntimeError(f"orc_ensemble_run failed: {s}")
+    if not want_output:
+        return None, st
+    res, o = [], 0
+    for spec, c, t, sz in zip(specs, cfgs, tm, sizes):
+        res.append(out[o:o + sz].reshape(spec.num_days(c.t0, t), spec.num_nodes, spec.state_size()))
+        o += sz
+    return res, st
+
+
+def oseir_graph_series(params, contact, populations, edge_start, edge_end, edge_coeffs, t0, dt_graph, dt_node, tmax,
+                       node, capacity=20000):
+    params = np.ascontiguousarray(params, dtype=np.float64)
+    contact = np.ascontiguousarray(contact, dtype=np.float64)
+    populations = np.ascontiguousarray(populations, dtype=np.float64)
+    es = np.ascontiguousarray(edge_start, dtype=np.int32)
+    ee = np.ascontiguousarray(edge_end, dtype=np.int32)
+    ec = np.ascontiguousarray(edge_coeffs, dtype=np.float64)
+    nn, G = populations.shape[0], populations.shape[1]
+    times = np.zeros(capacity)
+    vals = np.zeros((capacity, 4 * G))
+    np_ = lib().orc_oseir_graph_series(nn, G, capi.dptr(params), capi.dptr(contact), capi.dptr(populations),
+                                       len(es), capi.iptr(es), capi.iptr(ee), capi.dptr(ec), t0, dt_graph, dt_node,
+                                       tmax, node, capacity, capi.dptr(times), capi.dptr(vals))
+    if np_ < 0:
+        raise RuntimeError("orc_oseir_graph_series failed")
+    return times[:np_].copy(), vals[:np_].copy()
+
+
+def edge_scenario(spec, t, coeffs):
+    n = spec.state_size()
+    out = np.zeros((6, n))
+    c = np.ascontiguousarray(coeffs, dtype=np.float64)
+    s = lib().orc_edge_scenario(C.byref(spec.desc()), t, capi.dptr(c), capi.dptr(out))
+    if s != 0:
+        raise RuntimeError("orc_edge_scenario failed")
+    return out.reshape(3, 2, n)
+
+
+def integrate_test_problem(cfg, t0, tmax, dt, capacity=100000):
+    times = np.zeros(capacity)
+    vals = np.zeros(capacity)
+    np_ = lib().orc_integrate_test_problem(0, C.byref(cfg), t0, tmax, dt, capacity, capi.dptr(times), capi.dptr(vals))
+    if np_ < 0:
+        raise RuntimeError("orc_integrate_test_problem failed")
+    return times[:np_].copy(), vals[:np_].copy()
+
+
+def contact_at(spec, t, i, j):
+    return lib().orc_contact_at(C.byref(spec.desc()), t, i, j)

@@ -1,0 +1,148 @@
+"""Model set-ups of the reference's own tests, rebuilt as GraphSpecs.
+
+Each function cites the reference test it mirrors; population initialisation repeats the reference's
+arithmetic (set_total / set_difference_from_total / set_difference_from_group_total,
+cpp/memilio/epidemiology/populations.h:135-249) so the initial values are bit-identical.
+"""
+import os
+
+import numpy as np
+
+from memilio_amd import capi
+from memilio_amd.graphspec import (GraphSpec, S, E, INS, INSC, ISY, ISYC, ISEV, ICR, R, D, NC)
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def load_csv(name):
+    """cpp/tests/load_test_data.h:37-72: whitespace separated, '#' lines and non-numeric rows skipped."""
+    rows = []
+    with open(os.path.join(GOLDEN, name)) as f:
+        for line in f:
+            if line.lstrip().startswith("#") or not line.strip():
+                continue
+            try:
+                rows.append([float(x) for x in line.split()])
+            except ValueError:
+                continue
+    return np.array(rows)
+
+
+def _set_common(spec, k, start_day, seasonality, ttc, T, tp, rel, Rr, risk, maxrisk, sev, cps, dpc, dps=0.0):
+    sc = spec.node_scalars[k]
+    sc[capi.P_START_DAY] = start_day
+    sc[capi.P_SEASONALITY] = seasonality
+    sc[capi.P_TEST_AND_TRACE_CAPACITY] = ttc
+    a = spec.age_params[k]
+    a[capi.A_TIME_EXPOSED] = T[0]
+    a[capi.A_TIME_INFECTED_NO_SYMPTOMS] = T[1]
+    a[capi.A_TIME_INFECTED_SYMPTOMS] = T[2]
+    a[capi.A_TIME_INFECTED_SEVERE] = T[3]
+    a[capi.A_TIME_INFECTED_CRITICAL] = T[4]
+    a[capi.A_TRANSMISSION_PROBABILITY_ON_CONTACT] = tp
+    a[capi.A_RELATIVE_TRANSMISSION_NO_SYMPTOMS] = rel
+    a[capi.A_RECOVERED_PER_INFECTED_NO_SYMPTOMS] = Rr
+    a[capi.A_RISK_OF_INFECTION_FROM_SYMPTOMATIC] = risk
+    a[capi.A_MAX_RISK_OF_INFECTION_FROM_SYMPTOMATIC] = maxrisk
+    a[capi.A_SEVERE_PER_INFECTED_SYMPTOMS] = sev
+    a[capi.A_CRITICAL_PER_SEVERE] = cps
+    a[capi.A_DEATHS_PER_SEVERE] = dps
+    a[capi.A_DEATHS_PER_CRITICAL] = dpc
+
+
+def _group_total(v):
+    """std::accumulate(group slice, 0.0) in compartment order (populations.h:136-142)."""
+    s = 0.0
+    for x in v:
+        s = s + float(x)
+    return s
+
+
+def secihurd_single(cps, set_total_first):
+    """G=1 set-up of test_odesecir.cpp:39-100 (set_total_first=True, cps 0.3) and of
+    pycode test_osecir.py:38-90 (set_total_first=False, cps 0.25)."""
+    spec = GraphSpec(1, 1)
+    p = spec.populations[0, 0]
+    if set_total_first:
+        # Populations::set_total on an all-zero array distributes 10000 evenly (populations.h:216-231)
+        p[:] = 10000.0 / 10.0
+    p[E], p[INS], p[INSC], p[ISY], p[ISYC], p[ISEV], p[ICR], p[R], p[D] = 100, 50, 0, 50, 0, 20, 10, 10, 0
+    # set_difference_from_total (populations.h:240-249): current = get_total() - y[S]
+    current = float(np.sum(p)) - p[S]
+    p[S] = 10000.0 - current
+    _set_common(spec, 0, 60.0, 0.2, 35.0, (3.2, 2.0, 5.8, 9.5, 7.1), 0.05, 0.7, 0.09, 0.25, 0.45, 0.2, cps, 0.3)
+    spec.contact_baseline[0] = 10.0
+    spec.set_dampings([[(30.0, 0, 0, 0.7)]])
+    return spec
+
+
+def secihurd_ageres(cps, G=3):
+    """G=3 set-ups of test_odesecir_ageres.cpp:27-189 and pycode test_osecir_ageres.py:37-99."""
+    fact = 1.0 / G
+    spec = GraphSpec(1, G)
+    for g in range(G):
+        p = spec.populations[0, g]
+        p[E] = fact * 100
+        p[INS] = fact * 50
+        p[INSC] = 0.0
+        p[ISY] = fact * 50
+        p[ISYC] = 0.0
+        p[ISEV] = fact * 20
+        p[ICR] = fact * 10
+        p[R] = fact * 10
+        p[D] = fact * 0
+        # set_difference_from_group_total (populations.h:194-205)
+        current = _group_total(p) - p[S]
+        p[S] = fact * 10000 - current
+    _set_common(spec, 0, 60.0, 0.2, 35.0, (3.2, 2.0, 5.8, 9.5, 7.1), 0.05, 0.7, 0.09, 0.25, 0.45, 0.2, cps, 0.3)
+    spec.contact_baseline[0] = fact * 10
+    spec.set_dampings([[(30.0, 0, 0, 0.7)]])
+    return spec
+
+
+def rkf45_cfg(rel, abs_, dt_min, dt_max, t0=0.0, dt=0.3):
+    return capi.sim_cfg(t0=t0, dt_node=dt, integrator=capi.integrator_cfg(capi.INTEGRATOR_RKF45, abs_, rel, dt_min,
+                                                                          dt_max))
+
+
+def ck_cfg(rel=1e-5, abs_=1e-10, dt_min=capi.DBL_MIN, dt_max=capi.DBL_MAX, t0=0.0, dt=0.1, dt_graph=0.5):
+    return capi.sim_cfg(t0=t0, dt_node=dt, dt_graph=dt_graph,
+                        integrator=capi.integrator_cfg(capi.INTEGRATOR_CASH_KARP54, abs_, rel, dt_min, dt_max))
+
+
+def oseir_fixture_inputs():
+    """test_graph_simulation.cpp:278-339: three identical SEIR nodes, complete graph, coefficients 0.001."""
+    total = 10000.0
+    pop = np.array([0.0, 100.0, 100.0, 100.0])
+    pop[0] = total - pop[1] - pop[2] - pop[3]
+    params = np.tile(np.array([5.2, 6.0, 0.04]), (3, 1))
+    contact = np.full((3, 1, 1), 10.0)
+    populations = np.tile(pop[None, None, :], (3, 1, 1))
+    es, ee = [], []
+    for i in range(3):
+        for j in range(3):
+            if i != j:
+                es.append(i)
+                ee.append(j)
+    coeffs = np.full((len(es), 4), 0.001)
+    return params, contact, populations, es, ee, coeffs
+
+
+def mobility_pair_model():
+    """test_mobility.cpp:154-178: G=1, contact 5, ISy 10 of 1000, tp = risk = rel = 1, sev 0.5, T_E = T_INS = 1."""
+    spec = GraphSpec(1, 1)
+    p = spec.populations[0, 0]
+    p[ISY] = 10
+    p[INSC] = 0
+    p[ISYC] = 0
+    current = float(np.sum(p)) - p[S]
+    p[S] = 1000.0 - current
+    a = spec.age_params[0]
+    a[capi.A_TRANSMISSION_PROBABILITY_ON_CONTACT] = 1.0
+    a[capi.A_RISK_OF_INFECTION_FROM_SYMPTOMATIC] = 1.0
+    a[capi.A_RELATIVE_TRANSMISSION_NO_SYMPTOMS] = 1.0
+    a[capi.A_SEVERE_PER_INFECTED_SYMPTOMS] = 0.5
+    a[capi.A_TIME_EXPOSED] = 1.0
+    a[capi.A_TIME_INFECTED_NO_SYMPTOMS] = 1.0
+    spec.contact_baseline[0, 0, 0] = 5.0
+    return spec

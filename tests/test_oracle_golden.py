@@ -1,0 +1,125 @@
+"""The oracle against the reference's golden fixtures and known answers (CPU only).
+
+These pin the oracle before it is trusted as the checker of the device kernels.
+"""
+import numpy as np
+import pytest
+
+from memilio_amd import capi
+from tests import setups
+
+
+def _compare_series(times, vals, ref, tol, groups=1):
+    assert len(times) == ref.shape[0], f"number of time points {len(times)} != {ref.shape[0]}"
+    np.testing.assert_allclose(times, ref[:, 0], rtol=0, atol=tol)
+    summed = vals.reshape(len(times), groups, 10).sum(axis=1) if groups > 1 else vals
+    ncol = ref.shape[1] - 1
+    np.testing.assert_allclose(summed[:, :ncol], ref[:, 1:], rtol=0, atol=tol)
+
+
+def test_secihurd_rkf45_single(oracle):
+    """test_odesecir.cpp:39-112 compareWithPreviousRun: RKF45, dt in [0.3, 1], rel 1e-3, abs 1e-1."""
+    spec = setups.secihurd_single(cps=0.3, set_total_first=True)
+    cfg = setups.rkf45_cfg(rel=1e-3, abs_=1e-1, dt_min=0.3, dt_max=1.0)
+    t, v = oracle.simulate(spec, cfg, 50.0)
+    _compare_series(t, v, setups.load_csv("secihurd-compare.csv"), 1e-10)
+
+
+def test_secihurd_rkf45_ageres(oracle):
+    """test_odesecir_ageres.cpp:27-108: G=3, RKF45, rel 1e-4, summed over age groups."""
+    spec = setups.secihurd_ageres(cps=0.3)
+    cfg = setups.rkf45_cfg(rel=1e-4, abs_=1e-1, dt_min=0.3, dt_max=1.0)
+    t, v = oracle.simulate(spec, cfg, 50.0)
+    _compare_series(t, v, setups.load_csv("secihurd-compare.csv"), 1e-10, groups=3)
+
+
+def test_secihurd_cashkarp_ageres(oracle):
+    """test_odesecir_ageres.cpp:110-189: G=3, odeint Cash-Karp, dt in [0.3, 1], rel 1e-4, abs 1e-1."""
+    spec = setups.secihurd_ageres(cps=0.25)
+    cfg = setups.ck_cfg(rel=1e-4, abs_=1e-1, dt_min=0.3, dt_max=1.0, dt=0.3)
+    t, v = oracle.simulate(spec, cfg, 50.0)
+    _compare_series(t, v, setups.load_csv("secihurd-compare-cashkarp.csv"), 1e-10, groups=3)
+
+
+def test_default_integrator_single(oracle):
+    """pycode test_osecir.py:122-150: default core (Cash-Karp, abs 1e-10, rel 1e-5), dt0 0.1, tmax 50.
+
+    The time column pins the odeint controller: 0.1 -> 0.55 is the x4.5 growth cap and the rejection
+    after t=25.3 the x0.2 shrink floor."""
+    spec = setups.secihurd_single(cps=0.25, set_total_first=False)
+    cfg = setups.ck_cfg()
+    t, v = oracle.simulate(spec, cfg, 50.0)
+    _compare_series(t, v, setups.load_csv("ode-secihurd-compare.csv"), 1e-10)
+
+
+def test_default_integrator_ageres(oracle):
+    """pycode test_osecir_ageres.py:116-150: G=3, default core, summed over age groups, 1e-10."""
+    spec = setups.secihurd_ageres(cps=0.25)
+    cfg = setups.ck_cfg()
+    t, v = oracle.simulate(spec, cfg, 50.0)
+    _compare_series(t, v, setups.load_csv("ode-secihurd-ageres-compare.csv"), 1e-10, groups=3)
+
+
+def test_graph_fixture_oseir(oracle):
+    """test_graph_simulation.cpp:278-339 consistencyFlowMobility: node 0 of the 3-node SEIR graph."""
+    params, contact, pops, es, ee, coeffs = setups.oseir_fixture_inputs()
+    t, v = oracle.oseir_graph_series(params, contact, pops, es, ee, coeffs, 0.0, 0.001, 0.1, 1.0, 0)
+    ref = setups.load_csv("graphsimulation-compare.csv")
+    assert len(t) == ref.shape[0]
+    np.testing.assert_allclose(t, ref[:, 0], rtol=0, atol=1e-10)
+    np.testing.assert_allclose(v, ref[:, 1:], rtol=0, atol=1e-10)
+
+
+def test_edge_apply_mobility_known_answers(oracle):
+    """test_mobility.cpp:154-215 edgeApplyMobility: exact departure values and conserved returns."""
+    spec = setups.mobility_pair_model()
+    out = oracle.edge_scenario(spec, 3.125, np.full(10, 0.1))
+    n1, n2 = out[0]
+    exp1 = np.array([990 - 99, 0, 0, 0, 10 - 1, 0, 0, 0, 0, 0], float)
+    exp2 = np.array([990 + 99, 0, 0, 0, 10 + 1, 0, 0, 0, 0, 0], float)
+    assert np.array_equal(n1, exp1)
+    assert np.array_equal(n2, exp2)
+    v = out[1][0]
+    assert v.sum() == pytest.approx(1000, rel=4 * np.finfo(float).eps)
+    assert v[0] < 990 and abs(v[0] - 990) < 50
+    assert v[1] > 0 and abs(v[1]) < 50
+    assert v[2] > 0 and abs(v[2]) < 5
+    assert abs(v[4] - 10) < 5
+    assert v[6] > 0 and abs(v[6]) < 5
+    assert out[1][1].sum() == pytest.approx(1000, rel=4 * np.finfo(float).eps)
+    assert out[2][0].sum() == pytest.approx(900, rel=4 * np.finfo(float).eps)
+    assert out[2][1].sum() == pytest.approx(1100, rel=4 * np.finfo(float).eps)
+
+
+@pytest.mark.parametrize("kind", [capi.INTEGRATOR_CASH_KARP54, capi.INTEGRATOR_RKF45])
+def test_adaptive_sine(oracle, kind):
+    """test_numericalIntegration.cpp:119-160: adaptive cores (abs = rel = 1e-7, dt in [1e-3, 1]) on
+    y' = cos t, y = sin t; error metric sqrt(sum e^2)/n < 1e-7 as the reference computes it."""
+    cfg = capi.integrator_cfg(kind, 1e-7, 1e-7, 1e-3, 1.0)
+    t, y = oracle.integrate_test_problem(cfg, 0.0, 2 * np.pi, 2 * np.pi / 10)
+    assert t[-1] == pytest.approx(2 * np.pi, abs=1e-12)
+    err = np.sqrt(np.sum((y - np.sin(t)) ** 2)) / (len(t) - 1)
+    assert err < 1e-7
+
+
+@pytest.mark.parametrize("kind", [capi.INTEGRATOR_CASH_KARP54_FIXED, capi.INTEGRATOR_EULER])
+def test_fixed_step_sine(oracle, kind):
+    """test_numericalIntegration.cpp:82-109: fixed-step cores on y' = cos t with n = 1000 steps; error
+    metric sqrt(sum e^2)/n < 1e-3; the 5th-order Cash-Karp solution is also checked far tighter."""
+    cfg = capi.integrator_cfg(kind, 0, 0, 0, 0)
+    t, y = oracle.integrate_test_problem(cfg, 0.0, 2 * np.pi, 2 * np.pi / 1000)
+    assert len(t) == 1001
+    err = np.sqrt(np.sum((y - np.sin(t)) ** 2)) / 1000
+    assert err < 1e-3
+    if kind == capi.INTEGRATOR_CASH_KARP54_FIXED:
+        assert np.max(np.abs(y - np.sin(t))) < 1e-12
+
+
+def test_adaptive_step_size_contract(oracle):
+    """test_numericalIntegration.cpp:162-247 adaptiveStepSizing: every step lies in [dt_min, dt_max]."""
+    for kind in (capi.INTEGRATOR_CASH_KARP54, capi.INTEGRATOR_RKF45):
+        cfg = capi.integrator_cfg(kind, 1e-6, 1e-6, 0.01, 0.5)
+        t, _ = oracle.integrate_test_problem(cfg, 0.0, 10.0, 0.1)
+        dts = np.diff(t)
+        assert np.all(dts[:-1] >= 0.01 - 1e-15) and np.all(dts <= 0.5 + 1e-15)
+        assert t[-1] == pytest.approx(10.0, abs=1e-12)
