@@ -156,6 +156,7 @@ typedef struct mgo_device_cfg {
 } mgo_device_cfg;
 
 #define MGO_CTX_NO_GRAPH_CAPTURE 1   /* launch the half-day kernels directly instead of replaying a hipGraph */
+#define MGO_CTX_PROFILE_KERNELS 2    /* time every node / edge kernel launch with HIP events (no graph capture) */
 
 mgo_status mgo_context_create(const mgo_device_cfg* cfg, mgo_context** out);
 mgo_status mgo_context_destroy(mgo_context* ctx);
@@ -210,6 +211,10 @@ typedef struct mgo_run_info {
     int64_t attempts;          /* step attempts summed over all nodes */
     int64_t exact_replays;     /* return phases that needed the sequential negative-value fix-up replay */
     int32_t half_day_cycles;   /* graph steps executed */
+    int32_t node_launches;     /* node-phase kernel launches timed (MGO_CTX_PROFILE_KERNELS) */
+    double node_kernel_ms;     /* summed duration of those launches */
+    int32_t edge_launches;     /* edge-phase kernel launches timed */
+    double edge_kernel_ms;
 } mgo_run_info;
 
 mgo_status mgo_ensemble_run(mgo_context* ctx, const mgo_ensemble_desc* desc, double* daily_out,
@@ -220,7 +225,12 @@ mgo_status mgo_ensemble_run(mgo_context* ctx, const mgo_ensemble_desc* desc, dou
 typedef int64_t mgo_batch_id;
 mgo_status mgo_batch_create(mgo_context* ctx, const mgo_ensemble_desc* desc, mgo_batch_id* out);
 mgo_status mgo_batch_run(mgo_context* ctx, mgo_batch_id id, mgo_run_info* info);
-mgo_status mgo_batch_read_daily(mgo_context* ctx, mgo_batch_id id, double* daily_out);
+/* Daily values of runs first_run .. first_run+num_runs-1, laid out as mgo_ensemble_run's daily_out. */
+mgo_status mgo_batch_read_daily(mgo_context* ctx, mgo_batch_id id, int32_t first_run, int32_t num_runs,
+                                double* daily_out);
+/* Restores the batch to its state at creation (initial values, clocks, step sizes, counters), so that the
+ * same resident inputs can be run again. */
+mgo_status mgo_batch_reset(mgo_context* ctx, mgo_batch_id id);
 mgo_status mgo_batch_destroy(mgo_context* ctx, mgo_batch_id id);
 
 /* ---------------------------------------------------------------- serving loop

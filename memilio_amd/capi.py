@@ -59,6 +59,7 @@ INTEGRATOR_CASH_KARP54_FIXED = 2
 INTEGRATOR_EULER = 3
 
 CTX_NO_GRAPH_CAPTURE = 1
+CTX_PROFILE_KERNELS = 2
 
 DBL_MAX = np.finfo(np.float64).max
 DBL_MIN = np.finfo(np.float64).tiny
@@ -133,6 +134,10 @@ class RunInfo(C.Structure):
         ("attempts", C.c_int64),
         ("exact_replays", C.c_int64),
         ("half_day_cycles", C.c_int32),
+        ("node_launches", C.c_int32),
+        ("node_kernel_ms", C.c_double),
+        ("edge_launches", C.c_int32),
+        ("edge_kernel_ms", C.c_double),
     ]
 
 
@@ -191,7 +196,8 @@ SIGNATURES = {
     "mgo_ensemble_run": (C.c_int32, [C.c_void_p, C.POINTER(EnsembleDesc), _dp, C.POINTER(RunInfo)]),
     "mgo_batch_create": (C.c_int32, [C.c_void_p, C.POINTER(EnsembleDesc), C.POINTER(C.c_int64)]),
     "mgo_batch_run": (C.c_int32, [C.c_void_p, C.c_int64, C.POINTER(RunInfo)]),
-    "mgo_batch_read_daily": (C.c_int32, [C.c_void_p, C.c_int64, _dp]),
+    "mgo_batch_read_daily": (C.c_int32, [C.c_void_p, C.c_int64, C.c_int32, C.c_int32, _dp]),
+    "mgo_batch_reset": (C.c_int32, [C.c_void_p, C.c_int64]),
     "mgo_batch_destroy": (C.c_int32, [C.c_void_p, C.c_int64]),
     "mgo_serve_start": (C.c_int32, [C.c_void_p, C.POINTER(ServeCfg)]),
     "mgo_serve_submit": (C.c_int32, [C.c_void_p, C.POINTER(Request), C.POINTER(C.c_int64)]),

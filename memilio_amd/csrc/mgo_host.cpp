@@ -1,0 +1,1350 @@
+// mgo_host.cpp - host engine behind the C ABI of include/mgo.h.
+//
+// Builds device-resident batches of graph simulations (parameters, state, CSR edge structures,
+// damping caches, daily output buffers), drives the half-day cycle of
+// GraphSimulation::advance (cpp/memilio/mobility/graph_simulation.h:101-120) with the HIP kernels of
+// mgo_kernels.hip, and runs the serving loop (continuous batching over the same engine).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <tuple>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/mgo.h"
+#include "mgo_internal.h"
+
+namespace mgo
+{
+hipError_t launch_clock(InstDev* inst, int n, hipStream_t s);
+hipError_t launch_node(const ClassDev& c, InstDev* inst, hipStream_t s);
+hipError_t launch_edge(const ClassDev& c, InstDev* inst, hipStream_t s);
+} // namespace mgo
+
+using namespace mgo;
+
+namespace
+{
+
+struct Error {
+    mgo_status code;
+    std::string msg;
+};
+
+[[noreturn]] void fail(mgo_status code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    throw Error{code, buf};
+}
+
+void hip_check(hipError_t e, const char* what)
+{
+    if (e != hipSuccess) {
+        if (e == hipErrorOutOfMemory) {
+            fail(MGO_ERR_OUT_OF_MEMORY, "%s: %s", what, hipGetErrorString(e));
+        }
+        fail(MGO_ERR_DEVICE, "%s: %s", what, hipGetErrorString(e));
+    }
+}
+
+// device buffer owned by a batch
+struct DevBuf {
+    void* p      = nullptr;
+    size_t bytes = 0;
+    DevBuf()     = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf()
+    {
+        if (p) {
+            (void)hipFree(p);
+        }
+    }
+    void alloc(size_t b)
+    {
+        bytes = std::max<size_t>(b, 8);
+        hip_check(hipMalloc(&p, bytes), "hipMalloc");
+    }
+    template <class T>
+    void upload(const std::vector<T>& v, hipStream_t s)
+    {
+        alloc(v.size() * sizeof(T));
+        if (!v.empty()) {
+            hip_check(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s), "upload");
+        }
+    }
+    template <class T>
+    T* as() const
+    {
+        return static_cast<T*>(p);
+    }
+};
+
+// ------------------------------------------------------------------------------------------ dampings
+// Accumulated damping cache of one contact matrix (cpp/memilio/epidemiology/damping.h:549-658):
+// entries (time, D) with sentinels at lowest() and max(); same-level dampings add, different levels
+// combine as 1 - prod(1 - d), equal (time, type, level) replaces.
+struct DampIn {
+    double time;
+    int level, type;
+    std::vector<double> m;
+};
+
+void damping_cache(std::vector<DampIn> list, int G, std::vector<double>& times, std::vector<double>& mats)
+{
+    // insert_sorted_replace on key (time, type, level) in input order (utils/stl_util.h:82-96)
+    std::vector<DampIn> sorted;
+    auto less = [](const DampIn& a, const DampIn& b) {
+        return std::make_tuple(a.time, a.type, a.level) < std::make_tuple(b.time, b.type, b.level);
+    };
+    for (auto& d : list) {
+        auto lb = std::lower_bound(sorted.begin(), sorted.end(), d, less);
+        if (lb != sorted.end() && !less(d, *lb)) {
+            *lb = d;
+        }
+        else {
+            sorted.insert(lb, d);
+        }
+    }
+    const size_t n = size_t(G) * G;
+    times.assign(1, std::numeric_limits<double>::lowest());
+    mats.assign(n, 0.0);
+    std::vector<std::tuple<std::vector<double>, int, int>> active;
+    std::vector<std::pair<std::vector<double>, int>> by_level;
+    for (auto& d : sorted) {
+        auto ita = std::find_if(active.begin(), active.end(), [&](auto& a) {
+            return std::get<1>(a) == d.level && std::get<2>(a) == d.type;
+        });
+        if (ita != active.end()) {
+            auto& sum = *std::find_if(by_level.begin(), by_level.end(), [&](auto& s) {
+                return s.second == d.level;
+            });
+            auto& act = std::get<0>(*ita);
+            for (size_t k = 0; k < n; ++k) {
+                sum.first[k] += d.m[k] - act[k];
+            }
+            act = d.m;
+        }
+        else {
+            active.emplace_back(d.m, d.level, d.type);
+            auto its = std::find_if(by_level.begin(), by_level.end(), [&](auto& s) {
+                return s.second == d.level;
+            });
+            if (its != by_level.end()) {
+                for (size_t k = 0; k < n; ++k) {
+                    its->first[k] += d.m[k];
+                }
+            }
+            else {
+                by_level.emplace_back(d.m, d.level);
+            }
+        }
+        std::vector<double> comb = by_level.front().first;
+        for (size_t l = 1; l < by_level.size(); ++l) {
+            for (size_t k = 0; k < n; ++k) {
+                const double prod = comb[k] * by_level[l].first[k];
+                comb[k]           = comb[k] + by_level[l].first[k] - prod;
+            }
+        }
+        const double tb   = times.back();
+        const double diff = std::abs(d.time - tb);
+        if (diff <= 1e-15 || diff <= std::max(std::abs(d.time), std::abs(tb)) * 1e-15) {
+            std::copy(comb.begin(), comb.end(), mats.end() - long(n));
+        }
+        else {
+            times.push_back(d.time);
+            mats.insert(mats.end(), comb.begin(), comb.end());
+        }
+    }
+    times.push_back(std::numeric_limits<double>::max());
+    std::vector<double> last(mats.end() - long(n), mats.end());
+    mats.insert(mats.end(), last.begin(), last.end());
+}
+
+// ------------------------------------------------------------------------------------------ batch
+struct RunSpec {
+    const mgo_graph_desc* g;
+    mgo_sim_cfg cfg;
+    double tmax;     // target of this run (ensemble) or the output horizon (graph API)
+    bool single;     // mgo_simulate_single: one phase, exact end at tmax
+    std::vector<DampIn> extra_damping; // scenario updates added at submission (serving)
+};
+
+struct EdgeStructHost {
+    std::vector<int32_t> in_ptr, out_ptr, split, seq;
+    std::vector<EdgeEnt> in_ent, out_ent;
+    int n = 0, num_edges = 0;
+};
+
+struct ClassHost {
+    int G, kind;
+    std::vector<int> insts; // global instance indices
+    int num_nodes = 0;
+    // device buffers
+    DevBuf y, snap, ypre, fac, rc, lcoef, cbase, cdelta, nodep, dtime, dmat, tnode, dtnode, steps, attempts, out,
+        ipool, epool, cvec, estructs, inst_ids;
+    DevBuf y0;                         // initial state, for reset
+    std::vector<double> tnode0, dtnode0; // initial clocks, for reset
+    size_t out_doubles = 0;
+    ClassDev dev{};
+};
+
+struct InstHost {
+    int cls;
+    int node_base;
+    int num_nodes;
+    int G;
+    double t;        // host mirror of the device clock (same arithmetic)
+    double dt_graph;
+    double tmax;
+    double t0;
+    int day0, ndays;
+    int64_t out_base;
+    bool has_edges;
+    bool single;
+};
+
+int day_floor(double t)
+{
+    return int(std::floor(t + 1e-10));
+}
+int day_ceil(double t)
+{
+    return int(std::ceil(t - 1e-10));
+}
+
+void validate(const mgo_graph_desc* g, const mgo_sim_cfg& cfg)
+{
+    if (!g) {
+        fail(MGO_ERR_INVALID_VALUE, "graph description is NULL");
+    }
+    if (g->num_nodes < 1) {
+        fail(MGO_ERR_INVALID_VALUE, "num_nodes must be >= 1 (got %d)", g->num_nodes);
+    }
+    if (g->num_groups < 1 || g->num_groups > MGO_MAX_GROUPS) {
+        fail(MGO_ERR_UNSUPPORTED, "num_groups must be in 1..%d (got %d)", MGO_MAX_GROUPS, g->num_groups);
+    }
+    if (!g->node_scalars || !g->age_params || !g->contact_baseline || !g->populations) {
+        fail(MGO_ERR_INVALID_VALUE, "node_scalars, age_params, contact_baseline and populations are required");
+    }
+    if (g->num_edges < 0 || (g->num_edges > 0 && (!g->edge_start || !g->edge_end || !g->edge_coeffs))) {
+        fail(MGO_ERR_INVALID_VALUE, "edge arrays missing");
+    }
+    if (g->damping_offsets && (!g->damping_times || !g->damping_coeffs)) {
+        fail(MGO_ERR_INVALID_VALUE, "damping arrays missing");
+    }
+    const int k = cfg.integrator.kind;
+    if (k < MGO_INTEGRATOR_CASH_KARP54 || k > MGO_INTEGRATOR_EULER) {
+        fail(MGO_ERR_INVALID_VALUE, "unknown integrator kind %d", k);
+    }
+    if ((k == MGO_INTEGRATOR_CASH_KARP54 || k == MGO_INTEGRATOR_RKF45) &&
+        (!(cfg.integrator.dt_min >= 0) || !(cfg.integrator.dt_min <= cfg.integrator.dt_max) ||
+         !(cfg.integrator.abs_tol >= 0) || !(cfg.integrator.rel_tol >= 0))) {
+        fail(MGO_ERR_INVALID_VALUE, "integrator bounds/tolerances invalid");
+    }
+    if (!(cfg.dt_node > 0) || !std::isfinite(cfg.t0)) {
+        fail(MGO_ERR_INVALID_VALUE, "dt_node must be > 0 and t0 finite");
+    }
+    const int G = g->num_groups;
+    for (int e = 0; e < g->num_edges; ++e) {
+        if (g->edge_start[e] < 0 || g->edge_start[e] >= g->num_nodes || g->edge_end[e] < 0 ||
+            g->edge_end[e] >= g->num_nodes) {
+            fail(MGO_ERR_OUT_OF_RANGE, "edge %d references node out of range", e);
+        }
+    }
+    for (int v = 0; v < g->num_nodes; ++v) {
+        const double* a = g->age_params + size_t(v) * MGO_NUM_AGE_PARAMS * G;
+        for (int q = 0; q < 5 * G; ++q) {
+            if (!(a[q] > 0)) {
+                fail(MGO_ERR_INVALID_VALUE, "node %d: stay times must be > 0", v);
+            }
+        }
+        for (int q = 0; q < G * NC; ++q) {
+            if (!std::isfinite(g->populations[size_t(v) * G * NC + q])) {
+                fail(MGO_ERR_INVALID_VALUE, "node %d: populations must be finite", v);
+            }
+        }
+    }
+}
+
+// lane coefficients: the reference's subexpressions (cpp/models/ode_secir/model.h:93-212)
+void lane_coefficients(const mgo_graph_desc* g, int v, int a, double* c)
+{
+    const int G       = g->num_groups;
+    auto P            = [&](int k) {
+        return g->age_params[(size_t(v) * MGO_NUM_AGE_PARAMS + k) * G + a];
+    };
+    const double TE   = P(MGO_A_TIME_EXPOSED), TINS = P(MGO_A_TIME_INFECTED_NO_SYMPTOMS),
+                 TISY = P(MGO_A_TIME_INFECTED_SYMPTOMS), TISEV = P(MGO_A_TIME_INFECTED_SEVERE),
+                 TICR = P(MGO_A_TIME_INFECTED_CRITICAL);
+    const double Rr = P(MGO_A_RECOVERED_PER_INFECTED_NO_SYMPTOMS), sev = P(MGO_A_SEVERE_PER_INFECTED_SYMPTOMS),
+                 cps = P(MGO_A_CRITICAL_PER_SEVERE), dps = P(MGO_A_DEATHS_PER_SEVERE),
+                 dpc = P(MGO_A_DEATHS_PER_CRITICAL);
+    c[LC_E]        = (1.0 / TE);
+    c[LC_INS_ISY]  = (1 - Rr) * (1 / TINS);
+    c[LC_INS_R]    = Rr * (1.0 / TINS);
+    c[LC_ISY_ISEV] = sev / TISY;
+    c[LC_ISY_R]    = (1 - sev) / TISY;
+    c[LC_ISEV_ICR] = cps / TISEV;
+    c[LC_ISEV_R]   = (1.0 - cps - dps) / TISEV;
+    c[LC_ISEV_D]   = (dps + (cps - cps)) / TISEV;
+    c[LC_ICR_D]    = dpc / TICR;
+    c[LC_ICR_R]    = (1.0 - dpc) / TICR;
+    c[LC_TP]       = P(MGO_A_TRANSMISSION_PROBABILITY_ON_CONTACT);
+    c[LC_REL]      = P(MGO_A_RELATIVE_TRANSMISSION_NO_SYMPTOMS);
+    c[LC_RISK]     = P(MGO_A_RISK_OF_INFECTION_FROM_SYMPTOMATIC);
+    c[LC_MAXRISK]  = P(MGO_A_MAX_RISK_OF_INFECTION_FROM_SYMPTOMATIC);
+    c[LC_TTR]      = (1 - Rr) / TINS;
+    c[LC_CPS]      = cps;
+    c[LC_DPS]      = dps;
+    c[LC_TISEV]    = TISEV;
+}
+
+// CSR of one graph: Graph::add_edge ordering (sorted by (start, end), duplicates replaced by the later
+// edge, graph.h:231-241); edges without a positive coefficient never move anyone and are dropped.
+void build_edges(const mgo_graph_desc* g, EdgeStructHost& es, std::vector<double>& cvec)
+{
+    const int n  = g->num_nodes;
+    const int G  = g->num_groups;
+    const int nv = G * NC;
+    std::map<std::pair<int, int>, int> last; // (start, end) -> edge index of the last occurrence
+    for (int e = 0; e < g->num_edges; ++e) {
+        last[{g->edge_start[e], g->edge_end[e]}] = e;
+    }
+    struct E {
+        int s, d;
+        double cval;
+        int cidx;
+    };
+    std::vector<E> edges;
+    edges.reserve(last.size());
+    for (auto& kv : last) {
+        const double* c = g->edge_coeffs + size_t(kv.second) * nv;
+        bool any_pos = false, uniform = true;
+        for (int k = 0; k < nv; ++k) {
+            any_pos = any_pos || (c[k] > 0.0);
+            uniform = uniform && (c[k] == c[0]);
+        }
+        if (!any_pos) {
+            continue;
+        }
+        E ed{kv.first.first, kv.first.second, c[0], -1};
+        if (!uniform) {
+            ed.cidx = int(cvec.size() / size_t(nv));
+            cvec.insert(cvec.end(), c, c + nv);
+        }
+        edges.push_back(ed);
+    }
+    es.n         = n;
+    es.num_edges = int(edges.size());
+    es.out_ptr.assign(size_t(n) + 1, 0);
+    es.in_ptr.assign(size_t(n) + 1, 0);
+    for (auto& e : edges) {
+        es.out_ptr[size_t(e.s) + 1]++;
+        es.in_ptr[size_t(e.d) + 1]++;
+    }
+    for (int v = 0; v < n; ++v) {
+        es.out_ptr[size_t(v) + 1] += es.out_ptr[size_t(v)];
+        es.in_ptr[size_t(v) + 1] += es.in_ptr[size_t(v)];
+    }
+    es.out_ent.resize(edges.size());
+    es.in_ent.resize(edges.size());
+    es.split.assign(size_t(n), 0);
+    es.seq.clear();
+    std::vector<int> ofill(es.out_ptr.begin(), es.out_ptr.end() - 1), ifill(es.in_ptr.begin(), es.in_ptr.end() - 1);
+    // edges are sorted by (s, d): out lists come out sorted by d, in lists sorted by s
+    for (auto& e : edges) {
+        const int ko    = ofill[size_t(e.s)]++;
+        es.out_ent[size_t(ko)] = EdgeEnt{e.cval, e.d, e.cidx};
+        const int ki    = ifill[size_t(e.d)]++;
+        es.in_ent[size_t(ki)] = EdgeEnt{e.cval, e.s, e.cidx};
+        if (e.s < e.d) {
+            es.split[size_t(e.d)]++;
+        }
+        es.seq.push_back(e.s);
+        es.seq.push_back(e.d);
+        es.seq.push_back(ko);
+    }
+}
+
+struct Batch {
+    hipStream_t stream = nullptr;
+    std::deque<ClassHost> classes;
+    std::vector<InstHost> inst;
+    DevBuf inst_dev;
+    int64_t cycles_run = 0;
+    double last_ms     = 0;
+    hipGraphExec_t graph_exec = nullptr;
+    bool use_graph            = true;
+    bool profile              = false;
+    size_t device_bytes       = 0;
+    std::vector<InstDev> inst0; // instance table at creation (tmax set), for reset
+    // kernel timing of the last run (profile mode)
+    double node_ms = 0, edge_ms = 0;
+    int node_launches = 0, edge_launches = 0;
+
+    ~Batch()
+    {
+        if (graph_exec) {
+            (void)hipGraphExecDestroy(graph_exec);
+        }
+    }
+
+    size_t class_index(std::map<std::pair<int, int>, int>& cmap, int G, int kind)
+    {
+        auto it = cmap.find({G, kind});
+        if (it != cmap.end()) {
+            return size_t(it->second);
+        }
+        cmap[{G, kind}] = int(classes.size());
+        classes.emplace_back();
+        classes.back().G    = G;
+        classes.back().kind = kind;
+        return classes.size() - 1;
+    }
+
+    void build(std::vector<RunSpec>& runs, hipStream_t s)
+    {
+        stream = s;
+        for (auto& r : runs) {
+            validate(r.g, r.cfg);
+            if (!r.single && !(r.cfg.dt_graph > 0)) {
+                fail(MGO_ERR_INVALID_VALUE, "dt_graph must be > 0");
+            }
+        }
+        std::map<std::pair<int, int>, int> cmap;
+        inst.resize(runs.size());
+        for (size_t i = 0; i < runs.size(); ++i) {
+            const auto& r = runs[i];
+            const int ci  = int(class_index(cmap, r.g->num_groups, r.cfg.integrator.kind));
+            auto& C       = classes[size_t(ci)];
+            InstHost& I   = inst[i];
+            I.cls         = ci;
+            I.G           = r.g->num_groups;
+            I.num_nodes   = r.g->num_nodes;
+            I.node_base   = C.num_nodes;
+            C.num_nodes += I.num_nodes;
+            C.insts.push_back(int(i));
+            I.t0       = r.cfg.t0;
+            I.t        = r.cfg.t0;
+            I.dt_graph = r.single ? INFINITY : r.cfg.dt_graph;
+            I.tmax     = r.cfg.t0;
+            I.single   = r.single;
+            I.day0     = day_ceil(r.cfg.t0);
+            I.ndays    = std::max(0, day_floor(r.tmax) - I.day0 + 1);
+        }
+        std::vector<InstDev> idev(runs.size());
+        for (size_t ci = 0; ci < classes.size(); ++ci) {
+            build_class(ci, runs, idev);
+        }
+        inst_dev.upload(idev, stream);
+        device_bytes += inst_dev.bytes;
+        hip_check(hipStreamSynchronize(stream), "batch upload");
+    }
+
+    void build_class(size_t ci, std::vector<RunSpec>& runs, std::vector<InstDev>& idev)
+    {
+        ClassHost& C = classes[ci];
+        const int G  = C.G;
+        const size_t N = size_t(C.num_nodes);
+        std::vector<double> y(N * G * NC), lcoef(N * G * NUM_LC), cbase(N * G * G), cdelta(N * G * G), tnode(N),
+            dtnode(N), dtime, dmat;
+        std::vector<NodeP> nodep(N);
+        std::vector<int32_t> ipool, inst_ids;
+        std::vector<EdgeEnt> epool;
+        std::vector<double> cvec;
+        struct StructOff {
+            size_t in_ptr, out_ptr, split, seq, in_ent, out_ent;
+            int n, num_edges;
+        };
+        std::vector<StructOff> soffs;
+        std::vector<size_t> damp_off(N, 0);
+        std::map<std::tuple<const void*, const void*, const void*, int, int>, int> struct_cache;
+        int64_t out_total = 0;
+        for (int ii : C.insts) {
+            RunSpec& r    = runs[size_t(ii)];
+            InstHost& I   = inst[size_t(ii)];
+            const auto* g = r.g;
+            inst_ids.push_back(ii);
+            for (int v = 0; v < g->num_nodes; ++v) {
+                const size_t node = size_t(I.node_base + v);
+                const double* sc  = g->node_scalars + size_t(v) * MGO_NUM_SCALARS;
+                NodeP& np         = nodep[node];
+                np.start_day      = sc[MGO_P_START_DAY];
+                np.seasonality    = sc[MGO_P_SEASONALITY];
+                np.ttc            = sc[MGO_P_TEST_AND_TRACE_CAPACITY];
+                np.ttc_hi         = sc[MGO_P_TEST_AND_TRACE_CAPACITY] * sc[MGO_P_TEST_AND_TRACE_MAX_RISK];
+                np.icu            = sc[MGO_P_ICU_CAPACITY];
+                np.nondetect      = sc[MGO_P_COMMUTER_NONDETECTION];
+                np.det_start      = sc[MGO_P_START_COMMUTER_DETECTION];
+                np.det_end        = sc[MGO_P_END_COMMUTER_DETECTION];
+                np.inst           = ii;
+                np.flags          = 0;
+                if (np.ttc < DBL_MAX) {
+                    np.flags |= NF_TTC_ACTIVE;
+                }
+                if (np.icu < DBL_MAX) {
+                    np.flags |= NF_ICU_ACTIVE;
+                }
+                if (np.det_start < np.det_end) {
+                    np.flags |= NF_DETECTION;
+                }
+                for (int a = 0; a < G; ++a) {
+                    lane_coefficients(g, v, a, &lcoef[(node * G + size_t(a)) * NUM_LC]);
+                    for (int j = 0; j < G; ++j) {
+                        const size_t k = (size_t(v) * G + size_t(a)) * G + size_t(j);
+                        const double b = g->contact_baseline[k];
+                        const double m = g->contact_minimum ? g->contact_minimum[k] : 0.0;
+                        cbase[(node * G + size_t(a)) * G + size_t(j)]  = b;
+                        cdelta[(node * G + size_t(a)) * G + size_t(j)] = b - m;
+                    }
+                }
+                std::copy(g->populations + size_t(v) * G * NC, g->populations + size_t(v + 1) * G * NC,
+                          y.begin() + long(node * G * NC));
+                tnode[node]  = r.cfg.t0;
+                dtnode[node] = r.cfg.dt_node;
+                // damping cache
+                std::vector<DampIn> dl;
+                if (g->damping_offsets) {
+                    for (int d = g->damping_offsets[v]; d < g->damping_offsets[v + 1]; ++d) {
+                        DampIn di;
+                        di.time  = g->damping_times[d];
+                        di.level = g->damping_levels ? g->damping_levels[d] : 0;
+                        di.type  = g->damping_types ? g->damping_types[d] : 0;
+                        di.m.assign(g->damping_coeffs + size_t(d) * G * G, g->damping_coeffs + size_t(d + 1) * G * G);
+                        dl.push_back(di);
+                    }
+                }
+                for (auto& d : r.extra_damping) {
+                    dl.push_back(d);
+                }
+                damp_off[node] = dtime.size();
+                np.damp_cnt    = 0;
+                np.damp_time   = nullptr;
+                np.damp_mat    = nullptr;
+                if (!dl.empty()) {
+                    std::vector<double> tt, mm;
+                    damping_cache(dl, G, tt, mm);
+                    np.flags |= NF_DAMPED;
+                    np.damp_cnt = int(tt.size());
+                    dtime.insert(dtime.end(), tt.begin(), tt.end());
+                    dmat.insert(dmat.end(), mm.begin(), mm.end());
+                }
+            }
+            // edge structure (shared between runs that pass the same edge arrays)
+            InstDev& D = idev[size_t(ii)];
+            D.estruct  = -1;
+            I.has_edges = false;
+            if (!r.single && g->num_edges > 0) {
+                auto key = std::make_tuple((const void*)g->edge_start, (const void*)g->edge_end,
+                                           (const void*)g->edge_coeffs, g->num_edges, g->num_nodes);
+                auto it  = struct_cache.find(key);
+                int sidx;
+                if (it != struct_cache.end()) {
+                    sidx = it->second;
+                }
+                else {
+                    EdgeStructHost es;
+                    build_edges(g, es, cvec);
+                    StructOff so{};
+                    so.n         = es.n;
+                    so.num_edges = es.num_edges;
+                    so.in_ptr    = ipool.size();
+                    ipool.insert(ipool.end(), es.in_ptr.begin(), es.in_ptr.end());
+                    so.out_ptr = ipool.size();
+                    ipool.insert(ipool.end(), es.out_ptr.begin(), es.out_ptr.end());
+                    so.split = ipool.size();
+                    ipool.insert(ipool.end(), es.split.begin(), es.split.end());
+                    so.seq = ipool.size();
+                    ipool.insert(ipool.end(), es.seq.begin(), es.seq.end());
+                    so.in_ent = epool.size();
+                    epool.insert(epool.end(), es.in_ent.begin(), es.in_ent.end());
+                    so.out_ent = epool.size();
+                    epool.insert(epool.end(), es.out_ent.begin(), es.out_ent.end());
+                    sidx = int(soffs.size());
+                    soffs.push_back(so);
+                    struct_cache[key] = sidx;
+                }
+                if (soffs[size_t(sidx)].num_edges > 0) {
+                    D.estruct   = sidx;
+                    I.has_edges = true;
+                }
+            }
+            D.t         = r.cfg.t0;
+            D.dt_graph  = I.dt_graph;
+            D.tmax      = r.cfg.t0;
+            D.abs_tol   = r.cfg.integrator.abs_tol;
+            D.rel_tol   = r.cfg.integrator.rel_tol;
+            D.dt_min    = r.cfg.integrator.dt_min;
+            D.dt_max    = r.cfg.integrator.dt_max;
+            D.node_base = I.node_base;
+            D.num_nodes = I.num_nodes;
+            D.day0      = I.day0;
+            D.ndays     = I.ndays;
+            D.exact_end = r.single ? 1 : 0;
+            I.out_base  = out_total;
+            out_total += int64_t(I.ndays) * I.num_nodes * G * NC;
+        }
+        // initial day: interpolate_simulation_result keeps t0 if t0 >= day0 (analyze_result.h:228-233)
+        std::vector<double> out0; // written on the device below
+        C.out_doubles = size_t(out_total);
+        auto up = [&](DevBuf& b, const auto& v) {
+            b.upload(v, stream);
+            device_bytes += b.bytes;
+        };
+        up(C.y, y);
+        up(C.y0, y);
+        C.tnode0  = tnode;
+        C.dtnode0 = dtnode;
+        C.snap.alloc(y.size() * sizeof(double));
+        C.ypre.alloc(y.size() * sizeof(double));
+        C.fac.alloc(N * G * sizeof(double));
+        C.rc.alloc(N * G * RC_STRIDE * sizeof(double));
+        device_bytes += C.snap.bytes + C.ypre.bytes + C.fac.bytes + C.rc.bytes;
+        up(C.lcoef, lcoef);
+        up(C.cbase, cbase);
+        up(C.cdelta, cdelta);
+        up(C.dtime, dtime);
+        up(C.dmat, dmat);
+        for (size_t node = 0; node < N; ++node) {
+            if (nodep[node].damp_cnt > 0) {
+                nodep[node].damp_time = C.dtime.as<double>() + damp_off[node];
+                nodep[node].damp_mat  = C.dmat.as<double>() + damp_off[node] * G * G;
+            }
+        }
+        up(C.nodep, nodep);
+        up(C.tnode, tnode);
+        up(C.dtnode, dtnode);
+        std::vector<uint32_t> zeros(N, 0);
+        up(C.steps, zeros);
+        up(C.attempts, zeros);
+        C.out.alloc(C.out_doubles * sizeof(double));
+        device_bytes += C.out.bytes;
+        hip_check(hipMemsetAsync(C.out.p, 0, C.out.bytes, stream), "memset out");
+        up(C.ipool, ipool);
+        up(C.epool, epool);
+        up(C.cvec, cvec);
+        std::vector<EdgeStructDev> estructs(soffs.size());
+        for (size_t k = 0; k < soffs.size(); ++k) {
+            const auto& so          = soffs[k];
+            EdgeStructDev& sd       = estructs[k];
+            const int32_t* ib       = C.ipool.as<int32_t>();
+            const EdgeEnt* eb       = C.epool.as<EdgeEnt>();
+            sd.in_ptr               = ib + so.in_ptr;
+            sd.out_ptr              = ib + so.out_ptr;
+            sd.split                = ib + so.split;
+            sd.seq                  = ib + so.seq;
+            sd.in_ent               = eb + so.in_ent;
+            sd.out_ent              = eb + so.out_ent;
+            sd.cvec                 = C.cvec.as<double>();
+            sd.n                    = so.n;
+            sd.num_edges            = so.num_edges;
+        }
+        up(C.estructs, estructs);
+        up(C.inst_ids, inst_ids);
+        for (int ii : C.insts) {
+            idev[size_t(ii)].out = C.out.as<double>() + inst[size_t(ii)].out_base;
+        }
+        // day-0 values
+        for (int ii : C.insts) {
+            InstHost& I = inst[size_t(ii)];
+            if (I.ndays > 0 && I.t0 >= double(I.day0) - 1e-10 && I.t0 <= double(I.day0)) {
+                const size_t nd = size_t(I.num_nodes) * G * NC;
+                hip_check(hipMemcpyAsync(C.out.as<double>() + I.out_base, C.y.as<double>() + size_t(I.node_base) * G * NC,
+                                         nd * sizeof(double), hipMemcpyDeviceToDevice, stream),
+                          "day0 copy");
+            }
+        }
+        ClassDev& d = C.dev;
+        d.G         = G;
+        d.kind      = C.kind;
+        d.num_nodes = C.num_nodes;
+        d.num_inst  = int(C.insts.size());
+        d.inst_ids  = C.inst_ids.as<int32_t>();
+        d.y         = C.y.as<double>();
+        d.snap      = C.snap.as<double>();
+        d.ypre      = C.ypre.as<double>();
+        d.fac       = C.fac.as<double>();
+        d.rc        = C.rc.as<double>();
+        d.lcoef     = C.lcoef.as<double>();
+        d.cbase     = C.cbase.as<double>();
+        d.cdelta    = C.cdelta.as<double>();
+        d.nodep     = C.nodep.as<NodeP>();
+        d.tnode     = C.tnode.as<double>();
+        d.dtnode    = C.dtnode.as<double>();
+        d.steps     = C.steps.as<uint32_t>();
+        d.attempts  = C.attempts.as<uint32_t>();
+        d.estructs  = C.estructs.as<EdgeStructDev>();
+    }
+
+    // enqueue one half-day cycle for every instance; with ev != nullptr, events bracket the node and the
+    // edge launches (ev[0..1] node, ev[2..3] edge)
+    void enqueue_cycle(hipEvent_t* ev = nullptr)
+    {
+        InstDev* id = inst_dev.as<InstDev>();
+        hip_check(launch_clock(id, int(inst.size()), stream), "clock kernel");
+        if (ev) {
+            hip_check(hipEventRecord(ev[0], stream), "event");
+        }
+        for (auto& C : classes) {
+            hip_check(launch_node(C.dev, id, stream), "node kernel");
+        }
+        if (ev) {
+            hip_check(hipEventRecord(ev[1], stream), "event");
+            hip_check(hipEventRecord(ev[2], stream), "event");
+        }
+        for (auto& C : classes) {
+            bool any_edges = false;
+            for (int ii : C.insts) {
+                any_edges = any_edges || inst[size_t(ii)].has_edges;
+            }
+            if (any_edges) {
+                hip_check(launch_edge(C.dev, id, stream), "edge kernel");
+            }
+        }
+        if (ev) {
+            hip_check(hipEventRecord(ev[3], stream), "event");
+        }
+    }
+
+    int edge_classes() const
+    {
+        int n = 0;
+        for (auto& C : classes) {
+            bool any_edges = false;
+            for (int ii : C.insts) {
+                any_edges = any_edges || inst[size_t(ii)].has_edges;
+            }
+            n += any_edges ? 1 : 0;
+        }
+        return n;
+    }
+
+    // number of GraphSimulation loop iterations from the host mirror of every clock to its tmax
+    int64_t cycles_needed(std::vector<double>* t_after = nullptr) const
+    {
+        int64_t mx = 0;
+        for (size_t i = 0; i < inst.size(); ++i) {
+            const InstHost& I = inst[i];
+            double t          = I.t;
+            int64_t n         = 0;
+            while (t < I.tmax) {
+                double step = I.dt_graph;
+                if (t + step > I.tmax) {
+                    step = I.tmax - t;
+                }
+                t = (I.single && step != I.dt_graph) ? I.tmax : t + step;
+                ++n;
+            }
+            mx = std::max(mx, n);
+            if (t_after) {
+                t_after->push_back(t);
+            }
+        }
+        return mx;
+    }
+
+    // advance every instance to its tmax (already set on host and device)
+    void run(double* elapsed_ms)
+    {
+        std::vector<double> t_after;
+        const int64_t n = cycles_needed(&t_after);
+        hipEvent_t e0, e1;
+        hip_check(hipEventCreate(&e0), "event");
+        hip_check(hipEventCreate(&e1), "event");
+        hip_check(hipEventRecord(e0, stream), "event record");
+        std::vector<hipEvent_t> evs;
+        node_ms = edge_ms = 0;
+        node_launches = edge_launches = 0;
+        if (n > 0 && profile) {
+            evs.resize(size_t(4 * n));
+            for (auto& e : evs) {
+                hip_check(hipEventCreate(&e), "event");
+            }
+            for (int64_t k = 0; k < n; ++k) {
+                enqueue_cycle(&evs[size_t(4 * k)]);
+            }
+        }
+        else if (n > 0) {
+            if (use_graph) {
+                if (!graph_exec) {
+                    hipGraph_t g;
+                    hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "begin capture");
+                    enqueue_cycle();
+                    hip_check(hipStreamEndCapture(stream, &g), "end capture");
+                    hip_check(hipGraphInstantiate(&graph_exec, g, nullptr, nullptr, 0), "graph instantiate");
+                    (void)hipGraphDestroy(g);
+                }
+                for (int64_t k = 0; k < n; ++k) {
+                    hip_check(hipGraphLaunch(graph_exec, stream), "graph launch");
+                }
+            }
+            else {
+                for (int64_t k = 0; k < n; ++k) {
+                    enqueue_cycle();
+                }
+            }
+        }
+        hip_check(hipEventRecord(e1, stream), "event record");
+        hip_check(hipEventSynchronize(e1), "run");
+        float ms = 0;
+        hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        for (size_t k = 0; k + 3 < evs.size(); k += 4) {
+            float a = 0, b = 0;
+            hip_check(hipEventElapsedTime(&a, evs[k], evs[k + 1]), "elapsed");
+            hip_check(hipEventElapsedTime(&b, evs[k + 2], evs[k + 3]), "elapsed");
+            node_ms += a;
+            edge_ms += b;
+        }
+        if (!evs.empty()) {
+            node_launches = int(n) * int(classes.size());
+            edge_launches = int(n) * edge_classes();
+        }
+        for (auto& e : evs) {
+            (void)hipEventDestroy(e);
+        }
+        last_ms = ms;
+        if (elapsed_ms) {
+            *elapsed_ms = ms;
+        }
+        cycles_run += n;
+        for (size_t i = 0; i < inst.size(); ++i) {
+            inst[i].t = t_after[i];
+        }
+    }
+
+    void reset()
+    {
+        for (auto& C : classes) {
+            hip_check(hipMemcpyAsync(C.y.p, C.y0.p, C.y.bytes, hipMemcpyDeviceToDevice, stream), "reset y");
+            hip_check(hipMemcpyAsync(C.tnode.p, C.tnode0.data(), C.tnode0.size() * sizeof(double),
+                                     hipMemcpyHostToDevice, stream),
+                      "reset t");
+            hip_check(hipMemcpyAsync(C.dtnode.p, C.dtnode0.data(), C.dtnode0.size() * sizeof(double),
+                                     hipMemcpyHostToDevice, stream),
+                      "reset dt");
+            hip_check(hipMemsetAsync(C.steps.p, 0, C.steps.bytes, stream), "reset steps");
+            hip_check(hipMemsetAsync(C.attempts.p, 0, C.attempts.bytes, stream), "reset attempts");
+        }
+        hip_check(hipMemcpyAsync(inst_dev.p, inst0.data(), inst0.size() * sizeof(InstDev), hipMemcpyHostToDevice,
+                                 stream),
+                  "reset instances");
+        hip_check(hipStreamSynchronize(stream), "reset");
+        for (size_t i = 0; i < inst.size(); ++i) {
+            inst[i].t = inst[i].t0;
+        }
+    }
+
+    void set_tmax(size_t i, double tmax)
+    {
+        inst[i].tmax = tmax;
+        InstDev* p   = inst_dev.as<InstDev>() + i;
+        hip_check(hipMemcpyAsync(reinterpret_cast<char*>(p) + offsetof(InstDev, tmax), &inst[i].tmax, sizeof(double),
+                                 hipMemcpyHostToDevice, stream),
+                  "set tmax");
+        hip_check(hipStreamSynchronize(stream), "set tmax");
+    }
+
+    std::vector<InstDev> read_inst()
+    {
+        std::vector<InstDev> v(inst.size());
+        hip_check(hipMemcpyAsync(v.data(), inst_dev.p, v.size() * sizeof(InstDev), hipMemcpyDeviceToHost, stream),
+                  "read instances");
+        hip_check(hipStreamSynchronize(stream), "read instances");
+        return v;
+    }
+
+    void check_status()
+    {
+        auto v = read_inst();
+        for (size_t i = 0; i < v.size(); ++i) {
+            if (v[i].status == MGO_ERR_UNSUPPORTED) {
+                fail(v[i].status, "run %zu: several pending traveler moves per edge (tmax off the dt grid) are not "
+                                  "supported by the device path",
+                     i);
+            }
+            if (v[i].status != 0) {
+                fail(v[i].status, "run %zu: a node made no progress within %u step attempts of one half-day", i,
+                     1u << 20);
+            }
+        }
+    }
+
+    // daily values of instance i that are complete: days up to the current time
+    int days_available(size_t i) const
+    {
+        const InstHost& I = inst[i];
+        if (I.ndays == 0) {
+            return 0;
+        }
+        const int last = std::min(day_floor(I.t), I.day0 + I.ndays - 1);
+        return std::max(0, last - I.day0 + 1);
+    }
+
+    void read_daily(size_t i, int ndays, double* out)
+    {
+        const InstHost& I = inst[i];
+        const ClassHost& C = classes[size_t(I.cls)];
+        const size_t nd    = size_t(ndays) * I.num_nodes * I.G * NC;
+        if (nd == 0) {
+            return;
+        }
+        hip_check(hipMemcpyAsync(out, C.out.as<double>() + I.out_base, nd * sizeof(double), hipMemcpyDeviceToHost,
+                                 stream),
+                  "read daily");
+        hip_check(hipStreamSynchronize(stream), "read daily");
+    }
+
+    void read_state(size_t i, double* out)
+    {
+        const InstHost& I  = inst[i];
+        const ClassHost& C = classes[size_t(I.cls)];
+        hip_check(hipMemcpyAsync(out, C.y.as<double>() + size_t(I.node_base) * I.G * NC,
+                                 size_t(I.num_nodes) * I.G * NC * sizeof(double), hipMemcpyDeviceToHost, stream),
+                  "read state");
+        hip_check(hipStreamSynchronize(stream), "read state");
+    }
+
+    void read_stats(size_t i, int64_t* steps, int64_t* attempts)
+    {
+        const InstHost& I  = inst[i];
+        const ClassHost& C = classes[size_t(I.cls)];
+        std::vector<uint32_t> s(size_t(I.num_nodes)), a(size_t(I.num_nodes));
+        hip_check(hipMemcpyAsync(s.data(), C.steps.as<uint32_t>() + I.node_base, s.size() * 4, hipMemcpyDeviceToHost,
+                                 stream),
+                  "read stats");
+        hip_check(hipMemcpyAsync(a.data(), C.attempts.as<uint32_t>() + I.node_base, a.size() * 4,
+                                 hipMemcpyDeviceToHost, stream),
+                  "read stats");
+        hip_check(hipStreamSynchronize(stream), "read stats");
+        for (size_t k = 0; k < s.size(); ++k) {
+            if (steps) {
+                steps[k] = s[k];
+            }
+            if (attempts) {
+                attempts[k] = a[k];
+            }
+        }
+    }
+
+    void totals(int64_t& steps, int64_t& attempts)
+    {
+        steps = attempts = 0;
+        for (auto& C : classes) {
+            std::vector<uint32_t> s(size_t(C.num_nodes)), a(size_t(C.num_nodes));
+            hip_check(hipMemcpyAsync(s.data(), C.steps.p, s.size() * 4, hipMemcpyDeviceToHost, stream), "stats");
+            hip_check(hipMemcpyAsync(a.data(), C.attempts.p, a.size() * 4, hipMemcpyDeviceToHost, stream), "stats");
+            hip_check(hipStreamSynchronize(stream), "stats");
+            for (auto x : s) {
+                steps += x;
+            }
+            for (auto x : a) {
+                attempts += x;
+            }
+        }
+    }
+};
+
+} // namespace
+
+// ------------------------------------------------------------------------------------------ context
+struct ServeState;
+
+struct mgo_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    size_t hbm_budget  = 0;
+    int flags          = 0;
+    std::recursive_mutex mu;
+    std::string last_error;
+    int64_t next_id = 1;
+    std::unordered_map<int64_t, std::unique_ptr<Batch>> graphs;
+    std::unordered_map<int64_t, std::unique_ptr<Batch>> batches;
+    std::chrono::steady_clock::time_point t_origin;
+    std::unique_ptr<ServeState> serve;
+};
+
+namespace
+{
+thread_local std::string tl_error;
+
+template <class F>
+mgo_status guarded(mgo_context* ctx, F&& f)
+{
+    if (!ctx) {
+        tl_error = "context is NULL";
+        return MGO_ERR_INVALID_VALUE;
+    }
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    try {
+        hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+        f();
+        return MGO_OK;
+    }
+    catch (const Error& e) {
+        ctx->last_error = e.msg;
+        return e.code;
+    }
+    catch (const std::bad_alloc&) {
+        ctx->last_error = "host allocation failed";
+        return MGO_ERR_OUT_OF_MEMORY;
+    }
+}
+
+Batch* find_batch(std::unordered_map<int64_t, std::unique_ptr<Batch>>& m, int64_t id)
+{
+    auto it = m.find(id);
+    if (it == m.end()) {
+        fail(MGO_ERR_NOT_FOUND, "unknown id %lld", (long long)id);
+    }
+    return it->second.get();
+}
+} // namespace
+
+#include "mgo_serve.inc"
+
+extern "C" {
+
+const char* mgo_version(void)
+{
+    return "mgo 0.1 (gfx950, fp64)";
+}
+
+mgo_status mgo_context_create(const mgo_device_cfg* cfg, mgo_context** out)
+{
+    if (!out) {
+        return MGO_ERR_INVALID_VALUE;
+    }
+    *out = nullptr;
+    auto ctx      = std::make_unique<mgo_context>();
+    ctx->device   = cfg ? cfg->device : 0;
+    ctx->flags    = cfg ? cfg->flags : 0;
+    ctx->t_origin = std::chrono::steady_clock::now();
+    int n         = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= ctx->device) {
+        tl_error = "no HIP device available";
+        return MGO_ERR_DEVICE;
+    }
+    if (hipSetDevice(ctx->device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        tl_error = "cannot create HIP stream";
+        return MGO_ERR_DEVICE;
+    }
+    size_t free_b = 0, total_b = 0;
+    (void)hipMemGetInfo(&free_b, &total_b);
+    ctx->hbm_budget = (cfg && cfg->hbm_budget_bytes) ? cfg->hbm_budget_bytes : size_t(double(free_b) * 0.9);
+    *out            = ctx.release();
+    return MGO_OK;
+}
+
+mgo_status mgo_context_destroy(mgo_context* ctx)
+{
+    if (!ctx) {
+        return MGO_ERR_INVALID_VALUE;
+    }
+    serve_shutdown(ctx);
+    {
+        std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+        (void)hipSetDevice(ctx->device);
+        ctx->graphs.clear();
+        ctx->batches.clear();
+        if (ctx->stream) {
+            (void)hipStreamSynchronize(ctx->stream);
+            (void)hipStreamDestroy(ctx->stream);
+        }
+    }
+    delete ctx;
+    return MGO_OK;
+}
+
+const char* mgo_last_error(const mgo_context* ctx)
+{
+    if (!ctx) {
+        return tl_error.c_str();
+    }
+    tl_error = ctx->last_error;
+    return tl_error.c_str();
+}
+
+double mgo_clock_ms(const mgo_context* ctx)
+{
+    if (!ctx) {
+        return 0.0;
+    }
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ctx->t_origin).count();
+}
+
+mgo_status mgo_graph_create(mgo_context* ctx, const mgo_graph_desc* graph, const mgo_sim_cfg* cfg, mgo_graph_id* out)
+{
+    return guarded(ctx, [&] {
+        if (!cfg || !out) {
+            fail(MGO_ERR_INVALID_VALUE, "cfg and out are required");
+        }
+        if (!(cfg->horizon >= cfg->t0)) {
+            fail(MGO_ERR_INVALID_VALUE, "cfg.horizon must be >= t0 (it sizes the daily output)");
+        }
+        std::vector<RunSpec> runs(1);
+        runs[0] = RunSpec{graph, *cfg, cfg->horizon, false, {}};
+        auto b  = std::make_unique<Batch>();
+        b->use_graph = !(ctx->flags & MGO_CTX_NO_GRAPH_CAPTURE);
+        b->profile   = (ctx->flags & MGO_CTX_PROFILE_KERNELS) != 0;
+        b->build(runs, ctx->stream);
+        *out = ctx->next_id++;
+        ctx->graphs[*out] = std::move(b);
+    });
+}
+
+mgo_status mgo_graph_advance(mgo_context* ctx, mgo_graph_id id, double tmax)
+{
+    return guarded(ctx, [&] {
+        Batch* b = find_batch(ctx->graphs, id);
+        if (!std::isfinite(tmax)) {
+            fail(MGO_ERR_INVALID_VALUE, "tmax must be finite");
+        }
+        if (day_floor(tmax) > b->inst[0].day0 + b->inst[0].ndays - 1 && b->inst[0].ndays >= 0) {
+            fail(MGO_ERR_OUT_OF_RANGE, "tmax %g beyond the horizon given at creation", tmax);
+        }
+        b->set_tmax(0, tmax);
+        b->run(nullptr);
+        b->check_status();
+    });
+}
+
+mgo_status mgo_graph_get_time(mgo_context* ctx, mgo_graph_id id, double* t)
+{
+    return guarded(ctx, [&] {
+        Batch* b = find_batch(ctx->graphs, id);
+        if (!t) {
+            fail(MGO_ERR_INVALID_VALUE, "t is NULL");
+        }
+        *t = b->inst[0].t;
+    });
+}
+
+mgo_status mgo_graph_read_state(mgo_context* ctx, mgo_graph_id id, double* y_out)
+{
+    return guarded(ctx, [&] {
+        Batch* b = find_batch(ctx->graphs, id);
+        if (!y_out) {
+            fail(MGO_ERR_INVALID_VALUE, "y_out is NULL");
+        }
+        b->read_state(0, y_out);
+    });
+}
+
+mgo_status mgo_graph_read_daily(mgo_context* ctx, mgo_graph_id id, int32_t capacity_days, double* out, int32_t* num_days)
+{
+    return guarded(ctx, [&] {
+        Batch* b  = find_batch(ctx->graphs, id);
+        const int n = std::min<int>(b->days_available(0), capacity_days);
+        if (out && n > 0) {
+            b->read_daily(0, n, out);
+        }
+        if (num_days) {
+            *num_days = b->days_available(0);
+        }
+    });
+}
+
+mgo_status mgo_graph_read_stats(mgo_context* ctx, mgo_graph_id id, int64_t* steps, int64_t* attempts)
+{
+    return guarded(ctx, [&] {
+        Batch* b = find_batch(ctx->graphs, id);
+        b->read_stats(0, steps, attempts);
+    });
+}
+
+mgo_status mgo_graph_destroy(mgo_context* ctx, mgo_graph_id id)
+{
+    return guarded(ctx, [&] {
+        find_batch(ctx->graphs, id);
+        ctx->graphs.erase(id);
+    });
+}
+
+mgo_status mgo_simulate_single(mgo_context* ctx, const mgo_graph_desc* models, const mgo_sim_cfg* cfg, double tmax,
+                               int32_t capacity_days, double* daily_out, int32_t* num_days)
+{
+    return guarded(ctx, [&] {
+        if (!cfg || !models) {
+            fail(MGO_ERR_INVALID_VALUE, "models and cfg are required");
+        }
+        if (models->num_edges != 0) {
+            fail(MGO_ERR_INVALID_VALUE, "mgo_simulate_single takes independent models (num_edges == 0)");
+        }
+        if (!(tmax > cfg->t0)) {
+            fail(MGO_ERR_INVALID_VALUE, "tmax must be > t0");
+        }
+        std::vector<RunSpec> runs(1);
+        runs[0] = RunSpec{models, *cfg, tmax, true, {}};
+        Batch b;
+        b.use_graph = false;
+        b.build(runs, ctx->stream);
+        b.set_tmax(0, tmax);
+        b.run(nullptr);
+        b.check_status();
+        const int n = std::min<int>(b.days_available(0), capacity_days);
+        if (daily_out && n > 0) {
+            b.read_daily(0, n, daily_out);
+        }
+        if (num_days) {
+            *num_days = b.days_available(0);
+        }
+    });
+}
+
+static void build_ensemble(mgo_context* ctx, const mgo_ensemble_desc* desc, Batch& b)
+{
+    if (!desc || desc->num_runs < 1 || !desc->graphs || !desc->cfgs || !desc->tmax) {
+        fail(MGO_ERR_INVALID_VALUE, "ensemble description incomplete");
+    }
+    std::vector<RunSpec> runs(size_t(desc->num_runs));
+    for (int r = 0; r < desc->num_runs; ++r) {
+        if (!std::isfinite(desc->tmax[r]) || !(desc->tmax[r] >= desc->cfgs[r].t0)) {
+            fail(MGO_ERR_INVALID_VALUE, "run %d: tmax must be finite and >= t0", r);
+        }
+        runs[size_t(r)] = RunSpec{&desc->graphs[r], desc->cfgs[r], desc->tmax[r], false, {}};
+    }
+    b.use_graph = !(ctx->flags & MGO_CTX_NO_GRAPH_CAPTURE);
+    b.profile   = (ctx->flags & MGO_CTX_PROFILE_KERNELS) != 0;
+    b.build(runs, ctx->stream);
+    for (int r = 0; r < desc->num_runs; ++r) {
+        b.inst[size_t(r)].tmax = desc->tmax[r];
+    }
+    // write all tmax values at once
+    std::vector<InstDev> v = b.read_inst();
+    for (int r = 0; r < desc->num_runs; ++r) {
+        v[size_t(r)].tmax = desc->tmax[r];
+    }
+    hip_check(hipMemcpyAsync(b.inst_dev.p, v.data(), v.size() * sizeof(InstDev), hipMemcpyHostToDevice, ctx->stream),
+              "tmax upload");
+    hip_check(hipStreamSynchronize(ctx->stream), "tmax upload");
+    b.inst0 = v;
+}
+
+static void fill_info(Batch& b, mgo_run_info* info, double ms, int64_t cycles)
+{
+    if (!info) {
+        return;
+    }
+    std::memset(info, 0, sizeof(*info));
+    info->device_ms = ms;
+    double nd       = 0;
+    for (auto& I : b.inst) {
+        nd += double(I.num_nodes) * (I.tmax - I.t0);
+    }
+    info->node_days = nd;
+    b.totals(info->steps, info->attempts);
+    auto v = b.read_inst();
+    for (auto& I : v) {
+        info->exact_replays += I.replays;
+    }
+    info->half_day_cycles = int32_t(cycles);
+    info->node_launches   = b.node_launches;
+    info->node_kernel_ms  = b.node_ms;
+    info->edge_launches   = b.edge_launches;
+    info->edge_kernel_ms  = b.edge_ms;
+}
+
+static void read_all_daily(Batch& b, double* out, size_t first = 0, size_t count = SIZE_MAX)
+{
+    size_t off = 0;
+    for (size_t i = first; i < b.inst.size() && i - first < count; ++i) {
+        const auto& I = b.inst[i];
+        b.read_daily(i, I.ndays, out + off);
+        off += size_t(I.ndays) * I.num_nodes * I.G * NC;
+    }
+}
+
+mgo_status mgo_ensemble_run(mgo_context* ctx, const mgo_ensemble_desc* desc, double* daily_out, mgo_run_info* info)
+{
+    return guarded(ctx, [&] {
+        Batch b;
+        build_ensemble(ctx, desc, b);
+        const int64_t cyc = b.cycles_needed();
+        double ms         = 0;
+        b.run(&ms);
+        b.check_status();
+        if (daily_out) {
+            read_all_daily(b, daily_out);
+        }
+        fill_info(b, info, ms, cyc);
+    });
+}
+
+mgo_status mgo_batch_create(mgo_context* ctx, const mgo_ensemble_desc* desc, mgo_batch_id* out)
+{
+    return guarded(ctx, [&] {
+        if (!out) {
+            fail(MGO_ERR_INVALID_VALUE, "out is NULL");
+        }
+        auto b = std::make_unique<Batch>();
+        build_ensemble(ctx, desc, *b);
+        *out = ctx->next_id++;
+        ctx->batches[*out] = std::move(b);
+    });
+}
+
+mgo_status mgo_batch_run(mgo_context* ctx, mgo_batch_id id, mgo_run_info* info)
+{
+    return guarded(ctx, [&] {
+        Batch* b          = find_batch(ctx->batches, id);
+        const int64_t cyc = b->cycles_needed();
+        double ms         = 0;
+        b->run(&ms);
+        b->check_status();
+        fill_info(*b, info, ms, cyc);
+    });
+}
+
+mgo_status mgo_batch_read_daily(mgo_context* ctx, mgo_batch_id id, int32_t first_run, int32_t num_runs,
+                                double* daily_out)
+{
+    return guarded(ctx, [&] {
+        Batch* b = find_batch(ctx->batches, id);
+        if (!daily_out) {
+            fail(MGO_ERR_INVALID_VALUE, "daily_out is NULL");
+        }
+        if (first_run < 0 || num_runs < 0 || size_t(first_run) + size_t(num_runs) > b->inst.size()) {
+            fail(MGO_ERR_OUT_OF_RANGE, "runs %d..%d out of range", first_run, first_run + num_runs);
+        }
+        read_all_daily(*b, daily_out, size_t(first_run), size_t(num_runs));
+    });
+}
+
+mgo_status mgo_batch_reset(mgo_context* ctx, mgo_batch_id id)
+{
+    return guarded(ctx, [&] {
+        Batch* b = find_batch(ctx->batches, id);
+        b->reset();
+    });
+}
+
+mgo_status mgo_batch_destroy(mgo_context* ctx, mgo_batch_id id)
+{
+    return guarded(ctx, [&] {
+        find_batch(ctx->batches, id);
+        ctx->batches.erase(id);
+    });
+}
+
+} // extern "C"

@@ -1,0 +1,167 @@
+// mgo_internal.h - data layout shared by the host engine (mgo_host.cpp) and the HIP kernels
+// (mgo_kernels.hip). Not part of the public ABI.
+//
+// Layout in HBM (one "class" = all nodes of the resident instances that share the number of age
+// groups G and the integrator kind; kernels are specialised per class):
+//   lane slot   = node * G + g   (one GPU lane per (node, age group); 64/G nodes per wavefront)
+//   y, snap, ypre       [node][g][10]          state, departure snapshot, pre-return state
+//   lcoef               [node][g][NUM_LC]      per-lane rate coefficients (see LaneCoef)
+//   cbase, cdelta       [node][g][G]           contact row B_g., B_g. - M_g.
+//   fac                 [node][g]              ISy mobility factor of the departure snapshot
+//   rc                  [node][g][RC_STRIDE]   return coefficients of the node as a destination, frozen at
+//                                              the departure time (see RetCoef)
+//   nodep               [node]                 scalar parameters + instance index
+//   tnode, dtnode       [node]                 integrator time and carried step size
+// Instances (one graph simulation each) live in a device table; a clock kernel turns the instance
+// clock into a per-cycle record read by the node and edge kernels.
+#pragma once
+
+#include <cstdint>
+
+namespace mgo
+{
+
+constexpr int NC    = 10; // compartments per age group
+constexpr int WAVE  = 64;
+constexpr int BLOCK = 256;
+
+enum Compartment : int { S = 0, E, INS, INSC, ISY, ISYC, ISEV, ICR, R, D };
+
+// Per-lane coefficients. Each is the exact subexpression the reference multiplies the compartment
+// with (cpp/models/ode_secir/model.h:154-212), so coef * y rounds as the reference's expression does.
+enum LaneCoef : int {
+    LC_E = 0,      // (1.0 / T_E)
+    LC_INS_ISY,    // (1 - R) * (1 / T_INS)          (also INSC -> ISyC)
+    LC_INS_R,      // R * (1.0 / T_INS)              (also INSC -> R)
+    LC_ISY_ISEV,   // sev / T_ISy                    (also ISyC -> ISev)
+    LC_ISY_R,      // (1 - sev) / T_ISy              (also ISyC -> R)
+    LC_ISEV_ICR,   // cps / T_ISev                   (ICU capacity inactive)
+    LC_ISEV_R,     // (1.0 - cps - dps) / T_ISev
+    LC_ISEV_D,     // (dps + (cps - cps)) / T_ISev   (ICU capacity inactive)
+    LC_ICR_D,      // dpc / T_ICr
+    LC_ICR_R,      // (1.0 - dpc) / T_ICr
+    LC_TP,         // TransmissionProbabilityOnContact
+    LC_REL,        // RelativeTransmissionNoSymptoms
+    LC_RISK,       // RiskOfInfectionFromSymptomatic
+    LC_MAXRISK,    // MaxRiskOfInfectionFromSymptomatic
+    LC_TTR,        // (1 - R) / T_INS   (test-and-trace demand per INS)
+    LC_CPS,        // CriticalPerSevere
+    LC_DPS,        // DeathsPerSevere
+    LC_TISEV,      // TimeInfectedSevere
+    NUM_LC
+};
+
+// Return coefficients of a destination lane, computed from its post-departure state (the `pop` of
+// calculate_mobility_returns, metapopulation_mobility_instant.h:460-471) at the departure time.
+enum RetCoef : int {
+    RC_KAPPA = 0, // season(t_dep) * tp * sum_j C_gj(t_dep) w_j(pop): S->E rate per traveler in S
+    RC_E,
+    RC_INS_ISY,
+    RC_INS_R,
+    RC_ISY_ISEV,
+    RC_ISY_R,
+    RC_ISEV_ICR,  // ICU-adjusted at the departure state
+    RC_ISEV_R,
+    RC_ISEV_D,    // ICU-adjusted at the departure state
+    RC_ICR_D,
+    RC_ICR_R,
+    RC_STRIDE = 12
+};
+
+// node flags
+enum : int32_t {
+    NF_TTC_ACTIVE  = 1, // TestAndTraceCapacity < DBL_MAX: the risk smoother depends on the state
+    NF_ICU_ACTIVE  = 2, // ICUCapacity < DBL_MAX: the critical-care smoother depends on the state
+    NF_DAMPED      = 4, // contact matrix has dampings
+    NF_DETECTION   = 8, // commuter detection interval non-empty
+};
+
+struct NodeP {
+    double start_day;
+    double seasonality;
+    double ttc;       // TestAndTraceCapacity
+    double ttc_hi;    // TestAndTraceCapacity * TestAndTraceCapacityMaxRisk
+    double icu;       // ICUCapacity
+    double nondetect; // commuter nondetection
+    double det_start;
+    double det_end;
+    const double* damp_time; // damping cache: entry times (sentinels lowest()/max() included)
+    const double* damp_mat;  // damping cache: [entry][G][G]
+    int32_t flags;
+    int32_t inst;            // instance index
+    int32_t damp_cnt;        // number of damping-cache entries
+    int32_t pad;
+};
+
+// per-instance state; the c_* fields are the record of the current half-day cycle
+struct InstDev {
+    double t;         // GraphSimulation::get_t()
+    double dt_graph;
+    double tmax;
+    double t_dep;     // pending departure time
+    double ret_time;  // pending return time
+    double abs_tol, rel_tol, dt_min, dt_max;
+    double c_t0, c_t1, c_step, c_tdep;
+    double* out;      // daily values [day][node][G][10]
+    int32_t node_base;
+    int32_t num_nodes;
+    int32_t estruct;  // edge structure index, -1 = no edges
+    int32_t pending;
+    int32_t toggle;   // MobilityEdge::m_return_mobile_population (identical on all edges)
+    int32_t status;
+    int32_t day0;
+    int32_t ndays;
+    int32_t c_active;
+    int32_t c_action; // 0 none, 1 departure, 2 return
+    int32_t c_out_day;
+    int32_t c_replay; // set by the return kernel if the negative-value fix-up is needed
+    int32_t replays;  // number of exact replays so far
+    int32_t exact_end; // single-model run: the last step ends exactly at tmax
+};
+
+// CSR edge structure of one graph (node indices local to the instance). In-edges of node v are sorted
+// by source; in_split[v] = number of in-edges whose source < v, so walking
+//   in[0, split) , out[all] , in[split, end)
+// visits v's edges in the reference's global (start, end) order (graph.h:231-241).
+// one CSR entry: neighbour, coefficient (uniform) or index of its coefficient vector
+struct EdgeEnt {
+    double cval;      // coefficient if the edge's vector is uniform
+    int32_t nbr;      // neighbour (local index)
+    int32_t cidx;     // -1 uniform, else index of the [10*G] vector in cvec
+};
+
+struct EdgeStructDev {
+    const int32_t* in_ptr;   // [n+1]
+    const int32_t* out_ptr;  // [n+1]
+    const int32_t* split;    // [n]
+    const int32_t* seq;      // [num_edges][3] = (start, end, out-entry) in global edge order, exact replay
+    const EdgeEnt* in_ent;
+    const EdgeEnt* out_ent;
+    const double* cvec;      // non-uniform coefficient vectors [k][10*G]
+    int32_t n;
+    int32_t num_edges;
+};
+
+struct ClassDev {
+    int32_t G;
+    int32_t kind;
+    int32_t num_nodes;
+    int32_t num_inst;
+    const int32_t* inst_ids; // instances of this class (global instance indices)
+    double* y;
+    double* snap;
+    double* ypre;
+    double* fac;
+    double* rc;
+    const double* lcoef;
+    const double* cbase;
+    const double* cdelta;
+    const NodeP* nodep;
+    double* tnode;
+    double* dtnode;
+    uint32_t* steps;
+    uint32_t* attempts;
+    const EdgeStructDev* estructs; // indexed by InstDev::estruct
+};
+
+} // namespace mgo

@@ -1,0 +1,1022 @@
+// mgo_kernels.hip - gfx950 kernels of the Graph-ODE engine (FP64 throughout).
+//
+// One lane per (node, age group): a wavefront holds 64/G nodes; a node's 10 compartments of one age
+// group live in the lane's registers. Cross-age-group terms (force of infection, test-and-trace demand,
+// ICU occupancy, the step's error norm) are group reductions over the node's G consecutive lanes with
+// ds_bpermute shuffles; the G x G contact row of the lane stays in registers. There is no dense
+// contraction, so everything runs on the FP64 vector ALUs.
+//
+// Kernels per half-day cycle (GraphSimulation::advance, cpp/memilio/mobility/graph_simulation.h:101-120):
+//   clock_kernel       per instance: step size, edge action and output day of this cycle
+//   node_phase<G,K>    per node lane: SystemIntegrator::advance to t+dt with integrator K
+//   edge_phase<G>      per node lane: the node's departures or returns of all incident edges, walked in
+//                      the reference's edge order (deterministic segmented sum, no atomics)
+//   replay_phase<G>    per flagged instance: sequential return pass with the negative-value fix-up
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstdint>
+
+#include "../../include/mgo.h"
+#include "mgo_internal.h"
+
+namespace mgo
+{
+
+#define MGO_PI 3.14159265358979323846
+
+// cpp/memilio/math/smoother.h:46-60
+__device__ __forceinline__ double smoother_cosine(double x, double xl, double xr, double yl, double yr)
+{
+    if (x <= xl) {
+        return yl;
+    }
+    if (x >= xr) {
+        return yr;
+    }
+    return 0.5 * (yl - yr) * cos(MGO_PI / (xr - xl) * (x - xl)) + 0.5 * (yl + yr);
+}
+
+// Blocks are dispatched round-robin over the 8 XCDs; map the hardware block id so that each XCD gets a
+// contiguous range of nodes (instances stay within one XCD's L2 for the edge gathers). The host pads
+// the grid to a multiple of 8.
+__device__ __forceinline__ int xcd_block()
+{
+    const int per = int(gridDim.x) >> 3;
+    const int b   = int(blockIdx.x);
+    return (b & 7) * per + (b >> 3);
+}
+
+template <int G>
+__device__ __forceinline__ double group_sum(double v, int base)
+{
+    double s = __shfl(v, base);
+#pragma unroll
+    for (int j = 1; j < G; ++j) {
+        s = s + __shfl(v, base + j);
+    }
+    return s;
+}
+
+template <int G>
+__device__ __forceinline__ double group_max(double v, int base)
+{
+    double s = __shfl(v, base);
+#pragma unroll
+    for (int j = 1; j < G; ++j) {
+        s = fmax(s, __shfl(v, base + j));
+    }
+    return s;
+}
+
+template <int G>
+__device__ __forceinline__ double group_min(double v, int base)
+{
+    double s = __shfl(v, base);
+#pragma unroll
+    for (int j = 1; j < G; ++j) {
+        s = fmin(s, __shfl(v, base + j));
+    }
+    return s;
+}
+
+template <int G>
+__device__ __forceinline__ bool group_all(bool v, int base)
+{
+    int s = __shfl(int(v), base);
+#pragma unroll
+    for (int j = 1; j < G; ++j) {
+        s &= __shfl(int(v), base + j);
+    }
+    return s != 0;
+}
+
+// Per-lane context of one node lane, loaded once per kernel.
+template <int G>
+struct Lane {
+    double c[NUM_LC];
+    double crow[G];   // contact row at the last evaluated time (constant without dampings)
+    NodeP np;
+    int node;
+    int age;
+    int base;         // first lane of the node's group
+};
+
+// Effective contact row C(t)_age,. = B - D(t) o (B - M) (contact_matrix.h:190-194, damping.h:381-396)
+template <int G>
+__device__ __forceinline__ void contact_row(const ClassDev& c, Lane<G>& L, double t)
+{
+    const double* B = c.cbase + (size_t(L.node) * G + L.age) * G;
+    if (!(L.np.flags & NF_DAMPED)) {
+#pragma unroll
+        for (int j = 0; j < G; ++j) {
+            L.crow[j] = B[j];
+        }
+        return;
+    }
+    const double* tk = L.np.damp_time;
+    int k            = 1;
+    while (k < L.np.damp_cnt - 1 && !(tk[k] > t)) {
+        ++k;
+    }
+    const double xr    = tk[k];
+    const double xl    = xr - 1.0;
+    const double* Dl   = L.np.damp_mat + (size_t(k - 1) * G + L.age) * G;
+    const double* Dr   = L.np.damp_mat + (size_t(k) * G + L.age) * G;
+    const double* Dd   = c.cdelta + (size_t(L.node) * G + L.age) * G;
+    const bool left    = t <= xl;
+    const bool right   = t >= xr;
+    const double cs    = (left || right) ? 0.0 : cos(MGO_PI / (xr - xl) * (t - xl));
+#pragma unroll
+    for (int j = 0; j < G; ++j) {
+        const double d = left ? Dl[j] : (right ? Dr[j] : 0.5 * (Dl[j] - Dr[j]) * cs + 0.5 * (Dl[j] + Dr[j]));
+        L.crow[j]      = B[j] - d * Dd[j];
+    }
+}
+
+// Force-of-infection coefficient of the lane's age group: season(t) * tp * sum_j C_gj(t) w_j(pop),
+// w_j = divN_j * (rel_j pop[INS_j] + risk_j pop[ISy_j]) (model.h:99-145, factored over j). Also returns
+// the ICU-adjusted ISev->ICr / ISev->D coefficients (model.h:147-152, 196-204).
+template <int G>
+__device__ __forceinline__ double force_of_infection(const ClassDev& c, Lane<G>& L, const double (&pop)[NC], double t,
+                                                     double& c_icr, double& c_d)
+{
+    const double season = 1.0 + L.np.seasonality * sin(MGO_PI * ((L.np.start_day + t) / 182.5 + 0.5));
+    double risk         = L.c[LC_RISK];
+    if (L.np.flags & NF_TTC_ACTIVE) {
+        const double ttr = group_sum<G>(L.c[LC_TTR] * pop[INS], L.base);
+        risk             = smoother_cosine(ttr, L.np.ttc, L.np.ttc_hi, L.c[LC_RISK], L.c[LC_MAXRISK]);
+    }
+    const double Nj   = pop[S] + pop[E] + pop[INS] + pop[ISY] + pop[ISEV] + pop[ICR] + pop[R];
+    const double divN = (Nj < 1e-12) ? 0.0 : 1.0 / Nj;
+    const double w    = divN * (L.c[LC_REL] * pop[INS] + risk * pop[ISY]);
+    if (L.np.flags & NF_DAMPED) {
+        contact_row<G>(c, L, t);
+    }
+    double lam = 0.0;
+#pragma unroll
+    for (int j = 0; j < G; ++j) {
+        lam += L.crow[j] * __shfl(w, L.base + j);
+    }
+    c_icr = L.c[LC_ISEV_ICR];
+    c_d   = L.c[LC_ISEV_D];
+    if (L.np.flags & NF_ICU_ACTIVE) {
+        const double icu  = group_sum<G>(pop[ICR], L.base);
+        const double crit = smoother_cosine(icu, 0.90 * L.np.icu, L.np.icu, L.c[LC_CPS], 0.0);
+        c_icr             = crit / L.c[LC_TISEV];
+        c_d               = (L.c[LC_DPS] + (L.c[LC_CPS] - crit)) / L.c[LC_TISEV];
+    }
+    return season * L.c[LC_TP] * lam;
+}
+
+// flows of one age group -> derivative (flow_model.h:112-125,219-235 with the osecir Flows order)
+__device__ __forceinline__ void flows_to_dydt(const double (&y)[NC], double kappa, double cE, double cINS_ISY,
+                                              double cINS_R, double cISY_ISEV, double cISY_R, double cISEV_ICR,
+                                              double cISEV_R, double cISEV_D, double cICR_D, double cICR_R,
+                                              double (&dy)[NC])
+{
+    const double f0  = y[S] * kappa;
+    const double f1  = cE * y[E];
+    const double f2  = cINS_ISY * y[INS];
+    const double f3  = cINS_R * y[INS];
+    const double f4  = cINS_ISY * y[INSC];
+    const double f5  = cINS_R * y[INSC];
+    const double f6  = cISY_ISEV * y[ISY];
+    const double f7  = cISY_R * y[ISY];
+    const double f8  = cISY_ISEV * y[ISYC];
+    const double f9  = cISY_R * y[ISYC];
+    const double f10 = cISEV_ICR * y[ISEV];
+    const double f11 = cISEV_R * y[ISEV];
+    const double f12 = cISEV_D * y[ISEV];
+    const double f13 = cICR_D * y[ICR];
+    const double f14 = cICR_R * y[ICR];
+    dy[S]    = -f0;
+    dy[E]    = f0 - f1;
+    dy[INS]  = (f1 - f2) - f3;
+    dy[INSC] = -f4 - f5;
+    dy[ISY]  = (f2 - f6) - f7;
+    dy[ISYC] = (f4 - f8) - f9;
+    dy[ISEV] = (((f6 + f8) - f10) - f11) - f12;
+    dy[ICR]  = (f10 - f13) - f14;
+    dy[R]    = ((((f3 + f5) + f7) + f9) + f11) + f14;
+    dy[D]    = f12 + f13;
+}
+
+// Right-hand side of the node ODE, pop == y (osecir::Model::get_flows + FlowModel::get_derivatives)
+template <int G>
+__device__ __forceinline__ void rhs(const ClassDev& c, Lane<G>& L, const double (&y)[NC], double t, double (&dy)[NC])
+{
+    double c_icr, c_d;
+    const double kappa = force_of_infection<G>(c, L, y, t, c_icr, c_d);
+    flows_to_dydt(y, kappa, L.c[LC_E], L.c[LC_INS_ISY], L.c[LC_INS_R], L.c[LC_ISY_ISEV], L.c[LC_ISY_R], c_icr,
+                  L.c[LC_ISEV_R], c_d, L.c[LC_ICR_D], L.c[LC_ICR_R], dy);
+}
+
+// ------------------------------------------------------------------------------- Cash-Karp 5(4)
+// Cash & Karp, ACM TOMS 16 (1990) 201-222; the same ratios odeint's runge_kutta_cash_karp54 uses.
+namespace ck
+{
+constexpr double C2 = 1.0 / 5.0, C3 = 3.0 / 10.0, C4 = 3.0 / 5.0, C5 = 1.0, C6 = 7.0 / 8.0;
+constexpr double A21 = 1.0 / 5.0;
+constexpr double A31 = 3.0 / 40.0, A32 = 9.0 / 40.0;
+constexpr double A41 = 3.0 / 10.0, A42 = -9.0 / 10.0, A43 = 6.0 / 5.0;
+constexpr double A51 = -11.0 / 54.0, A52 = 5.0 / 2.0, A53 = -70.0 / 27.0, A54 = 35.0 / 27.0;
+constexpr double A61 = 1631.0 / 55296.0, A62 = 175.0 / 512.0, A63 = 575.0 / 13824.0, A64 = 44275.0 / 110592.0,
+                 A65 = 253.0 / 4096.0;
+constexpr double B1 = 37.0 / 378.0, B3 = 250.0 / 621.0, B4 = 125.0 / 594.0, B6 = 512.0 / 1771.0;
+constexpr double E1 = 37.0 / 378.0 - 2825.0 / 27648.0, E3 = 250.0 / 621.0 - 18575.0 / 48384.0,
+                 E4 = 125.0 / 594.0 - 13525.0 / 55296.0, E5 = -277.0 / 14336.0, E6 = 512.0 / 1771.0 - 1.0 / 4.0;
+} // namespace ck
+
+// One Cash-Karp step from (y, k1 = f(y, t)) with step h: 5th-order solution in yo, embedded error in er.
+template <int G>
+__device__ __forceinline__ void cash_karp_stages(const ClassDev& c, Lane<G>& L, const double (&y)[NC],
+                                                 const double (&k1)[NC], double t, double h, double (&yo)[NC],
+                                                 double (&er)[NC], bool want_err)
+{
+    double k2[NC], k3[NC], k4[NC], k5[NC], yt[NC];
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+        yt[i] = y[i] + (ck::A21 * h) * k1[i];
+    }
+    rhs<G>(c, L, yt, t + ck::C2 * h, k2);
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+        yt[i] = y[i] + (ck::A31 * h) * k1[i] + (ck::A32 * h) * k2[i];
+    }
+    rhs<G>(c, L, yt, t + ck::C3 * h, k3);
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+        yt[i] = y[i] + (ck::A41 * h) * k1[i] + (ck::A42 * h) * k2[i] + (ck::A43 * h) * k3[i];
+    }
+    rhs<G>(c, L, yt, t + ck::C4 * h, k4);
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+        yt[i] = y[i] + (ck::A51 * h) * k1[i] + (ck::A52 * h) * k2[i] + (ck::A53 * h) * k3[i] + (ck::A54 * h) * k4[i];
+    }
+    rhs<G>(c, L, yt, t + ck::C5 * h, k5);
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+        yt[i] = y[i] + (ck::A61 * h) * k1[i] + (ck::A62 * h) * k2[i] + (ck::A63 * h) * k3[i] + (ck::A64 * h) * k4[i] +
+                (ck::A65 * h) * k5[i];
+    }
+    double k6[NC];
+    rhs<G>(c, L, yt, t + ck::C6 * h, k6);
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+        yo[i] = y[i] + (ck::B1 * h) * k1[i] + (ck::B3 * h) * k3[i] + (ck::B4 * h) * k4[i] + (ck::B6 * h) * k6[i];
+    }
+    if (want_err) {
+#pragma unroll
+        for (int i = 0; i < NC; ++i) {
+            er[i] = (ck::E1 * h) * k1[i] + (ck::E3 * h) * k3[i] + (ck::E4 * h) * k4[i] + (ck::E5 * h) * k5[i] +
+                    (ck::E6 * h) * k6[i];
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------- RKF45 (RKIntegratorCore)
+// adapt_rk.h:71-137 default tableau
+namespace rkf
+{
+constexpr double C2 = 0.25, C3 = 3 / 8.0, C4 = 12 / 13.0, C5 = 1.0, C6 = 0.5;
+constexpr double A21 = 0.25;
+constexpr double A31 = 3 / 32.0, A32 = 9 / 32.0;
+constexpr double A41 = 1932 / 2197.0, A42 = -7200 / 2197.0, A43 = 7296 / 2197.0;
+constexpr double A51 = 439 / 216.0, A52 = -8.0, A53 = 3680 / 513.0, A54 = -845 / 4104.0;
+constexpr double A61 = -8 / 27.0, A62 = 2.0, A63 = -3544 / 2565.0, A64 = 1859 / 4104.0, A65 = -11 / 40.0;
+constexpr double BL1 = 25 / 216.0, BL3 = 1408 / 2565.0, BL4 = 2197 / 4104.0, BL5 = -0.2;
+constexpr double BH1 = 16 / 135.0, BH3 = 6656 / 12825.0, BH4 = 28561 / 56430.0, BH5 = -9 / 50.0, BH6 = 2 / 55.0;
+constexpr double D1 = BH1 - BL1, D3 = BH3 - BL3, D4 = BH4 - BL4, D5 = BH5 - BL5, D6 = BH6 - 0.0;
+} // namespace rkf
+
+template <int G>
+__device__ __forceinline__ void rkf45_stages(const ClassDev& c, Lane<G>& L, const double (&y)[NC],
+                                             const double (&k1)[NC], double t, double h, double (&yo)[NC],
+                                             double (&er)[NC])
+{
+    double k2[NC], k3[NC], k4[NC], k5[NC], k6[NC], yt[NC];
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+        yt[i] = y[i] + (h * rkf::A21) * k1[i];
+    }
+    rhs<G>(c, L, yt, t + rkf::C2 * h, k2);
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+        yt[i] = y[i] + (h * rkf::A31) * k1[i] + (h * rkf::A32) * k2[i];
+    }
+    rhs<G>(c, L, yt, t + rkf::C3 * h, k3);
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+        yt[i] = y[i] + (h * rkf::A41) * k1[i] + (h * rkf::A42) * k2[i] + (h * rkf::A43) * k3[i];
+    }
+    rhs<G>(c, L, yt, t + rkf::C4 * h, k4);
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+        yt[i] = y[i] + (h * rkf::A51) * k1[i] + (h * rkf::A52) * k2[i] + (h * rkf::A53) * k3[i] + (h * rkf::A54) * k4[i];
+    }
+    rhs<G>(c, L, yt, t + rkf::C5 * h, k5);
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+        yt[i] = y[i] + (h * rkf::A61) * k1[i] + (h * rkf::A62) * k2[i] + (h * rkf::A63) * k3[i] +
+                (h * rkf::A64) * k4[i] + (h * rkf::A65) * k5[i];
+    }
+    rhs<G>(c, L, yt, t + rkf::C6 * h, k6);
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+        // y + dt * (K * b_low) and dt * |K * (b_high - b_low)| (adapt_rk.h:268-271)
+        const double sl = k1[i] * rkf::BL1 + k3[i] * rkf::BL3 + k4[i] * rkf::BL4 + k5[i] * rkf::BL5;
+        yo[i]           = y[i] + sl * h;
+        const double sd = k1[i] * rkf::D1 + k3[i] * rkf::D3 + k4[i] * rkf::D4 + k5[i] * rkf::D5 + k6[i] * rkf::D6;
+        er[i]           = h * fabs(sd);
+    }
+}
+
+template <int G>
+__device__ __forceinline__ bool load_lane(const ClassDev& c, Lane<G>& L, int node_count)
+{
+    constexpr int NPW = WAVE / G;
+    const int gtid    = xcd_block() * BLOCK + int(threadIdx.x);
+    const int wave    = gtid / WAVE;
+    const int lane    = gtid % WAVE;
+    const int grp     = lane / G;
+    L.age             = lane - grp * G;
+    L.node            = wave * NPW + grp;
+    L.base            = grp * G; // lane index (within the wavefront) of the group's first lane
+    if (grp >= NPW || L.node >= node_count) {
+        return false;
+    }
+    L.np            = c.nodep[L.node];
+    const double* p = c.lcoef + (size_t(L.node) * G + L.age) * NUM_LC;
+#pragma unroll
+    for (int k = 0; k < NUM_LC; ++k) {
+        L.c[k] = p[k];
+    }
+    const double* B = c.cbase + (size_t(L.node) * G + L.age) * G;
+#pragma unroll
+    for (int j = 0; j < G; ++j) {
+        L.crow[j] = B[j];
+    }
+    return true;
+}
+
+// mobility factor of the ISy compartment of the lane's age group (model.h:687-715)
+template <int G>
+__device__ __forceinline__ double mobility_factor(const Lane<G>& L, const double (&y)[NC])
+{
+    if (!(L.np.flags & NF_TTC_ACTIVE)) {
+        return L.c[LC_RISK];
+    }
+    const double ttr = group_sum<G>(L.c[LC_TTR] * y[INS], L.base);
+    return smoother_cosine(ttr, L.np.ttc, L.np.ttc_hi, L.c[LC_RISK], L.c[LC_MAXRISK]);
+}
+
+// ------------------------------------------------------------------------------- clock
+// One GraphSimulation::advance loop iteration per instance (graph_simulation.h:101-120) and the
+// departure/return alternation of MobilityEdge::apply_mobility (metapopulation_mobility_instant.h:544-623).
+__global__ void clock_kernel(InstDev* inst, int n)
+{
+    const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n) {
+        return;
+    }
+    InstDev& I  = inst[i];
+    I.c_active  = 0;
+    I.c_action  = 0;
+    I.c_out_day = -1;
+    I.c_replay  = 0;
+    if (I.status != 0 || !(I.t < I.tmax)) {
+        return;
+    }
+    double step = I.dt_graph;
+    if (I.t + step > I.tmax) {
+        step = I.tmax - I.t;
+    }
+    // single-model runs end exactly at tmax (Simulation::advance(tmax)); graph runs at t + dt like
+    // SimulationNode::advance(t, dt)
+    const double tn = (I.exact_end && step != I.dt_graph) ? I.tmax : I.t + step;
+    I.c_active      = 1;
+    I.c_t0          = I.t;
+    I.c_t1          = tn;
+    I.c_step        = step;
+    if (I.estruct >= 0) {
+        if (I.pending && I.ret_time <= tn) {
+            if (I.toggle) {
+                I.c_action = 2;
+                I.c_tdep   = I.t_dep;
+                I.pending  = 0;
+            }
+            else {
+                I.status = MGO_ERR_UNSUPPORTED; // return and departure in one call: more than one pending move
+            }
+        }
+        else if (!I.toggle) {
+            if (I.pending) {
+                I.status = MGO_ERR_UNSUPPORTED;
+            }
+            else {
+                I.c_action = 1;
+                I.pending  = 1;
+                I.t_dep    = tn;
+                I.ret_time = tn + step;
+            }
+        }
+        I.toggle ^= 1;
+    }
+    I.t             = tn;
+    const double dn = rint(tn);
+    if (fabs(tn - dn) <= 1e-10) {
+        const int d = int(dn) - I.day0;
+        if (d >= 0 && d < I.ndays) {
+            I.c_out_day = d;
+        }
+    }
+}
+
+__device__ __forceinline__ double* out_ptr(const ClassDev& c, const InstDev& I, int node, int age, int G, int day)
+{
+    return I.out + (size_t(day) * I.num_nodes + (node - I.node_base)) * size_t(G * NC) + size_t(age) * NC;
+}
+
+// ------------------------------------------------------------------------------- node phase
+// SystemIntegrator::advance (integrator.h:184-249) of every node to the cycle's end time with the
+// integrator core K: odeint Cash-Karp controller (stepper_wrapper.h:112-163), RKIntegratorCore
+// (adapt_rk.h:217-298), fixed Cash-Karp (stepper_wrapper.h:240-255) or Euler (euler.h:54-62).
+// Loop is flattened to one try_step attempt per iteration so that a rejected node only idles through
+// the k1 evaluation of its neighbours in the wavefront.
+constexpr uint32_t kMaxAttemptsPerPhase = 1u << 20;
+
+template <int G, int K>
+__global__ void __launch_bounds__(BLOCK) node_phase(ClassDev c, InstDev* __restrict__ inst)
+{
+    Lane<G> L;
+    if (!load_lane<G>(c, L, c.num_nodes)) {
+        return;
+    }
+    InstDev& I = inst[L.np.inst];
+    if (!I.c_active) {
+        return;
+    }
+    double* yp = c.y + (size_t(L.node) * G + L.age) * NC;
+    double y[NC];
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+        y[i] = yp[i];
+    }
+    double t            = c.tnode[L.node];
+    double dt           = c.dtnode[L.node];
+    const double t0     = t;
+    const double tmax   = I.c_t1;
+    const double dt_max = I.dt_max;
+    double dt_min       = I.dt_min;
+    double dt_restore   = 0.0;
+    const double eps    = 1e-12;
+    // output days strictly inside this phase are interpolated here; a day at the phase end is written
+    // after the edge action
+    const int ndays  = I.ndays;
+    int next_day     = max(0, int(floor(I.c_t0 + 1e-10)) + 1 - I.day0);
+    const double tde = I.c_t1 - 1e-10;
+
+    bool active  = fabs((tmax - t) / (tmax - t0)) > eps;
+    bool need_k1 = true;
+    bool fail = true, valid = true;
+    uint32_t nsteps = 0, natt = 0;
+    double k1[NC];
+    while (active) {
+        if (natt >= kMaxAttemptsPerPhase) {
+            // a step size driven below any progress (dt_min ~ DBL_MIN with a persistent error): stop this
+            // node and report instead of spinning; the reference would not terminate here
+            if (L.age == 0) {
+                I.status = MGO_ERR_OUT_OF_RANGE;
+            }
+            break;
+        }
+        if (need_k1) {
+            if (dt > tmax - t) {
+                dt_restore = dt;
+                dt         = tmax - t;
+                dt_min     = fmin(tmax - t, dt_min);
+            }
+            rhs<G>(c, L, y, t, k1);
+            need_k1 = false;
+            fail    = true;
+            valid   = true;
+            if (K == MGO_INTEGRATOR_RKF45) {
+                dt = fmin(dt, dt_max);
+            }
+        }
+        double yo[NC], er[NC];
+        bool done       = false;
+        const double ta = t;
+        ++natt;
+        if (K == MGO_INTEGRATOR_CASH_KARP54) {
+            if (dt < dt_min) {
+                valid = false;
+                dt    = dt_min;
+            }
+            if (!(dt - dt_max <= DBL_EPSILON)) {
+                dt   = dt_max; // check_step_size_limit: rejected without a step
+                fail = true;
+            }
+            else {
+                cash_karp_stages<G>(c, L, y, k1, t, dt, yo, er, true);
+                double e = 0.0;
+#pragma unroll
+                for (int i = 0; i < NC; ++i) {
+                    e = fmax(e, fabs(er[i]) / (I.abs_tol + I.rel_tol * (fabs(y[i]) + dt * fabs(k1[i]))));
+                }
+                e = group_max<G>(e, L.base);
+                if (e > 1.0) {
+                    dt *= fmax(0.9 * pow(e, -1.0 / 3.0), 0.2);
+                    dt   = fmin(dt, dt_max);
+                    fail = true;
+                }
+                else {
+                    t += dt;
+                    if (e < 0.5) {
+                        e = fmax(1.0 / 3125.0, e);
+                        dt *= 0.9 * pow(e, -0.2);
+                        dt = fmin(dt, dt_max);
+                    }
+                    fail = false;
+                }
+            }
+            done = !(fail && valid);
+            if (done) {
+                dt = fmax(dt, dt_min);
+                if (fail) {
+                    t += dt_min;
+                }
+            }
+        }
+        else if (K == MGO_INTEGRATOR_RKF45) {
+            if (dt < dt_min) {
+                valid = false;
+                dt    = dt_min;
+            }
+            rkf45_stages<G>(c, L, y, k1, t, dt, yo, er);
+            bool conv = true;
+            double q  = INFINITY;
+#pragma unroll
+            for (int i = 0; i < NC; ++i) {
+                const double ep = I.abs_tol + fabs(yo[i]) * I.rel_tol;
+                conv            = conv && (er[i] <= ep);
+                q               = fmin(q, ep / er[i]);
+            }
+            conv = group_all<G>(conv, L.base);
+            q    = group_min<G>(q, L.base);
+            if (conv || !valid) {
+                t += dt;
+            }
+            double dt_new = dt * pow(q, 1.0 / 5.0);
+            dt_new *= 0.9;
+            dt   = fmin(dt_new, dt_max);
+            done = conv || !valid;
+            if (done) {
+                dt = fmax(dt, dt_min);
+            }
+        }
+        else if (K == MGO_INTEGRATOR_CASH_KARP54_FIXED) {
+            cash_karp_stages<G>(c, L, y, k1, t, dt, yo, er, false);
+            t += dt;
+            done = true;
+        }
+        else { // Euler
+#pragma unroll
+            for (int i = 0; i < NC; ++i) {
+                yo[i] = y[i] + dt * k1[i];
+            }
+            t += dt;
+            done = true;
+        }
+        if (done) {
+            ++nsteps;
+            // interpolate_simulation_result (analyze_result.h:235-252): days in (ta, t] strictly inside
+            // the phase
+            while (next_day < ndays && double(next_day + I.day0) <= t && double(next_day + I.day0) < tde) {
+                const double dd = double(next_day + I.day0);
+                if (dd > ta) {
+                    const double w = (dd - ta) / (t - ta);
+                    double* op     = out_ptr(c, I, L.node, L.age, G, next_day);
+#pragma unroll
+                    for (int i = 0; i < NC; ++i) {
+                        op[i] = y[i] + w * (yo[i] - y[i]);
+                    }
+                }
+                ++next_day;
+            }
+#pragma unroll
+            for (int i = 0; i < NC; ++i) {
+                y[i] = yo[i];
+            }
+            need_k1 = true;
+            active  = fabs((tmax - t) / (tmax - t0)) > eps;
+        }
+    }
+    dt = fmax(dt, dt_restore);
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+        yp[i] = y[i];
+    }
+    if (L.age == 0) {
+        c.tnode[L.node]  = t;
+        c.dtnode[L.node] = dt;
+        c.steps[L.node] += nsteps;
+        c.attempts[L.node] += natt;
+    }
+    const size_t off = (size_t(L.node) * G + L.age) * NC;
+    if (I.c_action == 1) {
+        // SimulationNode::m_last_state: departures and mobility factors use this snapshot
+#pragma unroll
+        for (int i = 0; i < NC; ++i) {
+            c.snap[off + i] = y[i];
+        }
+        c.fac[size_t(L.node) * G + L.age] = mobility_factor<G>(L, y);
+    }
+    else if (I.c_action == 2) {
+#pragma unroll
+        for (int i = 0; i < NC; ++i) {
+            c.ypre[off + i] = y[i];
+        }
+    }
+    else if (I.c_out_day >= 0) {
+        double* op = out_ptr(c, I, L.node, L.age, G, I.c_out_day);
+#pragma unroll
+        for (int i = 0; i < NC; ++i) {
+            op[i] = y[i];
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------- edge phase
+__device__ __forceinline__ double detect_nondetection(const NodeP& np, double t)
+{
+    // test_commuters (model.h:717-742): nondetection applies in [start, end)
+    return (t >= np.det_start && t < np.det_end) ? np.nondetect : 1.0;
+}
+
+// traveler vector of one edge from its source snapshot (metapopulation_mobility_instant.h:608-615)
+__device__ __forceinline__ void traveler_vector(const double* cvec, const EdgeEnt& e, const double* snap_src,
+                                                double fac_src, double nd_src, int G, int age, double (&m)[NC])
+{
+    if (e.cidx < 0) {
+#pragma unroll
+        for (int i = 0; i < NC; ++i) {
+            m[i] = snap_src[i] * e.cval;
+        }
+    }
+    else {
+        const double* cv = cvec + size_t(e.cidx) * G * NC + size_t(age) * NC;
+#pragma unroll
+        for (int i = 0; i < NC; ++i) {
+            m[i] = snap_src[i] * cv[i];
+        }
+    }
+    m[ISY] = m[ISY] * fac_src;
+    if (nd_src != 1.0) {
+        m[ISY] *= nd_src;
+        m[INS] *= nd_src;
+    }
+}
+
+// one explicit Euler step of the travelers with the destination's return coefficients
+// (calculate_mobility_returns, metapopulation_mobility_instant.h:460-471; euler.h:54-62)
+__device__ __forceinline__ void return_euler(const double* rc, double h, double (&m)[NC])
+{
+    double dy[NC];
+    flows_to_dydt(m, rc[RC_KAPPA], rc[RC_E], rc[RC_INS_ISY], rc[RC_INS_R], rc[RC_ISY_ISEV], rc[RC_ISY_R],
+                  rc[RC_ISEV_ICR], rc[RC_ISEV_R], rc[RC_ISEV_D], rc[RC_ICR_D], rc[RC_ICR_R], dy);
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+        m[i] = m[i] + h * dy[i];
+    }
+}
+
+template <int G>
+__global__ void __launch_bounds__(BLOCK) edge_phase(ClassDev c, InstDev* __restrict__ inst)
+{
+    Lane<G> L;
+    if (!load_lane<G>(c, L, c.num_nodes)) {
+        return;
+    }
+    InstDev& I = inst[L.np.inst];
+    if (!I.c_active || I.c_action == 0) {
+        return;
+    }
+    const EdgeStructDev& es = c.estructs[I.estruct];
+    const int v             = L.node - I.node_base;
+    const int in_b = es.in_ptr[v], in_e = es.in_ptr[v + 1];
+    const int in_s          = in_b + es.split[v];
+    const int out_b = es.out_ptr[v], out_e = es.out_ptr[v + 1];
+    const EdgeEnt* ine      = es.in_ent;
+    const EdgeEnt* oute     = es.out_ent;
+    const double* cvec      = es.cvec;
+    const size_t lane_off      = (size_t(L.node) * G + L.age);
+    double* yp                 = c.y + lane_off * NC;
+    double y[NC];
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+        y[i] = yp[i];
+    }
+    const double* snap_v = c.snap + lane_off * NC;
+    double sv[NC];
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+        sv[i] = snap_v[i];
+    }
+    const double fac_v = c.fac[lane_off];
+
+    if (I.c_action == 1) {
+        // departures at t = c_t1 (:605-621): to.y += m, from.y -= m, edges in (start, end) order
+        const double td   = I.c_t1;
+        const double nd_v = (L.np.flags & NF_DETECTION) ? detect_nondetection(L.np, td) : 1.0;
+        auto in_edge      = [&](int k) {
+            const EdgeEnt e = ine[k];
+            const int s     = I.node_base + e.nbr;
+            const NodeP& ns = c.nodep[s];
+            const double nd = (ns.flags & NF_DETECTION) ? detect_nondetection(ns, td) : 1.0;
+            double m[NC];
+            traveler_vector(cvec, e, c.snap + (size_t(s) * G + L.age) * NC, c.fac[size_t(s) * G + L.age], nd, G, L.age, m);
+#pragma unroll
+            for (int i = 0; i < NC; ++i) {
+                y[i] += m[i];
+            }
+        };
+        for (int k = in_b; k < in_s; ++k) {
+            in_edge(k);
+        }
+        for (int k = out_b; k < out_e; ++k) {
+            double m[NC];
+            traveler_vector(cvec, oute[k], sv, fac_v, 1.0, G, L.age, m);
+            if (nd_v != 1.0) {
+                y[INS] -= m[INS] * (1 - nd_v);
+                y[INSC] += m[INS] * (1 - nd_v);
+                y[ISY] -= m[ISY] * (1 - nd_v);
+                y[ISYC] += m[ISY] * (1 - nd_v);
+                m[ISY] *= nd_v;
+                m[INS] *= nd_v;
+            }
+#pragma unroll
+            for (int i = 0; i < NC; ++i) {
+                y[i] -= m[i];
+            }
+        }
+        for (int k = in_s; k < in_e; ++k) {
+            in_edge(k);
+        }
+        // return coefficients of this node as a destination, from its post-exchange state at t_dep
+        double c_icr, c_d;
+        const double kappa = force_of_infection<G>(c, L, y, td, c_icr, c_d);
+        double* rc         = c.rc + lane_off * RC_STRIDE;
+        rc[RC_KAPPA]       = kappa;
+        rc[RC_E]           = L.c[LC_E];
+        rc[RC_INS_ISY]     = L.c[LC_INS_ISY];
+        rc[RC_INS_R]       = L.c[LC_INS_R];
+        rc[RC_ISY_ISEV]    = L.c[LC_ISY_ISEV];
+        rc[RC_ISY_R]       = L.c[LC_ISY_R];
+        rc[RC_ISEV_ICR]    = c_icr;
+        rc[RC_ISEV_R]      = L.c[LC_ISEV_R];
+        rc[RC_ISEV_D]      = c_d;
+        rc[RC_ICR_D]       = L.c[LC_ICR_D];
+        rc[RC_ICR_R]       = L.c[LC_ICR_R];
+    }
+    else {
+        // returns at t = c_t1 of the travelers that left at c_tdep (:568-603)
+        const double td   = I.c_tdep;
+        const double h    = I.c_step;
+        const double nd_v = (L.np.flags & NF_DETECTION) ? detect_nondetection(L.np, td) : 1.0;
+        const double* rcv = c.rc + lane_off * RC_STRIDE;
+        double rcl[RC_STRIDE];
+#pragma unroll
+        for (int k = 0; k < RC_STRIDE; ++k) {
+            rcl[k] = rcv[k];
+        }
+        bool negative = false;
+        auto in_edge  = [&](int k) {
+            const EdgeEnt e = ine[k];
+            const int s     = I.node_base + e.nbr;
+            const NodeP& ns = c.nodep[s];
+            const double nd = (ns.flags & NF_DETECTION) ? detect_nondetection(ns, td) : 1.0;
+            double m[NC];
+            traveler_vector(cvec, e, c.snap + (size_t(s) * G + L.age) * NC, c.fac[size_t(s) * G + L.age], nd, G, L.age, m);
+            return_euler(rcl, h, m);
+#pragma unroll
+            for (int i = 0; i < NC; ++i) {
+                negative = negative || (y[i] - m[i] < 0);
+                y[i] -= m[i];
+            }
+        };
+        for (int k = in_b; k < in_s; ++k) {
+            in_edge(k);
+        }
+        for (int k = out_b; k < out_e; ++k) {
+            const EdgeEnt e = oute[k];
+            const int d     = I.node_base + e.nbr;
+            double m[NC];
+            traveler_vector(cvec, e, sv, fac_v, nd_v, G, L.age, m);
+            return_euler(c.rc + (size_t(d) * G + L.age) * RC_STRIDE, h, m);
+#pragma unroll
+            for (int i = 0; i < NC; ++i) {
+                y[i] += m[i];
+            }
+        }
+        for (int k = in_s; k < in_e; ++k) {
+            in_edge(k);
+        }
+        if (negative) {
+            I.c_replay = 1; // benign race: every writer stores 1
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+        yp[i] = y[i];
+    }
+    if (I.c_out_day >= 0) {
+        double* op = out_ptr(c, I, L.node, L.age, G, I.c_out_day);
+#pragma unroll
+        for (int i = 0; i < NC; ++i) {
+            op[i] = y[i];
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------- exact replay
+// Sequential return pass of one instance in the reference's edge order, with the negative-value fix-up
+// (metapopulation_mobility_instant.h:568-603). Runs only for instances the optimistic return kernel
+// flagged; lanes = age groups.
+template <int G>
+__global__ void __launch_bounds__(WAVE) replay_phase(ClassDev c, InstDev* __restrict__ inst)
+{
+    const int ii = c.inst_ids[blockIdx.x];
+    InstDev& I   = inst[ii];
+    if (!I.c_active || I.c_action != 2 || !I.c_replay) {
+        return;
+    }
+    const int a = int(threadIdx.x);
+    // restore the pre-return state of every node
+    for (int v = 0; v < I.num_nodes; ++v) {
+        if (a < G) {
+            const size_t off = (size_t(I.node_base + v) * G + a) * NC;
+            for (int i = 0; i < NC; ++i) {
+                c.y[off + i] = c.ypre[off + i];
+            }
+        }
+    }
+    __syncthreads();
+    const EdgeStructDev& es = c.estructs[I.estruct];
+    const int32_t* seq      = es.seq;
+    const EdgeEnt* oute     = es.out_ent;
+    const double* cvec      = es.cvec;
+    const double td        = I.c_tdep;
+    const double h         = I.c_step;
+    if (a < G) {
+        for (int k = 0; k < es.num_edges; ++k) {
+            const int s       = I.node_base + seq[3 * k];
+            const int d       = I.node_base + seq[3 * k + 1];
+            const EdgeEnt e   = oute[seq[3 * k + 2]];
+            const NodeP& ns   = c.nodep[s];
+            const double nd   = (ns.flags & NF_DETECTION) ? detect_nondetection(ns, td) : 1.0;
+            const size_t soff = size_t(s) * G + a;
+            const size_t doff = size_t(d) * G + a;
+            double m[NC];
+            traveler_vector(cvec, e, c.snap + soff * NC, c.fac[soff], nd, G, a, m);
+            return_euler(c.rc + doff * RC_STRIDE, h, m);
+            double* yd = c.y + doff * NC;
+            double* ys = c.y + soff * NC;
+            double rem[NC];
+            for (int i = 0; i < NC; ++i) {
+                rem[i] = yd[i] - m[i];
+            }
+            for (int j = 0; j < NC; ++j) {
+                if (rem[j] < 0) {
+                    int mx = 0;
+                    for (int q = 1; q < NC; ++q) {
+                        if (rem[q] > rem[mx]) {
+                            mx = q;
+                        }
+                    }
+                    m[mx] -= rem[j];
+                    m[j] += rem[j];
+                }
+            }
+            for (int i = 0; i < NC; ++i) {
+                ys[i] += m[i];
+            }
+            for (int i = 0; i < NC; ++i) {
+                yd[i] -= m[i];
+            }
+        }
+    }
+    __syncthreads();
+    if (a == 0) {
+        I.replays += 1;
+    }
+    if (I.c_out_day >= 0 && a < G) {
+        for (int v = 0; v < I.num_nodes; ++v) {
+            const size_t off = (size_t(I.node_base + v) * G + a) * NC;
+            double* op       = out_ptr(c, I, I.node_base + v, a, G, I.c_out_day);
+            for (int i = 0; i < NC; ++i) {
+                op[i] = c.y[off + i];
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------- launchers
+static int padded_blocks(int nodes, int G)
+{
+    const int npw    = WAVE / G;
+    const int waves  = (nodes + npw - 1) / npw;
+    const int blocks = (waves * WAVE + BLOCK - 1) / BLOCK;
+    return ((blocks + 7) / 8) * 8;
+}
+
+template <int G>
+static hipError_t launch_node_g(const ClassDev& c, InstDev* inst, hipStream_t s)
+{
+    const int nb = padded_blocks(c.num_nodes, G);
+    switch (c.kind) {
+    case MGO_INTEGRATOR_CASH_KARP54:
+        hipLaunchKernelGGL((node_phase<G, MGO_INTEGRATOR_CASH_KARP54>), dim3(nb), dim3(BLOCK), 0, s, c, inst);
+        break;
+    case MGO_INTEGRATOR_RKF45:
+        hipLaunchKernelGGL((node_phase<G, MGO_INTEGRATOR_RKF45>), dim3(nb), dim3(BLOCK), 0, s, c, inst);
+        break;
+    case MGO_INTEGRATOR_CASH_KARP54_FIXED:
+        hipLaunchKernelGGL((node_phase<G, MGO_INTEGRATOR_CASH_KARP54_FIXED>), dim3(nb), dim3(BLOCK), 0, s, c, inst);
+        break;
+    default:
+        hipLaunchKernelGGL((node_phase<G, MGO_INTEGRATOR_EULER>), dim3(nb), dim3(BLOCK), 0, s, c, inst);
+        break;
+    }
+    return hipGetLastError();
+}
+
+template <int G>
+static hipError_t launch_edge_g(const ClassDev& c, InstDev* inst, hipStream_t s)
+{
+    const int nb = padded_blocks(c.num_nodes, G);
+    hipLaunchKernelGGL((edge_phase<G>), dim3(nb), dim3(BLOCK), 0, s, c, inst);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) {
+        return err;
+    }
+    hipLaunchKernelGGL((replay_phase<G>), dim3(c.num_inst), dim3(WAVE), 0, s, c, inst);
+    return hipGetLastError();
+}
+
+hipError_t launch_clock(InstDev* inst, int n, hipStream_t s)
+{
+    hipLaunchKernelGGL(clock_kernel, dim3((n + 255) / 256), dim3(256), 0, s, inst, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_node(const ClassDev& c, InstDev* inst, hipStream_t s)
+{
+    switch (c.G) {
+    case 1:
+        return launch_node_g<1>(c, inst, s);
+    case 2:
+        return launch_node_g<2>(c, inst, s);
+    case 3:
+        return launch_node_g<3>(c, inst, s);
+    case 4:
+        return launch_node_g<4>(c, inst, s);
+    case 5:
+        return launch_node_g<5>(c, inst, s);
+    case 6:
+        return launch_node_g<6>(c, inst, s);
+    case 7:
+        return launch_node_g<7>(c, inst, s);
+    case 8:
+        return launch_node_g<8>(c, inst, s);
+    default:
+        return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_edge(const ClassDev& c, InstDev* inst, hipStream_t s)
+{
+    switch (c.G) {
+    case 1:
+        return launch_edge_g<1>(c, inst, s);
+    case 2:
+        return launch_edge_g<2>(c, inst, s);
+    case 3:
+        return launch_edge_g<3>(c, inst, s);
+    case 4:
+        return launch_edge_g<4>(c, inst, s);
+    case 5:
+        return launch_edge_g<5>(c, inst, s);
+    case 6:
+        return launch_edge_g<6>(c, inst, s);
+    case 7:
+        return launch_edge_g<7>(c, inst, s);
+    case 8:
+        return launch_edge_g<8>(c, inst, s);
+    default:
+        return hipErrorInvalidValue;
+    }
+}
+
+} // namespace mgo

@@ -1,0 +1,159 @@
+"""Thin Python driver over the C ABI of libmgo.so (include/mgo.h), used by tests and bench.py.
+
+Every call goes to the HIP engine; a missing library or device is an error, never a fallback.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import capi
+
+
+class MgoError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"mgo status {status}: {msg}")
+        self.status = status
+
+
+class Context:
+    def __init__(self, device=0, hbm_budget_bytes=0, flags=0):
+        self.lib = capi.load()
+        self.ptr = C.c_void_p()
+        cfg = capi.DeviceCfg(device, hbm_budget_bytes, flags)
+        st = self.lib.mgo_context_create(C.byref(cfg), C.byref(self.ptr))
+        if st != capi.OK:
+            raise MgoError(st, self.lib.mgo_last_error(None).decode())
+
+    def check(self, st):
+        if st != capi.OK:
+            raise MgoError(st, self.lib.mgo_last_error(self.ptr).decode())
+
+    def close(self):
+        if self.ptr:
+            self.lib.mgo_context_destroy(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def clock_ms(self):
+        return self.lib.mgo_clock_ms(self.ptr)
+
+    # ------------------------------------------------------------------ single model
+    def simulate_single(self, spec, cfg, tmax):
+        nd = spec.num_days(cfg.t0, tmax)
+        out = np.zeros((max(nd, 1), spec.num_nodes, spec.state_size()))
+        got = C.c_int32(0)
+        self.check(self.lib.mgo_simulate_single(self.ptr, C.byref(spec.desc()), C.byref(cfg), tmax, nd,
+                                                capi.dptr(out), C.byref(got)))
+        return out[:got.value]
+
+    # ------------------------------------------------------------------ ensemble
+    def _ensemble_desc(self, specs, cfgs, tmaxs):
+        nr = len(specs)
+        graphs = (capi.GraphDesc * nr)(*[s.desc() for s in specs])
+        cfg_arr = (capi.SimCfg * nr)(*cfgs)
+        tm = np.ascontiguousarray(tmaxs, dtype=np.float64)
+        desc = capi.EnsembleDesc(nr, graphs, cfg_arr, capi.dptr(tm))
+        keep = (graphs, cfg_arr, tm)
+        return desc, keep
+
+    def ensemble_run(self, specs, cfgs, tmaxs, want_output=True):
+        desc, keep = self._ensemble_desc(specs, cfgs, tmaxs)
+        sizes = [s.num_days(c.t0, t) * s.num_nodes * s.state_size() for s, c, t in zip(specs, cfgs, tmaxs)]
+        out = np.zeros(sum(sizes)) if want_output else None
+        info = capi.RunInfo()
+        self.check(self.lib.mgo_ensemble_run(self.ptr, C.byref(desc), capi.dptr(out) if want_output else None,
+                                             C.byref(info)))
+        del keep
+        if not want_output:
+            return None, info
+        res, o = [], 0
+        for s, c, t, sz in zip(specs, cfgs, tmaxs, sizes):
+            res.append(out[o:o + sz].reshape(s.num_days(c.t0, t), s.num_nodes, s.state_size()))
+            o += sz
+        return res, info
+
+
+class Batch:
+    """Device-resident ensemble: create once, run, read back (mgo_batch_*)."""
+
+    def __init__(self, ctx, specs, cfgs, tmaxs):
+        self.ctx = ctx
+        self.specs, self.cfgs, self.tmaxs = specs, cfgs, list(tmaxs)
+        desc, keep = ctx._ensemble_desc(specs, cfgs, tmaxs)
+        self.id = C.c_int64(0)
+        ctx.check(ctx.lib.mgo_batch_create(ctx.ptr, C.byref(desc), C.byref(self.id)))
+        del keep
+
+    def run(self):
+        info = capi.RunInfo()
+        self.ctx.check(self.ctx.lib.mgo_batch_run(self.ctx.ptr, self.id, C.byref(info)))
+        return info
+
+    def reset(self):
+        self.ctx.check(self.ctx.lib.mgo_batch_reset(self.ctx.ptr, self.id))
+
+    def read_daily(self, first=0, count=None):
+        count = len(self.specs) - first if count is None else count
+        sel = list(zip(self.specs, self.cfgs, self.tmaxs))[first:first + count]
+        sizes = [s.num_days(c.t0, t) * s.num_nodes * s.state_size() for s, c, t in sel]
+        out = np.zeros(sum(sizes))
+        self.ctx.check(self.ctx.lib.mgo_batch_read_daily(self.ctx.ptr, self.id, first, count, capi.dptr(out)))
+        res, o = [], 0
+        for (s, c, t), sz in zip(sel, sizes):
+            res.append(out[o:o + sz].reshape(s.num_days(c.t0, t), s.num_nodes, s.state_size()))
+            o += sz
+        return res
+
+    def close(self):
+        if self.id.value:
+            self.ctx.lib.mgo_batch_destroy(self.ctx.ptr, self.id)
+            self.id = C.c_int64(0)
+
+
+class GraphSim:
+    """make_mobility_sim(t0, dt, graph) on the device; advance(tmax) == GraphSimulation::advance."""
+
+    def __init__(self, ctx, spec, cfg, horizon):
+        self.ctx, self.spec = ctx, spec
+        cfg = capi.SimCfg(cfg.t0, cfg.dt_graph, cfg.dt_node, horizon, cfg.integrator)
+        self.cfg = cfg
+        self.id = C.c_int64(0)
+        ctx.check(ctx.lib.mgo_graph_create(ctx.ptr, C.byref(spec.desc()), C.byref(cfg), C.byref(self.id)))
+
+    def advance(self, tmax):
+        self.ctx.check(self.ctx.lib.mgo_graph_advance(self.ctx.ptr, self.id, tmax))
+
+    def get_t(self):
+        t = C.c_double(0)
+        self.ctx.check(self.ctx.lib.mgo_graph_get_time(self.ctx.ptr, self.id, C.byref(t)))
+        return t.value
+
+    def state(self):
+        y = np.zeros((self.spec.num_nodes, self.spec.state_size()))
+        self.ctx.check(self.ctx.lib.mgo_graph_read_state(self.ctx.ptr, self.id, capi.dptr(y)))
+        return y
+
+    def daily(self):
+        cap = self.spec.num_days(self.cfg.t0, self.cfg.horizon)
+        out = np.zeros((max(cap, 1), self.spec.num_nodes, self.spec.state_size()))
+        n = C.c_int32(0)
+        self.ctx.check(self.ctx.lib.mgo_graph_read_daily(self.ctx.ptr, self.id, cap, capi.dptr(out), C.byref(n)))
+        return out[:n.value]
+
+    def stats(self):
+        s = np.zeros(self.spec.num_nodes, np.int64)
+        a = np.zeros(self.spec.num_nodes, np.int64)
+        self.ctx.check(self.ctx.lib.mgo_graph_read_stats(self.ctx.ptr, self.id,
+                                                         s.ctypes.data_as(C.POINTER(C.c_int64)),
+                                                         a.ctypes.data_as(C.POINTER(C.c_int64))))
+        return s, a
+
+    def close(self):
+        if self.id.value:
+            self.ctx.lib.mgo_graph_destroy(self.ctx.ptr, self.id)
+            self.id = C.c_int64(0)

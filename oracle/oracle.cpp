@@ -1241,6 +1241,34 @@ int32_t orc_simulate(const mgo_graph_desc* model, const mgo_sim_cfg* cfg, double
     return np;
 }
 
+int32_t orc_simulate_daily(const mgo_graph_desc* model, const mgo_sim_cfg* cfg, double tmax, int32_t capacity_days,
+                           double* daily_out, orc_stats* stats)
+{
+    if (!valid_desc(model) || !cfg) {
+        return -MGO_ERR_INVALID_VALUE;
+    }
+    auto core = make_core(cfg->integrator);
+    if (!core) {
+        return -MGO_ERR_INVALID_VALUE;
+    }
+    OsecirModel m(model->num_groups);
+    build_osecir(model, 0, m);
+    Simulation sim(m, std::move(core), cfg->t0, cfg->dt_node);
+    sim.advance(tmax);
+    std::vector<double> days;
+    std::vector<Vec> vals;
+    interpolate_daily(sim.result, days, vals);
+    for (size_t d = 0; d < days.size() && int32_t(d) < capacity_days; ++d) {
+        std::copy(vals[d].begin(), vals[d].end(), daily_out + d * sim.result.n);
+    }
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        stats->steps    = sim.steps;
+        stats->attempts = sim.core->attempts;
+    }
+    return int32_t(days.size());
+}
+
 int32_t orc_graph_run(const mgo_graph_desc* graph, const mgo_sim_cfg* cfg, double tmax, int32_t capacity_days,
                       double* daily_out, int32_t* num_days, orc_stats* stats)
 {

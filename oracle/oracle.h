@@ -32,6 +32,11 @@ typedef struct orc_stats {
 int32_t orc_simulate(const mgo_graph_desc* model, const mgo_sim_cfg* cfg, double tmax, int32_t capacity,
                      double* times, double* values);
 
+/* orc_simulate followed by interpolate_simulation_result (cpp/memilio/data/analyze_result.h:186-265):
+ * daily_out[day][10*G]; returns the number of days (or negative status). */
+int32_t orc_simulate_daily(const mgo_graph_desc* model, const mgo_sim_cfg* cfg, double tmax, int32_t capacity_days,
+                           double* daily_out, orc_stats* stats);
+
 /* make_mobility_sim(t0, dt_graph, graph).advance(tmax), then interpolate_simulation_result on every node.
  * daily_out[day][node][10*G], *num_days days from ceil(t0). Returns MGO_OK or an error. */
 int32_t orc_graph_run(const mgo_graph_desc* graph, const mgo_sim_cfg* cfg, double tmax, int32_t capacity_days,

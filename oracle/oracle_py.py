@@ -37,6 +37,9 @@ def lib():
         L = C.CDLL(LIB_PATH)
         L.orc_simulate.restype = C.c_int32
         L.orc_simulate.argtypes = [C.POINTER(capi.GraphDesc), C.POINTER(capi.SimCfg), C.c_double, C.c_int32, _dp, _dp]
+        L.orc_simulate_daily.restype = C.c_int32
+        L.orc_simulate_daily.argtypes = [C.POINTER(capi.GraphDesc), C.POINTER(capi.SimCfg), C.c_double, C.c_int32,
+                                         _dp, C.POINTER(OrcStats)]
         L.orc_graph_run.restype = C.c_int32
         L.orc_graph_run.argtypes = [C.POINTER(capi.GraphDesc), C.POINTER(capi.SimCfg), C.c_double, C.c_int32, _dp,
                                     _ip, C.POINTER(OrcStats)]
@@ -70,6 +73,17 @@ def simulate(spec, cfg, tmax, capacity=200000):
     if np_ > capacity:
         return simulate(spec, cfg, tmax, np_)
     return times[:np_].copy(), vals[:np_].copy()
+
+
+def simulate_daily(spec, cfg, tmax):
+    """interpolate_simulation_result of orc_simulate: [day][10G] for node 0, and stats."""
+    nd = spec.num_days(cfg.t0, tmax)
+    out = np.zeros((nd, spec.state_size()))
+    st = OrcStats()
+    got = lib().orc_simulate_daily(C.byref(spec.desc()), C.byref(cfg), tmax, nd, capi.dptr(out), C.byref(st))
+    if got < 0:
+        raise RuntimeError(f"orc_simulate_daily failed: {got}")
+    return out[:got], st
 
 
 def graph_run(spec, cfg, tmax):
