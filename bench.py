@@ -88,7 +88,7 @@ def main():
             times.append(dt)
             infos.append(info)
             log(f"step {k}: {dt * 1e3:.1f} ms wall, {info.device_ms:.1f} ms device, steps {info.steps}, "
-                f"attempts {info.attempts}, replays {info.exact_replays}")
+                f"attempts {info.attempts}, fix-up passes {info.fixup_passes}, replays {info.exact_replays}")
         total = sum(times)
         value = node_days * args.steps / total
         cpu_n = min(args.cpu_samples, S)

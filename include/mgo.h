@@ -93,8 +93,10 @@ enum {
  * The effective matrix is B - D(t) o (B - M) with D(t) the accumulated damping (damping.h:381-396).
  *
  * Edges: MobilityEdge coefficients, one vector of length 10*G per edge (metapopulation_mobility_instant.h:
- * 319-327). Edges may be given in any order; duplicates (same start,end) keep the last one, as
- * Graph::add_edge's insert_sorted_replace does (graph.h:231-241).
+ * 319-327), or one value per edge that applies to every compartment (edge_coeff_len = 1, the
+ * Eigen::VectorX::Constant form the reference's examples use). Edges may be given in any order;
+ * duplicates (same start,end) keep the last one, as Graph::add_edge's insert_sorted_replace does
+ * (graph.h:231-241).
  */
 typedef struct mgo_graph_desc {
     int32_t num_nodes;
@@ -112,7 +114,8 @@ typedef struct mgo_graph_desc {
     int32_t num_edges;
     const int32_t* edge_start;        /* [num_edges] */
     const int32_t* edge_end;          /* [num_edges] */
-    const double* edge_coeffs;        /* [num_edges][10*G] */
+    const double* edge_coeffs;        /* [num_edges][edge_coeff_len] */
+    int32_t edge_coeff_len;           /* 10*G, or 1 (one coefficient for all compartments); 0 means 10*G */
 } mgo_graph_desc;
 
 /* ---------------------------------------------------------------- integrators
@@ -157,6 +160,7 @@ typedef struct mgo_device_cfg {
 
 #define MGO_CTX_NO_GRAPH_CAPTURE 1   /* launch the half-day kernels directly instead of replaying a hipGraph */
 #define MGO_CTX_PROFILE_KERNELS 2    /* time every node / edge kernel launch with HIP events (no graph capture) */
+#define MGO_CTX_SEQUENTIAL_FIXUP 4   /* resolve return-underflow fix-ups with the sequential replay only */
 
 mgo_status mgo_context_create(const mgo_device_cfg* cfg, mgo_context** out);
 mgo_status mgo_context_destroy(mgo_context* ctx);
@@ -209,7 +213,8 @@ typedef struct mgo_run_info {
     double node_days;          /* sum over runs of nodes * (tmax - t0) */
     int64_t steps;             /* accepted integration steps summed over all nodes */
     int64_t attempts;          /* step attempts summed over all nodes */
-    int64_t exact_replays;     /* return phases that needed the sequential negative-value fix-up replay */
+    int64_t exact_replays;     /* return phases resolved by the sequential negative-value fix-up replay */
+    int64_t fixup_passes;      /* extra parallel return passes run to settle negative-value fix-ups */
     int32_t half_day_cycles;   /* graph steps executed */
     int32_t node_launches;     /* node-phase kernel launches timed (MGO_CTX_PROFILE_KERNELS) */
     double node_kernel_ms;     /* summed duration of those launches */

@@ -60,6 +60,7 @@ INTEGRATOR_EULER = 3
 
 CTX_NO_GRAPH_CAPTURE = 1
 CTX_PROFILE_KERNELS = 2
+CTX_SEQUENTIAL_FIXUP = 4
 
 DBL_MAX = np.finfo(np.float64).max
 DBL_MIN = np.finfo(np.float64).tiny
@@ -86,6 +87,7 @@ class GraphDesc(C.Structure):
         ("edge_start", _ip),
         ("edge_end", _ip),
         ("edge_coeffs", _dp),
+        ("edge_coeff_len", C.c_int32),
     ]
 
 
@@ -133,6 +135,7 @@ class RunInfo(C.Structure):
         ("steps", C.c_int64),
         ("attempts", C.c_int64),
         ("exact_replays", C.c_int64),
+        ("fixup_passes", C.c_int64),
         ("half_day_cycles", C.c_int32),
         ("node_launches", C.c_int32),
         ("node_kernel_ms", C.c_double),

@@ -31,8 +31,8 @@
 namespace mgo
 {
 hipError_t launch_clock(InstDev* inst, int n, hipStream_t s);
-hipError_t launch_node(const ClassDev& c, InstDev* inst, hipStream_t s);
-hipError_t launch_edge(const ClassDev& c, InstDev* inst, hipStream_t s);
+hipError_t launch_node(const LaunchDev& c, InstDev* inst, hipStream_t s);
+hipError_t launch_edge(const LaunchDev& c, InstDev* inst, hipStream_t s);
 } // namespace mgo
 
 using namespace mgo;
@@ -65,23 +65,96 @@ void hip_check(hipError_t e, const char* what)
     }
 }
 
+// First-fit sub-allocator over one device allocation (serving: per-request buffers without hipMalloc).
+struct Arena {
+    char* base  = nullptr;
+    size_t size = 0;
+    size_t used = 0;
+    std::map<size_t, size_t> free_; // offset -> length
+    void init(size_t bytes)
+    {
+        size = bytes;
+        hip_check(hipMalloc(reinterpret_cast<void**>(&base), size), "serving arena");
+        free_.clear();
+        free_[0] = size;
+        used     = 0;
+    }
+    void release()
+    {
+        if (base) {
+            (void)hipFree(base);
+        }
+        base = nullptr;
+    }
+    void* alloc(size_t bytes)
+    {
+        bytes = (bytes + 255) & ~size_t(255);
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= bytes) {
+                const size_t off = it->first, len = it->second;
+                free_.erase(it);
+                if (len > bytes) {
+                    free_[off + bytes] = len - bytes;
+                }
+                used += bytes;
+                return base + off;
+            }
+        }
+        fail(MGO_ERR_OUT_OF_MEMORY, "serving arena exhausted (%zu of %zu bytes used)", used, size);
+    }
+    void free(void* p, size_t bytes)
+    {
+        bytes      = (bytes + 255) & ~size_t(255);
+        size_t off = size_t(static_cast<char*>(p) - base);
+        used -= bytes;
+        auto next = free_.lower_bound(off);
+        if (next != free_.end() && off + bytes == next->first) {
+            bytes += next->second;
+            next = free_.erase(next);
+        }
+        if (next != free_.begin()) {
+            auto prev = std::prev(next);
+            if (prev->first + prev->second == off) {
+                prev->second += bytes;
+                return;
+            }
+        }
+        free_[off] = bytes;
+    }
+};
+
+// allocations of the calling thread go to this arena when set (the serving worker)
+thread_local Arena* tl_arena = nullptr;
+
 // device buffer owned by a batch
 struct DevBuf {
     void* p      = nullptr;
     size_t bytes = 0;
+    Arena* arena = nullptr;
     DevBuf()     = default;
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
     ~DevBuf()
     {
         if (p) {
-            (void)hipFree(p);
+            if (arena) {
+                arena->free(p, bytes);
+            }
+            else {
+                (void)hipFree(p);
+            }
         }
     }
     void alloc(size_t b)
     {
         bytes = std::max<size_t>(b, 8);
-        hip_check(hipMalloc(&p, bytes), "hipMalloc");
+        arena = tl_arena;
+        if (arena) {
+            p = arena->alloc(bytes);
+        }
+        else {
+            hip_check(hipMalloc(&p, bytes), "hipMalloc");
+        }
     }
     template <class T>
     void upload(const std::vector<T>& v, hipStream_t s)
@@ -201,11 +274,37 @@ struct ClassHost {
     // device buffers
     DevBuf y, snap, ypre, fac, rc, lcoef, cbase, cdelta, nodep, dtime, dmat, tnode, dtnode, steps, attempts, out,
         ipool, epool, cvec, estructs, inst_ids;
+    DevBuf fx[2], fx_pool[2];          // fix-up records of the return passes
     DevBuf y0;                         // initial state, for reset
     std::vector<double> tnode0, dtnode0; // initial clocks, for reset
     size_t out_doubles = 0;
-    ClassDev dev{};
+    DevBuf seg_dev, block_seg, block_first; // launch tables (one segment = the whole class)
+    SegDev seg_host{};                       // host copy of the segment descriptor
+    LaunchDev launch{};
 };
+
+// blocks of a segment of n nodes with G age groups (BLOCK/WAVE wavefronts of 64/G nodes per block)
+int segment_blocks(int n, int G)
+{
+    const int nodes_per_block = (BLOCK / WAVE) * (WAVE / G);
+    return (n + nodes_per_block - 1) / nodes_per_block;
+}
+
+// block -> segment table for a list of segments; padded to a multiple of 8 blocks (XCD remap)
+void block_tables(const std::vector<int>& seg_nodes, int G, std::vector<int32_t>& block_seg,
+                  std::vector<int32_t>& block_first)
+{
+    block_seg.clear();
+    block_first.clear();
+    for (size_t k = 0; k < seg_nodes.size(); ++k) {
+        block_first.push_back(int32_t(block_seg.size()));
+        const int nb = segment_blocks(seg_nodes[k], G);
+        block_seg.insert(block_seg.end(), size_t(nb), int32_t(k));
+    }
+    while (block_seg.empty() || block_seg.size() % 8 != 0) {
+        block_seg.push_back(-1);
+    }
+}
 
 struct InstHost {
     int cls;
@@ -336,10 +435,11 @@ void build_edges(const mgo_graph_desc* g, EdgeStructHost& es, std::vector<double
     };
     std::vector<E> edges;
     edges.reserve(last.size());
+    const int len = (g->edge_coeff_len == 1) ? 1 : nv;
     for (auto& kv : last) {
-        const double* c = g->edge_coeffs + size_t(kv.second) * nv;
+        const double* c = g->edge_coeffs + size_t(kv.second) * len;
         bool any_pos = false, uniform = true;
-        for (int k = 0; k < nv; ++k) {
+        for (int k = 0; k < len; ++k) {
             any_pos = any_pos || (c[k] > 0.0);
             uniform = uniform && (c[k] == c[0]);
         }
@@ -373,9 +473,9 @@ void build_edges(const mgo_graph_desc* g, EdgeStructHost& es, std::vector<double
     // edges are sorted by (s, d): out lists come out sorted by d, in lists sorted by s
     for (auto& e : edges) {
         const int ko    = ofill[size_t(e.s)]++;
-        es.out_ent[size_t(ko)] = EdgeEnt{e.cval, e.d, e.cidx};
         const int ki    = ifill[size_t(e.d)]++;
-        es.in_ent[size_t(ki)] = EdgeEnt{e.cval, e.s, e.cidx};
+        es.out_ent[size_t(ko)] = EdgeEnt{e.cval, e.d, e.cidx, ki, 0};
+        es.in_ent[size_t(ki)]  = EdgeEnt{e.cval, e.s, e.cidx, ko, 0};
         if (e.s < e.d) {
             es.split[size_t(e.d)]++;
         }
@@ -395,6 +495,7 @@ struct Batch {
     hipGraphExec_t graph_exec = nullptr;
     bool use_graph            = true;
     bool profile              = false;
+    bool sequential_fixup     = false;
     size_t device_bytes       = 0;
     std::vector<InstDev> inst0; // instance table at creation (tmax set), for reset
     // kernel timing of the last run (profile mode)
@@ -420,6 +521,8 @@ struct Batch {
         classes.back().kind = kind;
         return classes.size() - 1;
     }
+
+    int inst_offset = 0; // global index of instance 0 (serving: the request's slot in the shared table)
 
     void build(std::vector<RunSpec>& runs, hipStream_t s)
     {
@@ -479,6 +582,8 @@ struct Batch {
         std::vector<size_t> damp_off(N, 0);
         std::map<std::tuple<const void*, const void*, const void*, int, int>, int> struct_cache;
         int64_t out_total = 0;
+        int64_t fx_total  = 0;
+        int64_t pool_slots = 0;
         for (int ii : C.insts) {
             RunSpec& r    = runs[size_t(ii)];
             InstHost& I   = inst[size_t(ii)];
@@ -496,7 +601,7 @@ struct Batch {
                 np.nondetect      = sc[MGO_P_COMMUTER_NONDETECTION];
                 np.det_start      = sc[MGO_P_START_COMMUTER_DETECTION];
                 np.det_end        = sc[MGO_P_END_COMMUTER_DETECTION];
-                np.inst           = ii;
+                np.inst           = inst_offset + ii;
                 np.flags          = 0;
                 if (np.ttc < DBL_MAX) {
                     np.flags |= NF_TTC_ACTIVE;
@@ -588,6 +693,12 @@ struct Batch {
                     I.has_edges = true;
                 }
             }
+            D.fx_base      = fx_total;
+            D.fx_pool_base = int64_t(pool_slots) * FX_SLOTS * G * NC;
+            if (D.estruct >= 0) {
+                fx_total += soffs[size_t(D.estruct)].num_edges;
+                ++pool_slots;
+            }
             D.t         = r.cfg.t0;
             D.dt_graph  = I.dt_graph;
             D.tmax      = r.cfg.t0;
@@ -600,6 +711,8 @@ struct Batch {
             D.day0      = I.day0;
             D.ndays     = I.ndays;
             D.exact_end = r.single ? 1 : 0;
+            D.fx_sequential = sequential_fixup ? 1 : 0;
+            D.seg           = 0;
             I.out_base  = out_total;
             out_total += int64_t(I.ndays) * I.num_nodes * G * NC;
         }
@@ -660,6 +773,11 @@ struct Batch {
         }
         up(C.estructs, estructs);
         up(C.inst_ids, inst_ids);
+        for (int k = 0; k < 2; ++k) {
+            C.fx[k].alloc(size_t(std::max<int64_t>(fx_total, 1)) * sizeof(int32_t));
+            C.fx_pool[k].alloc(size_t(std::max<int64_t>(pool_slots, 1)) * FX_SLOTS * G * NC * sizeof(double));
+            device_bytes += C.fx[k].bytes + C.fx_pool[k].bytes;
+        }
         for (int ii : C.insts) {
             idev[size_t(ii)].out = C.out.as<double>() + inst[size_t(ii)].out_base;
         }
@@ -673,12 +791,9 @@ struct Batch {
                           "day0 copy");
             }
         }
-        ClassDev& d = C.dev;
+        SegDev d{};
         d.G         = G;
-        d.kind      = C.kind;
         d.num_nodes = C.num_nodes;
-        d.num_inst  = int(C.insts.size());
-        d.inst_ids  = C.inst_ids.as<int32_t>();
         d.y         = C.y.as<double>();
         d.snap      = C.snap.as<double>();
         d.ypre      = C.ypre.as<double>();
@@ -693,6 +808,26 @@ struct Batch {
         d.steps     = C.steps.as<uint32_t>();
         d.attempts  = C.attempts.as<uint32_t>();
         d.estructs  = C.estructs.as<EdgeStructDev>();
+        for (int k = 0; k < 2; ++k) {
+            d.fx[k]      = C.fx[k].as<int32_t>();
+            d.fx_pool[k] = C.fx_pool[k].as<double>();
+        }
+        C.seg_host = d;
+        std::vector<SegDev> segs{d};
+        std::vector<int32_t> bseg, bfirst;
+        block_tables({C.num_nodes}, G, bseg, bfirst);
+        up(C.seg_dev, segs);
+        up(C.block_seg, bseg);
+        up(C.block_first, bfirst);
+        LaunchDev& ld  = C.launch;
+        ld.segs        = C.seg_dev.as<SegDev>();
+        ld.block_seg   = C.block_seg.as<int32_t>();
+        ld.block_first = C.block_first.as<int32_t>();
+        ld.inst_ids    = C.inst_ids.as<int32_t>();
+        ld.num_inst    = int(C.insts.size());
+        ld.num_blocks  = int(bseg.size());
+        ld.G           = G;
+        ld.kind        = C.kind;
     }
 
     // enqueue one half-day cycle for every instance; with ev != nullptr, events bracket the node and the
@@ -705,7 +840,7 @@ struct Batch {
             hip_check(hipEventRecord(ev[0], stream), "event");
         }
         for (auto& C : classes) {
-            hip_check(launch_node(C.dev, id, stream), "node kernel");
+            hip_check(launch_node(C.launch, id, stream), "node kernel");
         }
         if (ev) {
             hip_check(hipEventRecord(ev[1], stream), "event");
@@ -717,7 +852,7 @@ struct Batch {
                 any_edges = any_edges || inst[size_t(ii)].has_edges;
             }
             if (any_edges) {
-                hip_check(launch_edge(C.dev, id, stream), "edge kernel");
+                hip_check(launch_edge(C.launch, id, stream), "edge kernel");
             }
         }
         if (ev) {
@@ -1107,6 +1242,7 @@ mgo_status mgo_graph_create(mgo_context* ctx, const mgo_graph_desc* graph, const
         auto b  = std::make_unique<Batch>();
         b->use_graph = !(ctx->flags & MGO_CTX_NO_GRAPH_CAPTURE);
         b->profile   = (ctx->flags & MGO_CTX_PROFILE_KERNELS) != 0;
+        b->sequential_fixup = (ctx->flags & MGO_CTX_SEQUENTIAL_FIXUP) != 0;
         b->build(runs, ctx->stream);
         *out = ctx->next_id++;
         ctx->graphs[*out] = std::move(b);
@@ -1226,6 +1362,7 @@ static void build_ensemble(mgo_context* ctx, const mgo_ensemble_desc* desc, Batc
     }
     b.use_graph = !(ctx->flags & MGO_CTX_NO_GRAPH_CAPTURE);
     b.profile   = (ctx->flags & MGO_CTX_PROFILE_KERNELS) != 0;
+    b.sequential_fixup = (ctx->flags & MGO_CTX_SEQUENTIAL_FIXUP) != 0;
     b.build(runs, ctx->stream);
     for (int r = 0; r < desc->num_runs; ++r) {
         b.inst[size_t(r)].tmax = desc->tmax[r];
@@ -1257,6 +1394,7 @@ static void fill_info(Batch& b, mgo_run_info* info, double ms, int64_t cycles)
     auto v = b.read_inst();
     for (auto& I : v) {
         info->exact_replays += I.replays;
+        info->fixup_passes += I.fx_passes;
     }
     info->half_day_cycles = int32_t(cycles);
     info->node_launches   = b.node_launches;

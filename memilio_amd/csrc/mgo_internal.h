@@ -1,8 +1,10 @@
 // mgo_internal.h - data layout shared by the host engine (mgo_host.cpp) and the HIP kernels
 // (mgo_kernels.hip). Not part of the public ABI.
 //
-// Layout in HBM (one "class" = all nodes of the resident instances that share the number of age
-// groups G and the integrator kind; kernels are specialised per class):
+// Layout in HBM. A *segment* is a contiguous node range with its own buffers: one whole batch class
+// (all runs with the same number of age groups G and integrator kind) or one served request. A
+// launch covers the segments of one class through a block -> segment table; kernels are specialised
+// per (G, integrator). Within a segment:
 //   lane slot   = node * G + g   (one GPU lane per (node, age group); 64/G nodes per wavefront)
 //   y, snap, ypre       [node][g][10]          state, departure snapshot, pre-return state
 //   lcoef               [node][g][NUM_LC]      per-lane rate coefficients (see LaneCoef)
@@ -114,10 +116,24 @@ struct InstDev {
     int32_t c_active;
     int32_t c_action; // 0 none, 1 departure, 2 return
     int32_t c_out_day;
-    int32_t c_replay; // set by the return kernel if the negative-value fix-up is needed
-    int32_t replays;  // number of exact replays so far
+    int32_t replays;   // return phases resolved by the sequential replay so far
     int32_t exact_end; // single-model run: the last step ends exactly at tmax
+    // negative-value fix-up iteration of the current return phase (see return_walk)
+    int64_t fx_base;      // first in-edge record of this instance in ClassDev::fx[*]
+    int64_t fx_pool_base; // first slot of this instance in ClassDev::fx_pool[*] (doubles)
+    int32_t fx_pass;      // pass being run (0 = the first return walk)
+    int32_t fx_state;     // 0 first pass, 1 iterating, 2 sequential replay needed, 3 settled
+    int32_t fx_changed;   // set by a pass whose fix-up records differ from the previous pass
+    int32_t fx_count[2];  // slots used by the records of the even / odd pass
+    int32_t fx_passes;    // extra passes run so far (statistics)
+    int32_t fx_sequential; // resolve fix-ups with the sequential replay only
+    int32_t seg;          // segment (index into LaunchDev::segs of its class) holding the nodes
+    int32_t pad2;
 };
+
+// fix-up records: slots per instance and pass parity; passes before falling back to the sequential replay
+constexpr int FX_SLOTS      = 512;
+constexpr int FX_MAX_PASSES = 4;
 
 // CSR edge structure of one graph (node indices local to the instance). In-edges of node v are sorted
 // by source; in_split[v] = number of in-edges whose source < v, so walking
@@ -128,6 +144,8 @@ struct EdgeEnt {
     double cval;      // coefficient if the edge's vector is uniform
     int32_t nbr;      // neighbour (local index)
     int32_t cidx;     // -1 uniform, else index of the [10*G] vector in cvec
+    int32_t peer;     // out-entry: index of the same edge among the in-entries (its fix-up record)
+    int32_t pad;
 };
 
 struct EdgeStructDev {
@@ -142,12 +160,9 @@ struct EdgeStructDev {
     int32_t num_edges;
 };
 
-struct ClassDev {
+struct SegDev {
     int32_t G;
-    int32_t kind;
     int32_t num_nodes;
-    int32_t num_inst;
-    const int32_t* inst_ids; // instances of this class (global instance indices)
     double* y;
     double* snap;
     double* ypre;
@@ -162,6 +177,20 @@ struct ClassDev {
     uint32_t* steps;
     uint32_t* attempts;
     const EdgeStructDev* estructs; // indexed by InstDev::estruct
+    int32_t* fx[2];                // fix-up record per in-edge entry: slot or -1, even / odd pass
+    double* fx_pool[2];            // fixed traveler vectors [slot][G][10], even / odd pass
+};
+
+// One kernel launch over the segments of a class.
+struct LaunchDev {
+    const SegDev* segs;
+    const int32_t* block_seg;   // [num_blocks] segment of each (XCD-remapped) block, -1 = padding
+    const int32_t* block_first; // [num segments] first block of each segment
+    const int32_t* inst_ids;    // instances of this class
+    int32_t num_inst;
+    int32_t num_blocks;         // launched blocks (multiple of 8)
+    int32_t G;
+    int32_t kind;
 };
 
 } // namespace mgo

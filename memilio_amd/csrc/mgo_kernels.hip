@@ -104,7 +104,7 @@ struct Lane {
 
 // Effective contact row C(t)_age,. = B - D(t) o (B - M) (contact_matrix.h:190-194, damping.h:381-396)
 template <int G>
-__device__ __forceinline__ void contact_row(const ClassDev& c, Lane<G>& L, double t)
+__device__ __forceinline__ void contact_row(const SegDev& c, Lane<G>& L, double t)
 {
     const double* B = c.cbase + (size_t(L.node) * G + L.age) * G;
     if (!(L.np.flags & NF_DAMPED)) {
@@ -138,7 +138,7 @@ __device__ __forceinline__ void contact_row(const ClassDev& c, Lane<G>& L, doubl
 // w_j = divN_j * (rel_j pop[INS_j] + risk_j pop[ISy_j]) (model.h:99-145, factored over j). Also returns
 // the ICU-adjusted ISev->ICr / ISev->D coefficients (model.h:147-152, 196-204).
 template <int G>
-__device__ __forceinline__ double force_of_infection(const ClassDev& c, Lane<G>& L, const double (&pop)[NC], double t,
+__device__ __forceinline__ double force_of_infection(const SegDev& c, Lane<G>& L, const double (&pop)[NC], double t,
                                                      double& c_icr, double& c_d)
 {
     const double season = 1.0 + L.np.seasonality * sin(MGO_PI * ((L.np.start_day + t) / 182.5 + 0.5));
@@ -204,7 +204,7 @@ __device__ __forceinline__ void flows_to_dydt(const double (&y)[NC], double kapp
 
 // Right-hand side of the node ODE, pop == y (osecir::Model::get_flows + FlowModel::get_derivatives)
 template <int G>
-__device__ __forceinline__ void rhs(const ClassDev& c, Lane<G>& L, const double (&y)[NC], double t, double (&dy)[NC])
+__device__ __forceinline__ void rhs(const SegDev& c, Lane<G>& L, const double (&y)[NC], double t, double (&dy)[NC])
 {
     double c_icr, c_d;
     const double kappa = force_of_infection<G>(c, L, y, t, c_icr, c_d);
@@ -230,7 +230,7 @@ constexpr double E1 = 37.0 / 378.0 - 2825.0 / 27648.0, E3 = 250.0 / 621.0 - 1857
 
 // One Cash-Karp step from (y, k1 = f(y, t)) with step h: 5th-order solution in yo, embedded error in er.
 template <int G>
-__device__ __forceinline__ void cash_karp_stages(const ClassDev& c, Lane<G>& L, const double (&y)[NC],
+__device__ __forceinline__ void cash_karp_stages(const SegDev& c, Lane<G>& L, const double (&y)[NC],
                                                  const double (&k1)[NC], double t, double h, double (&yo)[NC],
                                                  double (&er)[NC], bool want_err)
 {
@@ -291,7 +291,7 @@ constexpr double D1 = BH1 - BL1, D3 = BH3 - BL3, D4 = BH4 - BL4, D5 = BH5 - BL5,
 } // namespace rkf
 
 template <int G>
-__device__ __forceinline__ void rkf45_stages(const ClassDev& c, Lane<G>& L, const double (&y)[NC],
+__device__ __forceinline__ void rkf45_stages(const SegDev& c, Lane<G>& L, const double (&y)[NC],
                                              const double (&k1)[NC], double t, double h, double (&yo)[NC],
                                              double (&er)[NC])
 {
@@ -333,31 +333,39 @@ __device__ __forceinline__ void rkf45_stages(const ClassDev& c, Lane<G>& L, cons
 }
 
 template <int G>
-__device__ __forceinline__ bool load_lane(const ClassDev& c, Lane<G>& L, int node_count)
+__device__ __forceinline__ const SegDev* load_lane(const LaunchDev& ld, Lane<G>& L)
 {
     constexpr int NPW = WAVE / G;
-    const int gtid    = xcd_block() * BLOCK + int(threadIdx.x);
-    const int wave    = gtid / WAVE;
-    const int lane    = gtid % WAVE;
-    const int grp     = lane / G;
-    L.age             = lane - grp * G;
-    L.node            = wave * NPW + grp;
-    L.base            = grp * G; // lane index (within the wavefront) of the group's first lane
-    if (grp >= NPW || L.node >= node_count) {
-        return false;
+    const int lb      = xcd_block();
+    if (lb >= ld.num_blocks) {
+        return nullptr;
     }
-    L.np            = c.nodep[L.node];
-    const double* p = c.lcoef + (size_t(L.node) * G + L.age) * NUM_LC;
+    const int seg = ld.block_seg[lb];
+    if (seg < 0) {
+        return nullptr;
+    }
+    const SegDev* c = ld.segs + seg;
+    const int wave  = (lb - ld.block_first[seg]) * (BLOCK / WAVE) + int(threadIdx.x) / WAVE;
+    const int lane  = int(threadIdx.x) % WAVE;
+    const int grp   = lane / G;
+    L.age           = lane - grp * G;
+    L.node          = wave * NPW + grp;
+    L.base          = grp * G; // lane index (within the wavefront) of the group's first lane
+    if (grp >= NPW || L.node >= c->num_nodes) {
+        return nullptr;
+    }
+    L.np            = c->nodep[L.node];
+    const double* p = c->lcoef + (size_t(L.node) * G + L.age) * NUM_LC;
 #pragma unroll
     for (int k = 0; k < NUM_LC; ++k) {
         L.c[k] = p[k];
     }
-    const double* B = c.cbase + (size_t(L.node) * G + L.age) * G;
+    const double* B = c->cbase + (size_t(L.node) * G + L.age) * G;
 #pragma unroll
     for (int j = 0; j < G; ++j) {
         L.crow[j] = B[j];
     }
-    return true;
+    return c;
 }
 
 // mobility factor of the ISy compartment of the lane's age group (model.h:687-715)
@@ -384,7 +392,11 @@ __global__ void clock_kernel(InstDev* inst, int n)
     I.c_active  = 0;
     I.c_action  = 0;
     I.c_out_day = -1;
-    I.c_replay  = 0;
+    I.fx_pass   = 0;
+    I.fx_state  = 0;
+    I.fx_changed  = 0;
+    I.fx_count[0] = 0;
+    I.fx_count[1] = 0;
     if (I.status != 0 || !(I.t < I.tmax)) {
         return;
     }
@@ -433,7 +445,7 @@ __global__ void clock_kernel(InstDev* inst, int n)
     }
 }
 
-__device__ __forceinline__ double* out_ptr(const ClassDev& c, const InstDev& I, int node, int age, int G, int day)
+__device__ __forceinline__ double* out_ptr(const SegDev& c, const InstDev& I, int node, int age, int G, int day)
 {
     return I.out + (size_t(day) * I.num_nodes + (node - I.node_base)) * size_t(G * NC) + size_t(age) * NC;
 }
@@ -447,12 +459,14 @@ __device__ __forceinline__ double* out_ptr(const ClassDev& c, const InstDev& I, 
 constexpr uint32_t kMaxAttemptsPerPhase = 1u << 20;
 
 template <int G, int K>
-__global__ void __launch_bounds__(BLOCK) node_phase(ClassDev c, InstDev* __restrict__ inst)
+__global__ void __launch_bounds__(BLOCK) node_phase(LaunchDev ld, InstDev* __restrict__ inst)
 {
     Lane<G> L;
-    if (!load_lane<G>(c, L, c.num_nodes)) {
+    const SegDev* cp = load_lane<G>(ld, L);
+    if (!cp) {
         return;
     }
+    const SegDev& c = *cp;
     InstDev& I = inst[L.np.inst];
     if (!I.c_active) {
         return;
@@ -693,16 +707,59 @@ __device__ __forceinline__ void return_euler(const double* rc, double h, double 
 }
 
 template <int G>
-__global__ void __launch_bounds__(BLOCK) edge_phase(ClassDev c, InstDev* __restrict__ inst)
+__device__ __forceinline__ bool group_any(bool v, int base)
 {
-    Lane<G> L;
-    if (!load_lane<G>(c, L, c.num_nodes)) {
-        return;
+    int s = __shfl(int(v), base);
+#pragma unroll
+    for (int j = 1; j < G; ++j) {
+        s |= __shfl(int(v), base + j);
     }
-    InstDev& I = inst[L.np.inst];
-    if (!I.c_active || I.c_action == 0) {
-        return;
+    return s != 0;
+}
+
+// MobilityEdge::apply_mobility return branch fix-up (metapopulation_mobility_instant.h:576-596): every
+// compartment of the age group that would go negative at the destination is taken from the group's
+// largest remaining compartment instead (remaining computed once, first maximum wins).
+__device__ __forceinline__ bool fixup_negative(const double (&y)[NC], double (&m)[NC])
+{
+    double rem[NC];
+    bool neg = false;
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+        rem[i] = y[i] - m[i];
+        neg    = neg || (rem[i] < 0);
     }
+    if (neg) {
+        int mx = 0;
+#pragma unroll
+        for (int q = 1; q < NC; ++q) {
+            if (rem[q] > rem[mx]) {
+                mx = q;
+            }
+        }
+        for (int j = 0; j < NC; ++j) {
+            if (rem[j] < 0) {
+                m[mx] -= rem[j];
+                m[j] += rem[j];
+            }
+        }
+    }
+    return neg;
+}
+
+// Return walk of one node (returns at c_t1 of the travelers that left at c_tdep,
+// metapopulation_mobility_instant.h:568-603), starting from the pre-return state and visiting the node's
+// edges in the reference's global (start, end) order.
+//
+// The fix-up of edge (s, d) depends on d's state at that point of the sequence, which includes d's
+// earlier out-edges (d, x) with their own fix-ups decided at x. Pass p applies the fix-ups of the node's
+// in-edges locally and takes the out-edge vectors from the records of pass p-1 (pass 0: unfixed). The
+// dependencies follow the global edge order (a DAG), so the records reach the reference's sequential
+// values after at most (longest dependency chain + 1) passes; a pass whose records equal the previous
+// ones reproduces the sequential result exactly.
+template <int G>
+__device__ __forceinline__ void return_walk(const SegDev& c, Lane<G>& L, InstDev& I, int pass)
+{
     const EdgeStructDev& es = c.estructs[I.estruct];
     const int v             = L.node - I.node_base;
     const int in_b = es.in_ptr[v], in_e = es.in_ptr[v + 1];
@@ -711,122 +768,115 @@ __global__ void __launch_bounds__(BLOCK) edge_phase(ClassDev c, InstDev* __restr
     const EdgeEnt* ine      = es.in_ent;
     const EdgeEnt* oute     = es.out_ent;
     const double* cvec      = es.cvec;
-    const size_t lane_off      = (size_t(L.node) * G + L.age);
-    double* yp                 = c.y + lane_off * NC;
+    const size_t lane_off   = (size_t(L.node) * G + L.age);
+    const int cur = pass & 1, prv = cur ^ 1;
+    int32_t* fxc            = c.fx[cur] + I.fx_base;
+    const int32_t* fxp      = c.fx[prv] + I.fx_base;
+    double* poolc           = c.fx_pool[cur] + I.fx_pool_base;
+    const double* poolp     = c.fx_pool[prv] + I.fx_pool_base;
+
     double y[NC];
 #pragma unroll
     for (int i = 0; i < NC; ++i) {
-        y[i] = yp[i];
+        y[i] = c.ypre[lane_off * NC + i];
     }
-    const double* snap_v = c.snap + lane_off * NC;
     double sv[NC];
 #pragma unroll
     for (int i = 0; i < NC; ++i) {
-        sv[i] = snap_v[i];
+        sv[i] = c.snap[lane_off * NC + i];
     }
     const double fac_v = c.fac[lane_off];
-
-    if (I.c_action == 1) {
-        // departures at t = c_t1 (:605-621): to.y += m, from.y -= m, edges in (start, end) order
-        const double td   = I.c_t1;
-        const double nd_v = (L.np.flags & NF_DETECTION) ? detect_nondetection(L.np, td) : 1.0;
-        auto in_edge      = [&](int k) {
-            const EdgeEnt e = ine[k];
-            const int s     = I.node_base + e.nbr;
-            const NodeP& ns = c.nodep[s];
-            const double nd = (ns.flags & NF_DETECTION) ? detect_nondetection(ns, td) : 1.0;
-            double m[NC];
-            traveler_vector(cvec, e, c.snap + (size_t(s) * G + L.age) * NC, c.fac[size_t(s) * G + L.age], nd, G, L.age, m);
+    const double td    = I.c_tdep;
+    const double h     = I.c_step;
+    const double nd_v  = (L.np.flags & NF_DETECTION) ? detect_nondetection(L.np, td) : 1.0;
+    double rcl[RC_STRIDE];
 #pragma unroll
-            for (int i = 0; i < NC; ++i) {
-                y[i] += m[i];
-            }
-        };
-        for (int k = in_b; k < in_s; ++k) {
-            in_edge(k);
-        }
-        for (int k = out_b; k < out_e; ++k) {
-            double m[NC];
-            traveler_vector(cvec, oute[k], sv, fac_v, 1.0, G, L.age, m);
-            if (nd_v != 1.0) {
-                y[INS] -= m[INS] * (1 - nd_v);
-                y[INSC] += m[INS] * (1 - nd_v);
-                y[ISY] -= m[ISY] * (1 - nd_v);
-                y[ISYC] += m[ISY] * (1 - nd_v);
-                m[ISY] *= nd_v;
-                m[INS] *= nd_v;
-            }
-#pragma unroll
-            for (int i = 0; i < NC; ++i) {
-                y[i] -= m[i];
-            }
-        }
-        for (int k = in_s; k < in_e; ++k) {
-            in_edge(k);
-        }
-        // return coefficients of this node as a destination, from its post-exchange state at t_dep
-        double c_icr, c_d;
-        const double kappa = force_of_infection<G>(c, L, y, td, c_icr, c_d);
-        double* rc         = c.rc + lane_off * RC_STRIDE;
-        rc[RC_KAPPA]       = kappa;
-        rc[RC_E]           = L.c[LC_E];
-        rc[RC_INS_ISY]     = L.c[LC_INS_ISY];
-        rc[RC_INS_R]       = L.c[LC_INS_R];
-        rc[RC_ISY_ISEV]    = L.c[LC_ISY_ISEV];
-        rc[RC_ISY_R]       = L.c[LC_ISY_R];
-        rc[RC_ISEV_ICR]    = c_icr;
-        rc[RC_ISEV_R]      = L.c[LC_ISEV_R];
-        rc[RC_ISEV_D]      = c_d;
-        rc[RC_ICR_D]       = L.c[LC_ICR_D];
-        rc[RC_ICR_R]       = L.c[LC_ICR_R];
+    for (int k = 0; k < RC_STRIDE; ++k) {
+        rcl[k] = c.rc[lane_off * RC_STRIDE + k];
     }
-    else {
-        // returns at t = c_t1 of the travelers that left at c_tdep (:568-603)
-        const double td   = I.c_tdep;
-        const double h    = I.c_step;
-        const double nd_v = (L.np.flags & NF_DETECTION) ? detect_nondetection(L.np, td) : 1.0;
-        const double* rcv = c.rc + lane_off * RC_STRIDE;
-        double rcl[RC_STRIDE];
+    bool changed = false;
+    auto in_edge = [&](int k) {
+        const EdgeEnt e = ine[k];
+        const int s     = I.node_base + e.nbr;
+        const NodeP& ns = c.nodep[s];
+        const double nd = (ns.flags & NF_DETECTION) ? detect_nondetection(ns, td) : 1.0;
+        double m[NC];
+        traveler_vector(cvec, e, c.snap + (size_t(s) * G + L.age) * NC, c.fac[size_t(s) * G + L.age], nd, G, L.age, m);
+        return_euler(rcl, h, m);
+        const bool neg = fixup_negative(y, m);
+        const bool any = group_any<G>(neg, L.base);
+        int slot       = -1;
+        if (any) {
+            if (L.age == 0) {
+                slot = atomicAdd(&I.fx_count[cur], 1);
+            }
+            slot = __shfl(slot, L.base);
+            if (slot < FX_SLOTS) {
 #pragma unroll
-        for (int k = 0; k < RC_STRIDE; ++k) {
-            rcl[k] = rcv[k];
+                for (int i = 0; i < NC; ++i) {
+                    poolc[(size_t(slot) * G + L.age) * NC + i] = m[i];
+                }
+            }
+            else {
+                slot = -1;
+                if (L.age == 0) {
+                    I.fx_state = 2; // record pool full: resolve this phase with the sequential replay
+                }
+            }
         }
-        bool negative = false;
-        auto in_edge  = [&](int k) {
-            const EdgeEnt e = ine[k];
-            const int s     = I.node_base + e.nbr;
-            const NodeP& ns = c.nodep[s];
-            const double nd = (ns.flags & NF_DETECTION) ? detect_nondetection(ns, td) : 1.0;
-            double m[NC];
-            traveler_vector(cvec, e, c.snap + (size_t(s) * G + L.age) * NC, c.fac[size_t(s) * G + L.age], nd, G, L.age, m);
-            return_euler(rcl, h, m);
+        if (L.age == 0) {
+            fxc[k] = slot;
+        }
+        if (pass == 0) {
+            changed = changed || any;
+        }
+        else {
+            const int ps = fxp[k];
+            if ((ps >= 0) != any) {
+                changed = true;
+            }
+            else if (any) {
+#pragma unroll
+                for (int i = 0; i < NC; ++i) {
+                    changed = changed || (poolp[(size_t(ps) * G + L.age) * NC + i] != m[i]);
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < NC; ++i) {
+            y[i] -= m[i];
+        }
+    };
+    for (int k = in_b; k < in_s; ++k) {
+        in_edge(k);
+    }
+    for (int k = out_b; k < out_e; ++k) {
+        const EdgeEnt e = oute[k];
+        double m[NC];
+        const int ps = (pass > 0) ? fxp[e.peer] : -1;
+        if (ps >= 0) {
 #pragma unroll
             for (int i = 0; i < NC; ++i) {
-                negative = negative || (y[i] - m[i] < 0);
-                y[i] -= m[i];
+                m[i] = poolp[(size_t(ps) * G + L.age) * NC + i];
             }
-        };
-        for (int k = in_b; k < in_s; ++k) {
-            in_edge(k);
         }
-        for (int k = out_b; k < out_e; ++k) {
-            const EdgeEnt e = oute[k];
-            const int d     = I.node_base + e.nbr;
-            double m[NC];
+        else {
+            const int d = I.node_base + e.nbr;
             traveler_vector(cvec, e, sv, fac_v, nd_v, G, L.age, m);
             return_euler(c.rc + (size_t(d) * G + L.age) * RC_STRIDE, h, m);
+        }
 #pragma unroll
-            for (int i = 0; i < NC; ++i) {
-                y[i] += m[i];
-            }
-        }
-        for (int k = in_s; k < in_e; ++k) {
-            in_edge(k);
-        }
-        if (negative) {
-            I.c_replay = 1; // benign race: every writer stores 1
+        for (int i = 0; i < NC; ++i) {
+            y[i] += m[i];
         }
     }
+    for (int k = in_s; k < in_e; ++k) {
+        in_edge(k);
+    }
+    if (changed) {
+        I.fx_changed = 1; // every writer stores 1
+    }
+    double* yp = c.y + lane_off * NC;
 #pragma unroll
     for (int i = 0; i < NC; ++i) {
         yp[i] = y[i];
@@ -840,18 +890,166 @@ __global__ void __launch_bounds__(BLOCK) edge_phase(ClassDev c, InstDev* __restr
     }
 }
 
-// ------------------------------------------------------------------------------- exact replay
-// Sequential return pass of one instance in the reference's edge order, with the negative-value fix-up
-// (metapopulation_mobility_instant.h:568-603). Runs only for instances the optimistic return kernel
-// flagged; lanes = age groups.
 template <int G>
-__global__ void __launch_bounds__(WAVE) replay_phase(ClassDev c, InstDev* __restrict__ inst)
+__global__ void __launch_bounds__(BLOCK) edge_phase(LaunchDev ld, InstDev* __restrict__ inst)
 {
-    const int ii = c.inst_ids[blockIdx.x];
-    InstDev& I   = inst[ii];
-    if (!I.c_active || I.c_action != 2 || !I.c_replay) {
+    Lane<G> L;
+    const SegDev* cp = load_lane<G>(ld, L);
+    if (!cp) {
         return;
     }
+    const SegDev& c = *cp;
+    InstDev& I = inst[L.np.inst];
+    if (!I.c_active || I.c_action == 0) {
+        return;
+    }
+    if (I.c_action == 2) {
+        return_walk<G>(c, L, I, 0);
+        return;
+    }
+    const EdgeStructDev& es = c.estructs[I.estruct];
+    const int v             = L.node - I.node_base;
+    const int in_b = es.in_ptr[v], in_e = es.in_ptr[v + 1];
+    const int in_s          = in_b + es.split[v];
+    const int out_b = es.out_ptr[v], out_e = es.out_ptr[v + 1];
+    const EdgeEnt* ine      = es.in_ent;
+    const EdgeEnt* oute     = es.out_ent;
+    const double* cvec      = es.cvec;
+    const size_t lane_off   = (size_t(L.node) * G + L.age);
+    double* yp              = c.y + lane_off * NC;
+    double y[NC];
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+        y[i] = yp[i];
+    }
+    double sv[NC];
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+        sv[i] = c.snap[lane_off * NC + i];
+    }
+    const double fac_v = c.fac[lane_off];
+
+    // departures at t = c_t1 (:605-621): to.y += m, from.y -= m, edges in (start, end) order
+    const double td   = I.c_t1;
+    const double nd_v = (L.np.flags & NF_DETECTION) ? detect_nondetection(L.np, td) : 1.0;
+    auto in_edge      = [&](int k) {
+        const EdgeEnt e = ine[k];
+        const int s     = I.node_base + e.nbr;
+        const NodeP& ns = c.nodep[s];
+        const double nd = (ns.flags & NF_DETECTION) ? detect_nondetection(ns, td) : 1.0;
+        double m[NC];
+        traveler_vector(cvec, e, c.snap + (size_t(s) * G + L.age) * NC, c.fac[size_t(s) * G + L.age], nd, G, L.age, m);
+#pragma unroll
+        for (int i = 0; i < NC; ++i) {
+            y[i] += m[i];
+        }
+    };
+    for (int k = in_b; k < in_s; ++k) {
+        in_edge(k);
+    }
+    for (int k = out_b; k < out_e; ++k) {
+        double m[NC];
+        traveler_vector(cvec, oute[k], sv, fac_v, 1.0, G, L.age, m);
+        if (nd_v != 1.0) {
+            y[INS] -= m[INS] * (1 - nd_v);
+            y[INSC] += m[INS] * (1 - nd_v);
+            y[ISY] -= m[ISY] * (1 - nd_v);
+            y[ISYC] += m[ISY] * (1 - nd_v);
+            m[ISY] *= nd_v;
+            m[INS] *= nd_v;
+        }
+#pragma unroll
+        for (int i = 0; i < NC; ++i) {
+            y[i] -= m[i];
+        }
+    }
+    for (int k = in_s; k < in_e; ++k) {
+        in_edge(k);
+    }
+    // return coefficients of this node as a destination, from its post-exchange state at t_dep
+    double c_icr, c_d;
+    const double kappa = force_of_infection<G>(c, L, y, td, c_icr, c_d);
+    double* rc         = c.rc + lane_off * RC_STRIDE;
+    rc[RC_KAPPA]       = kappa;
+    rc[RC_E]           = L.c[LC_E];
+    rc[RC_INS_ISY]     = L.c[LC_INS_ISY];
+    rc[RC_INS_R]       = L.c[LC_INS_R];
+    rc[RC_ISY_ISEV]    = L.c[LC_ISY_ISEV];
+    rc[RC_ISY_R]       = L.c[LC_ISY_R];
+    rc[RC_ISEV_ICR]    = c_icr;
+    rc[RC_ISEV_R]      = L.c[LC_ISEV_R];
+    rc[RC_ISEV_D]      = c_d;
+    rc[RC_ICR_D]       = L.c[LC_ICR_D];
+    rc[RC_ICR_R]       = L.c[LC_ICR_R];
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+        yp[i] = y[i];
+    }
+    if (I.c_out_day >= 0) {
+        double* op = out_ptr(c, I, L.node, L.age, G, I.c_out_day);
+#pragma unroll
+        for (int i = 0; i < NC; ++i) {
+            op[i] = y[i];
+        }
+    }
+}
+
+// After a return pass: settle the instance, schedule another pass, or hand it to the sequential replay.
+__global__ void fixup_check(LaunchDev ld, InstDev* inst)
+{
+    const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= ld.num_inst) {
+        return;
+    }
+    InstDev& I = inst[ld.inst_ids[i]];
+    if (!I.c_active || I.c_action != 2 || I.fx_state >= 2) {
+        return;
+    }
+    if (!I.fx_changed) {
+        I.fx_state = 3;
+        return;
+    }
+    if (I.fx_sequential || I.fx_pass + 1 >= FX_MAX_PASSES) {
+        I.fx_state = 2;
+        return;
+    }
+    I.fx_state   = 1;
+    I.fx_pass    = I.fx_pass + 1;
+    I.fx_changed = 0;
+    I.fx_count[I.fx_pass & 1] = 0;
+    I.fx_passes += 1;
+}
+
+// One more return pass for the instances that are iterating.
+template <int G>
+__global__ void __launch_bounds__(BLOCK) return_pass(LaunchDev ld, InstDev* __restrict__ inst)
+{
+    Lane<G> L;
+    const SegDev* cp = load_lane<G>(ld, L);
+    if (!cp) {
+        return;
+    }
+    const SegDev& c = *cp;
+    InstDev& I = inst[L.np.inst];
+    if (!I.c_active || I.c_action != 2 || I.fx_state != 1) {
+        return;
+    }
+    return_walk<G>(c, L, I, I.fx_pass);
+}
+
+// ------------------------------------------------------------------------------- exact replay
+// Sequential return pass of one instance in the reference's edge order with the negative-value fix-up
+// (metapopulation_mobility_instant.h:568-603). Fallback for the rare return phases whose fix-up records
+// did not settle within FX_MAX_PASSES passes or overflowed the record pool; lanes = age groups.
+template <int G>
+__global__ void __launch_bounds__(WAVE) replay_phase(LaunchDev ld, InstDev* __restrict__ inst)
+{
+    const int ii = ld.inst_ids[blockIdx.x];
+    InstDev& I   = inst[ii];
+    if (!I.c_active || I.c_action != 2 || I.fx_state != 2) {
+        return;
+    }
+    const SegDev& c = ld.segs[I.seg];
     const int a = int(threadIdx.x);
     // restore the pre-return state of every node
     for (int v = 0; v < I.num_nodes; ++v) {
@@ -883,22 +1081,11 @@ __global__ void __launch_bounds__(WAVE) replay_phase(ClassDev c, InstDev* __rest
             return_euler(c.rc + doff * RC_STRIDE, h, m);
             double* yd = c.y + doff * NC;
             double* ys = c.y + soff * NC;
-            double rem[NC];
+            double yl[NC];
             for (int i = 0; i < NC; ++i) {
-                rem[i] = yd[i] - m[i];
+                yl[i] = yd[i];
             }
-            for (int j = 0; j < NC; ++j) {
-                if (rem[j] < 0) {
-                    int mx = 0;
-                    for (int q = 1; q < NC; ++q) {
-                        if (rem[q] > rem[mx]) {
-                            mx = q;
-                        }
-                    }
-                    m[mx] -= rem[j];
-                    m[j] += rem[j];
-                }
-            }
+            fixup_negative(yl, m);
             for (int i = 0; i < NC; ++i) {
                 ys[i] += m[i];
             }
@@ -910,6 +1097,7 @@ __global__ void __launch_bounds__(WAVE) replay_phase(ClassDev c, InstDev* __rest
     __syncthreads();
     if (a == 0) {
         I.replays += 1;
+        I.fx_state = 3;
     }
     if (I.c_out_day >= 0 && a < G) {
         for (int v = 0; v < I.num_nodes; ++v) {
@@ -923,18 +1111,11 @@ __global__ void __launch_bounds__(WAVE) replay_phase(ClassDev c, InstDev* __rest
 }
 
 // ------------------------------------------------------------------------------- launchers
-static int padded_blocks(int nodes, int G)
-{
-    const int npw    = WAVE / G;
-    const int waves  = (nodes + npw - 1) / npw;
-    const int blocks = (waves * WAVE + BLOCK - 1) / BLOCK;
-    return ((blocks + 7) / 8) * 8;
-}
 
 template <int G>
-static hipError_t launch_node_g(const ClassDev& c, InstDev* inst, hipStream_t s)
+static hipError_t launch_node_g(const LaunchDev& c, InstDev* inst, hipStream_t s)
 {
-    const int nb = padded_blocks(c.num_nodes, G);
+    const int nb = c.num_blocks;
     switch (c.kind) {
     case MGO_INTEGRATOR_CASH_KARP54:
         hipLaunchKernelGGL((node_phase<G, MGO_INTEGRATOR_CASH_KARP54>), dim3(nb), dim3(BLOCK), 0, s, c, inst);
@@ -953,15 +1134,24 @@ static hipError_t launch_node_g(const ClassDev& c, InstDev* inst, hipStream_t s)
 }
 
 template <int G>
-static hipError_t launch_edge_g(const ClassDev& c, InstDev* inst, hipStream_t s)
+static hipError_t launch_edge_g(const LaunchDev& c, InstDev* inst, hipStream_t s)
 {
-    const int nb = padded_blocks(c.num_nodes, G);
+    const int nb = c.num_blocks;
     hipLaunchKernelGGL((edge_phase<G>), dim3(nb), dim3(BLOCK), 0, s, c, inst);
     hipError_t err = hipGetLastError();
+    const int ni   = (c.num_inst + 255) / 256;
+    for (int p = 1; p < FX_MAX_PASSES && err == hipSuccess; ++p) {
+        hipLaunchKernelGGL(fixup_check, dim3(ni), dim3(256), 0, s, c, inst);
+        hipLaunchKernelGGL((return_pass<G>), dim3(nb), dim3(BLOCK), 0, s, c, inst);
+        err = hipGetLastError();
+    }
     if (err != hipSuccess) {
         return err;
     }
-    hipLaunchKernelGGL((replay_phase<G>), dim3(c.num_inst), dim3(WAVE), 0, s, c, inst);
+    hipLaunchKernelGGL(fixup_check, dim3(ni), dim3(256), 0, s, c, inst);
+    if (c.num_inst > 0) {
+        hipLaunchKernelGGL((replay_phase<G>), dim3(c.num_inst), dim3(WAVE), 0, s, c, inst);
+    }
     return hipGetLastError();
 }
 
@@ -971,7 +1161,7 @@ hipError_t launch_clock(InstDev* inst, int n, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_node(const ClassDev& c, InstDev* inst, hipStream_t s)
+hipError_t launch_node(const LaunchDev& c, InstDev* inst, hipStream_t s)
 {
     switch (c.G) {
     case 1:
@@ -995,7 +1185,7 @@ hipError_t launch_node(const ClassDev& c, InstDev* inst, hipStream_t s)
     }
 }
 
-hipError_t launch_edge(const ClassDev& c, InstDev* inst, hipStream_t s)
+hipError_t launch_edge(const LaunchDev& c, InstDev* inst, hipStream_t s)
 {
     switch (c.G) {
     case 1:

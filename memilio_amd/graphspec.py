@@ -58,7 +58,7 @@ class GraphSpec:
         self.populations = np.zeros((n, G, NC))
         self.edge_start = np.zeros(0, np.int32)
         self.edge_end = np.zeros(0, np.int32)
-        self.edge_coeffs = np.zeros((0, NC * G))
+        self.edge_coeffs = np.zeros(0)   # [E] (one value per edge) or [E][10G]
         self._desc = None
 
     # ---------------------------------------------------------------- building
@@ -82,13 +82,18 @@ class GraphSpec:
         self._desc = None
 
     def set_edges(self, start, end, coeffs):
+        """coeffs: [E] (one coefficient for every compartment of the edge) or [E][10G]."""
         self.edge_start = np.ascontiguousarray(start, dtype=np.int32)
         self.edge_end = np.ascontiguousarray(end, dtype=np.int32)
         c = np.asarray(coeffs, dtype=np.float64)
-        if c.ndim == 1:
-            c = np.repeat(c[:, None], NC * self.G, axis=1)
+        assert c.ndim == 1 or c.shape[1] == NC * self.G
         self.edge_coeffs = np.ascontiguousarray(c)
         self._desc = None
+
+    def edge_coeff_matrix(self):
+        """[E][10G] view of the coefficients."""
+        c = self.edge_coeffs
+        return np.repeat(c[:, None], NC * self.G, axis=1) if c.ndim == 1 else c
 
     @property
     def num_edges(self):
@@ -126,6 +131,7 @@ class GraphSpec:
             d.edge_start = capi.iptr(self.edge_start) if self.num_edges else None
             d.edge_end = capi.iptr(self.edge_end) if self.num_edges else None
             d.edge_coeffs = capi.dptr(self.edge_coeffs) if self.num_edges else None
+            d.edge_coeff_len = 1 if self.edge_coeffs.ndim == 1 else NC * self.G
             self._desc = d
         return self._desc
 

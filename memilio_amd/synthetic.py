@@ -25,62 +25,77 @@ TIMES = (capi.A_TIME_EXPOSED, capi.A_TIME_INFECTED_NO_SYMPTOMS, capi.A_TIME_INFE
          capi.A_TIME_INFECTED_SEVERE, capi.A_TIME_INFECTED_CRITICAL)
 
 
-def _node_params(spec, k, rng, exposed_fraction=1e-3):
-    """One model of the single-model workload: contacts U[0.5,10], stay times U[2,14], transmission
-    probability U[0.02,0.2], group populations U[1e5,1e6], the given fraction initially exposed."""
-    G = spec.G
-    sc = spec.node_scalars[k]
-    sc[capi.P_SEASONALITY] = 0.2
-    sc[capi.P_START_DAY] = 0.0
-    a = spec.age_params[k]
+def _node_params(spec, rng, exposed_fraction=1e-3):
+    """Every node of spec as one model of the single-model workload (each node its own draws):
+    contacts U[0.5,10], stay times U[2,14], transmission probability U[0.02,0.2], group populations
+    U[1e5,1e6], the given fraction initially exposed."""
+    n, G = spec.num_nodes, spec.G
+    spec.node_scalars[:, capi.P_SEASONALITY] = 0.2
+    spec.node_scalars[:, capi.P_START_DAY] = 0.0
+    a = spec.age_params
     for p, v in FIXED.items():
-        a[p, :] = v
+        a[:, p, :] = v
     for p in TIMES:
-        a[p, :] = rng.uniform(2.0, 14.0, G)
-    a[capi.A_TRANSMISSION_PROBABILITY_ON_CONTACT, :] = rng.uniform(0.02, 0.2, G)
-    spec.contact_baseline[k] = rng.uniform(0.5, 10.0, (G, G))
-    n = rng.uniform(1e5, 1e6, G)
-    spec.populations[k, :, :] = 0.0
-    spec.populations[k, :, E] = exposed_fraction * n
-    spec.populations[k, :, S] = n - spec.populations[k, :, E]
+        a[:, p, :] = rng.uniform(2.0, 14.0, (n, G))
+    a[:, capi.A_TRANSMISSION_PROBABILITY_ON_CONTACT, :] = rng.uniform(0.02, 0.2, (n, G))
+    spec.contact_baseline[:] = rng.uniform(0.5, 10.0, (n, G, G))
+    tot = rng.uniform(1e5, 1e6, (n, G))
+    spec.populations[:] = 0.0
+    spec.populations[:, :, E] = exposed_fraction * tot
+    spec.populations[:, :, S] = tot - spec.populations[:, :, E]
 
 
 def single_model(seed=0, G=6):
     """Plain single osecir model: G age groups, one node."""
     spec = GraphSpec(1, G)
-    _node_params(spec, 0, np.random.default_rng(seed))
+    _node_params(spec, np.random.default_rng(seed))
     return spec
 
 
 def random_edges(rng, n, mean_degree=50.0, total_outflow=0.3):
-    """Directed random graph: Poisson(mean_degree) out-degree (capped at n-1), targets without
-    replacement, lognormal(-5, 1) commuter fractions rescaled so each node's outflow is <= total_outflow."""
-    starts, ends, coeffs = [], [], []
-    for s in range(n):
-        d = min(int(rng.poisson(mean_degree)), n - 1)
-        if d <= 0:
-            continue
-        others = np.delete(np.arange(n), s)
-        tg = np.sort(rng.choice(others, size=d, replace=False))
-        w = rng.lognormal(-5.0, 1.0, d)
-        tot = w.sum()
-        if tot > total_outflow:
-            w = w * (total_outflow / tot)
-        starts.append(np.full(d, s))
-        ends.append(tg)
-        coeffs.append(w)
-    if not starts:
+    """Directed random graph: Poisson(mean_degree) out-degree (capped at n-1), targets drawn without
+    replacement among the other nodes, lognormal(-5, 1) commuter fractions rescaled so each node's
+    outflow is <= total_outflow. Vectorised over all sources: draw with replacement, drop repeats,
+    redraw the missing targets until every source has its degree."""
+    deg = np.minimum(rng.poisson(mean_degree, n), n - 1).astype(np.int64)
+    if n <= 1 or deg.sum() == 0:
         return np.zeros(0, np.int32), np.zeros(0, np.int32), np.zeros(0)
-    return (np.concatenate(starts).astype(np.int32), np.concatenate(ends).astype(np.int32),
-            np.concatenate(coeffs))
+    if n <= 128:
+        # small graphs: a random order of the other nodes per source
+        keys = rng.random((n, n))
+        keys[np.arange(n), np.arange(n)] = np.inf
+        order = np.argsort(keys, axis=1)
+        src = np.repeat(np.arange(n), deg)
+        pos = np.concatenate([np.arange(d) for d in deg])
+        pairs = np.unique(src * n + order[src, pos])
+    else:
+        pairs = np.zeros(0, np.int64)
+        need = deg.copy()
+        while need.sum() > 0:
+            src = np.repeat(np.arange(n), need)
+            t = rng.integers(0, n - 1, src.size)
+            t = t + (t >= src)
+            pairs = np.unique(np.concatenate([pairs, src * n + t]))
+            have = np.bincount(pairs // n, minlength=n)
+            need = np.maximum(deg - have, 0)
+        # sources may have gained a few extra targets in the last round: keep the first deg per source
+        s_all = pairs // n
+        first = np.searchsorted(s_all, np.arange(n))
+        rank = np.arange(pairs.size) - first[s_all]
+        pairs = pairs[rank < deg[s_all]]
+    starts = (pairs // n).astype(np.int32)
+    ends = (pairs % n).astype(np.int32)
+    w = rng.lognormal(-5.0, 1.0, pairs.size)
+    tot = np.bincount(starts, weights=w, minlength=n)
+    scale = np.where(tot > total_outflow, total_outflow / np.maximum(tot, 1e-300), 1.0)
+    return starts, ends, w * scale[starts]
 
 
 def graph(num_nodes=400, G=6, seed=1, mean_degree=50.0):
     """Graph-ODE workload: num_nodes single-model nodes with their own seeds, random mobility edges."""
     spec = GraphSpec(num_nodes, G)
-    for k in range(num_nodes):
-        _node_params(spec, k, np.random.default_rng([seed, k]))
-    s, e, c = random_edges(np.random.default_rng([seed, 10 ** 6]), num_nodes, mean_degree)
+    _node_params(spec, np.random.default_rng([seed, 0]))
+    s, e, c = random_edges(np.random.default_rng([seed, 1]), num_nodes, mean_degree)
     spec.set_edges(s, e, c)
     return spec
 
@@ -117,10 +132,9 @@ def uneven(num_instances=4096, seed=3, max_nodes=2000):
         n = max(1, min(max_nodes, n))
         G = int(rng.integers(1, 9))
         spec = GraphSpec(n, G)
-        for k in range(n):
-            _node_params(spec, k, np.random.default_rng([seed, i, k]))
+        _node_params(spec, np.random.default_rng([seed, i, 0]))
         if n > 1:
-            s, e, c = random_edges(np.random.default_rng([seed, i, 10 ** 6]), n, min(50.0, n - 1))
+            s, e, c = random_edges(np.random.default_rng([seed, i, 1]), n, min(50.0, n - 1))
             spec.set_edges(s, e, c)
         horizon = float(rng.integers(30, 181))
         tol = float(rng.choice([1e-6, 1e-8, 1e-10]))

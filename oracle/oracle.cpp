@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <functional>
 #include <limits>
@@ -959,6 +960,10 @@ struct MobilityEdge {
                         max_index += group * nc;
                         m[max_index] -= remaining[j];
                         m[j] += remaining[j];
+#ifdef ORC_TRACE_FIXUPS
+                        fprintf(stderr, "fixup t=%g comp=%d group=%d rem=%.3e to_last=%.3e\n", t, j % nc, group,
+                                remaining[j], to_last[j]);
+#endif
                         if (st) {
                             st->negative_fixups++;
                         }
@@ -1169,9 +1174,12 @@ static void build_graph(const mgo_graph_desc* g, const mgo_sim_cfg* cfg, GraphSi
         build_osecir(g, k, m);
         sim.nodes.emplace_back(m, make_core(cfg->integrator), cfg->t0, cfg->dt_node);
     }
-    const size_t n = size_t(g->num_groups) * NC;
+    const size_t n   = size_t(g->num_groups) * NC;
+    const size_t len = (g->edge_coeff_len == 1) ? 1 : n;
     for (int e = 0; e < g->num_edges; ++e) {
-        Vec c(g->edge_coeffs + size_t(e) * n, g->edge_coeffs + size_t(e + 1) * n);
+        // MobilityEdge(Eigen::VectorX) coefficients; a single value stands for VectorX::Constant(n, c)
+        Vec c = (len == 1) ? Vec(n, g->edge_coeffs[e])
+                           : Vec(g->edge_coeffs + size_t(e) * n, g->edge_coeffs + size_t(e + 1) * n);
         sim.add_edge(g->edge_start[e], g->edge_end[e], c);
     }
 }

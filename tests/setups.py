@@ -146,3 +146,20 @@ def mobility_pair_model():
     a[capi.A_TIME_INFECTED_NO_SYMPTOMS] = 1.0
     spec.contact_baseline[0, 0, 0] = 5.0
     return spec
+
+
+def fixup_graph(seed=4, n=30, G=3):
+    """Heavily commuting graph (outflow 0.6 per node, low ISy mobility, high initial exposure) on which the
+    reference's return fix-up (metapopulation_mobility_instant.h:576-596) triggers within 20 days."""
+    from memilio_amd import synthetic
+    sp = synthetic.graph(num_nodes=n, G=G, seed=100 + seed, mean_degree=8.0)
+    c = sp.edge_coeffs.copy()
+    for s in range(sp.num_nodes):
+        m = sp.edge_start == s
+        if m.any():
+            c[m] *= 0.6 / c[m].sum()
+    sp.set_edges(sp.edge_start, sp.edge_end, c)
+    rng = np.random.default_rng(seed)
+    sp.populations[:, :, E] *= rng.uniform(1, 30, (n, G))
+    sp.age_params[:, capi.A_RISK_OF_INFECTION_FROM_SYMPTOMATIC, :] = 0.05
+    return sp

@@ -148,7 +148,7 @@ def test_graph_all_features(ctx, oracle):
     spec.contact_minimum = 0.2 * spec.contact_baseline
     spec.set_dampings([[(4.0, 0, 0, 0.5), (6.5, 1, 0, rng.uniform(0, 0.4, (3, 3))), (9.0, 0, 0, 0.2),
                         (9.0, 0, 1, 0.1)] if k % 2 == 0 else [] for k in range(spec.num_nodes)])
-    ec = np.repeat(spec.edge_coeffs[:, :1], 30, axis=1) * rng.uniform(0.5, 1.5, (spec.num_edges, 30))
+    ec = spec.edge_coeff_matrix() * rng.uniform(0.5, 1.5, (spec.num_edges, 30))
     spec.set_edges(spec.edge_start, spec.edge_end, ec)
     # a node with initial infections in every compartment so the smoothers switch on
     spec.populations[:, :, 2:9] = rng.uniform(0, 50, (spec.num_nodes, 3, 7))
@@ -189,6 +189,30 @@ def test_shared_structure_ensemble(ctx, oracle):
     for k in range(5):
         ref, _ = oracle.graph_run(specs[k], cfgs[k], 10.0)
         assert rel_err(out[k], ref) < 1e-12
+
+
+@pytest.mark.parametrize("kind", ["fixed", "adaptive"])
+def test_graph_negative_fixups(ctx, oracle, kind):
+    """Return phases with the negative-value fix-up (parallel passes until the fix-up records settle):
+    fixed dt to 1e-12, adaptive to 1e-6, and the fix-ups did occur."""
+    spec = setups.fixup_graph()
+    cfg = synthetic.fixed_cfg(0.1) if kind == "fixed" else synthetic.ck_cfg(1e-10)
+    out, info = ctx.ensemble_run([spec], [cfg], [20.0])
+    ref, st = oracle.graph_run(spec, cfg, 20.0)
+    assert st.negative_fixups > 0
+    assert info.fixup_passes > 0
+    assert rel_err(out[0], ref) < (1e-12 if kind == "fixed" else 1e-6)
+
+
+def test_graph_negative_fixups_sequential(oracle):
+    """The sequential replay (fallback of the parallel passes) gives the same result."""
+    spec = setups.fixup_graph()
+    cfg = synthetic.fixed_cfg(0.1)
+    with Context(flags=capi.CTX_SEQUENTIAL_FIXUP) as c2:
+        out, info = c2.ensemble_run([spec], [cfg], [20.0])
+    ref, st = oracle.graph_run(spec, cfg, 20.0)
+    assert info.exact_replays > 0
+    assert rel_err(out[0], ref) < 1e-12
 
 
 def test_deterministic(ctx):
