@@ -3,10 +3,17 @@
 
 Headline: node-days/s of the batched ensemble (1024 parameter-study samples of a 400-node, G=6 graph,
 100 days, adaptive Cash-Karp abs = rel = 1e-10) with inputs resident in HBM; one "step" is one full
-ensemble run (ParameterStudy::run over all samples). Next to it: the plain single-graph GPU path on the
-400-node graph, the per-kernel roofline of the node-phase kernel (HIP events on the library's stream),
-the maximum relative error against the oracle on the CPU-baseline sample, and the CPU baseline (the
-oracle, a restatement of the reference algorithm, timed on this host's cores on a bounded sample).
+ensemble run (ParameterStudy::run over all samples). The timed steps launch every kernel directly with
+HIP events around each node-phase and edge-phase launch on the library's stream, so the node kernel's
+average launch time (and its achieved FP64 rate) comes from the timed region itself. Next to it:
+
+* the same run replayed from hipGraph-captured half-day cycles (one graph launch per cycle);
+* the plain single-graph GPU path on the 400-node graph (make_mobility_sim + advance);
+* the serving loop: Poisson arrivals at 50 / 200 / 800 requests/s of solves drawn from the uneven-batch
+  size distribution (1..2000 nodes, 1..8 age groups, 30..180 days, rel tol 1e-6/1e-8/1e-10), 10 % of
+  arrivals being contact-scaling updates U[0.3, 1] on a resident request; p50/p99 latency, node-days/s;
+* the maximum relative error against the oracle on the CPU-baseline sample, and the CPU baseline (the
+  oracle, a restatement of the reference algorithm, timed on this host's cores on a bounded sample).
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1]
 """
@@ -22,7 +29,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 from memilio_amd import capi, synthetic  # noqa: E402
-from memilio_amd.engine import Batch, Context, GraphSim  # noqa: E402
+from memilio_amd.engine import Batch, Context, GraphSim, Server  # noqa: E402
 
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
 HBM_PEAK_GBS = 8000.0
@@ -42,36 +49,12 @@ def node_flops(steps, attempts, G):
     return steps * rhs_flops(G) + attempts * (5 * rhs_flops(G) + 540 * G)
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--samples", type=int, default=1024)
-    ap.add_argument("--nodes", type=int, default=400)
-    ap.add_argument("--groups", type=int, default=6)
-    ap.add_argument("--days", type=float, default=100.0)
-    ap.add_argument("--cpu-samples", type=int, default=8)
-    ap.add_argument("--cpu-threads", type=int, default=16)
-    ap.add_argument("--no-cpu", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (runs shard over GPUs with no "
-              f"communication; see DESIGN.md)", file=sys.stderr)
-        sys.exit(2)
-
-    S, N, G, days = args.samples, args.nodes, args.groups, args.days
-    log(f"workload: ensemble {S} x {N}-node graph, G={G}, {days:g} days")
-    t = time.perf_counter()
-    base = synthetic.graph(num_nodes=N, G=G, seed=1)
-    specs = synthetic.ensemble(base, num_samples=S, seed=2)
-    cfg = synthetic.ck_cfg(1e-10)
-    cfgs = [cfg] * S
-    tmaxs = [days] * S
-    log(f"inputs generated in {time.perf_counter() - t:.1f}s, {base.num_edges} edges")
-
+def ensemble_bench(args, specs, cfgs, tmaxs):
+    """Timed ensemble steps (events around every node / edge launch) and one graph-replayed run."""
+    S, N, days = len(specs), specs[0].num_nodes, tmaxs[0]
     node_days = S * N * days
-    with Context() as ctx:
+    out = {}
+    with Context(flags=capi.CTX_PROFILE_KERNELS) as ctx:
         t = time.perf_counter()
         batch = Batch(ctx, specs, cfgs, tmaxs)
         log(f"batch resident in HBM in {time.perf_counter() - t:.1f}s")
@@ -87,15 +70,32 @@ def main():
             dt = time.perf_counter() - t0
             times.append(dt)
             infos.append(info)
-            log(f"step {k}: {dt * 1e3:.1f} ms wall, {info.device_ms:.1f} ms device, steps {info.steps}, "
-                f"attempts {info.attempts}, fix-up passes {info.fixup_passes}, replays {info.exact_replays}")
+            log(f"step {k}: {dt * 1e3:.1f} ms wall, {info.device_ms:.1f} ms device, node kernel "
+                f"{info.node_kernel_ms:.1f} ms / {info.node_launches} launches, edge kernels "
+                f"{info.edge_kernel_ms:.1f} ms, steps {info.steps}, attempts {info.attempts}, fix-up passes "
+                f"{info.fixup_passes}, replays {info.exact_replays}")
         total = sum(times)
-        value = node_days * args.steps / total
-        cpu_n = min(args.cpu_samples, S)
-        gpu_sample = batch.read_daily(0, cpu_n) if not args.no_cpu else None
+        out["value"] = node_days * args.steps / total
+        out["ms_per_step"] = total / args.steps * 1e3
+        out["infos"] = infos
+        out["sample"] = batch.read_daily(0, min(args.cpu_samples, S)) if not args.no_cpu else None
         batch.close()
+    with Context() as gctx:
+        gb = Batch(gctx, specs, cfgs, tmaxs)
+        gb.run()               # capture + warm-up
+        gb.reset()
+        t0 = time.perf_counter()
+        ginfo = gb.run()
+        gs = time.perf_counter() - t0
+        gb.close()
+    out["graph_value"] = node_days / gs
+    log(f"hipGraph-replayed run: {gs * 1e3:.1f} ms wall ({ginfo.device_ms:.1f} ms device) -> "
+        f"{out['graph_value']:.3e} node-days/s")
+    return out
 
-        # plain single-graph path: the base graph alone, advance(days)
+
+def plain_bench(base, cfg, days):
+    with Context() as ctx:
         plain = GraphSim(ctx, base, cfg, horizon=days)
         plain.advance(1.0)   # warm-up (graph capture, first launches)
         plain.close()
@@ -104,24 +104,128 @@ def main():
         plain.advance(days)
         plain_s = time.perf_counter() - t0
         plain.close()
-        plain_value = N * days / plain_s
-        log(f"plain single graph: {plain_s * 1e3:.1f} ms -> {plain_value:.3e} node-days/s")
+    value = base.num_nodes * days / plain_s
+    log(f"plain single graph: {plain_s * 1e3:.1f} ms -> {value:.3e} node-days/s")
+    return value
 
-    # per-kernel timing of one profiled run (events bracket every node / edge launch)
-    with Context(flags=capi.CTX_PROFILE_KERNELS) as pctx:
-        pb = Batch(pctx, specs, cfgs, tmaxs)
-        pinfo = pb.run()
-        pb.close()
-    node_ms = pinfo.node_kernel_ms / max(pinfo.node_launches, 1)
-    flops_per_launch = node_flops(pinfo.steps, pinfo.attempts, G) / max(pinfo.node_launches, 1)
-    achieved = flops_per_launch / (node_ms * 1e-3) / 1e12
-    log(f"node kernel: {pinfo.node_launches} launches, avg {node_ms:.3f} ms, {achieved:.2f} TFLOP/s; "
-        f"edge kernel avg {pinfo.edge_kernel_ms / max(pinfo.edge_launches, 1):.3f} ms")
+
+def serving_bench(args):
+    """Poisson request streams over one serving loop per rate; requests drawn from the uneven-batch
+    distribution. Latency = completion - submission on the context clock."""
+    t = time.perf_counter()
+    pool = synthetic.uneven(args.serve_pool, seed=3)
+    log(f"serving pool of {len(pool)} requests generated in {time.perf_counter() - t:.1f}s")
+    results = []
+    for rate in args.serve_rates:
+        rng = np.random.default_rng([7, int(rate)])
+        n_arr = max(20, int(rate * args.serve_seconds))
+        gaps = rng.exponential(1.0 / rate, n_arr)
+        is_update = rng.random(n_arr) < 0.1
+        picks = rng.integers(0, len(pool), n_arr)
+        scales = rng.uniform(0.3, 1.0, n_arr)
+        with Context() as ctx:
+            srv = Server(ctx)
+            live, tickets, updates = [], [], 0
+            meta = {}
+            t_start = time.perf_counter()
+            t_next = t_start
+            for k in range(n_arr):
+                t_next += gaps[k]
+                while True:
+                    now = time.perf_counter()
+                    if now >= t_next:
+                        break
+                    time.sleep(min(t_next - now, 0.002))
+                if is_update[k]:
+                    for r in srv.poll():
+                        srv.finished[r.ticket] = r
+                    live = [tk for tk in live if tk not in srv.finished]
+                    if live:
+                        tk = live[int(rng.integers(0, len(live)))]
+                        try:
+                            srv.update(tk, float(scales[k]))
+                            updates += 1
+                        except Exception:
+                            pass   # finished between poll and update
+                    continue
+                spec, horizon, tol = pool[picks[k]]
+                tk = srv.submit(spec, synthetic.ck_cfg(tol), horizon)
+                tickets.append(tk)
+                live.append(tk)
+                meta[tk] = spec.num_nodes * horizon
+            infos = srv.wait(tickets, timeout_s=600.0)
+            t_end = time.perf_counter()
+            for tk in tickets:
+                srv.take(tk)
+            srv.stop()
+        bad = [tk for tk in tickets if infos[tk].status != capi.OK]
+        lat = np.array([infos[tk].done_ms - infos[tk].submit_ms for tk in tickets])
+        first = min(infos[tk].submit_ms for tk in tickets)
+        last = max(infos[tk].done_ms for tk in tickets)
+        nd = float(sum(meta.values()))
+        res = {"rate_rps": rate, "requests": len(tickets), "updates": updates, "failed": len(bad),
+               "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+               "node_days": nd, "node_days_per_s": nd / ((last - first) * 1e-3),
+               "wall_s": t_end - t_start}
+        log(f"serve {rate} req/s: {len(tickets)} solves + {updates} updates, p50 {res['p50_ms']:.1f} ms, "
+            f"p99 {res['p99_ms']:.1f} ms, {res['node_days_per_s']:.3e} node-days/s, failed {len(bad)}")
+        results.append(res)
+    return results
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--samples", type=int, default=1024)
+    ap.add_argument("--nodes", type=int, default=400)
+    ap.add_argument("--groups", type=int, default=6)
+    ap.add_argument("--days", type=float, default=100.0)
+    ap.add_argument("--cpu-samples", type=int, default=8)
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-rates", type=float, nargs="*", default=[50.0, 200.0, 800.0])
+    ap.add_argument("--serve-seconds", type=float, default=4.0)
+    ap.add_argument("--serve-pool", type=int, default=384)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: every run of this build uses "
+              f"a single GPU (ensembles would shard over GPUs by runs, with no communication)", file=sys.stderr)
+        sys.exit(2)
+
+    S, N, G, days = args.samples, args.nodes, args.groups, args.days
+    log(f"workload: ensemble {S} x {N}-node graph, G={G}, {days:g} days")
+    t = time.perf_counter()
+    base = synthetic.graph(num_nodes=N, G=G, seed=1)
+    specs = synthetic.ensemble(base, num_samples=S, seed=2)
+    cfg = synthetic.ck_cfg(1e-10)
+    cfgs = [cfg] * S
+    tmaxs = [days] * S
+    log(f"inputs generated in {time.perf_counter() - t:.1f}s, {base.num_edges} edges")
+
+    ens = ensemble_bench(args, specs, cfgs, tmaxs)
+    infos = ens["infos"]
+    node_ms_sum = sum(i.node_kernel_ms for i in infos)
+    launches = sum(i.node_launches for i in infos)
+    node_ms = node_ms_sum / max(launches, 1)
+    flops = sum(node_flops(i.steps, i.attempts, G) for i in infos)
+    flops_per_launch = flops / max(launches, 1)
+    achieved = flops / (node_ms_sum * 1e-3) / 1e12
+    edge_ms = sum(i.edge_kernel_ms for i in infos) / max(sum(i.edge_launches for i in infos), 1)
+    log(f"node kernel: {launches} launches, avg {node_ms:.3f} ms, {achieved:.2f} TFLOP/s; edge phase avg "
+        f"{edge_ms:.3f} ms per cycle")
+
+    plain_value = None if args.no_plain else plain_bench(base, cfg, days)
+    serving = None if args.no_serve else serving_bench(args)
 
     cpu = None
     max_rel = None
     if not args.no_cpu:
         from oracle import oracle_py
+        cpu_n = min(args.cpu_samples, S)
         sub = specs[:cpu_n]
         t0 = time.perf_counter()
         ref, st = oracle_py.ensemble_run(sub, cfgs[:cpu_n], tmaxs[:cpu_n], threads=args.cpu_threads)
@@ -131,9 +235,11 @@ def main():
                "sample": f"{cpu_n} of the {S} samples ({cpu_n * N * days:.0f} node-days), oracle = reference "
                          f"algorithm restated in C++ (-O3, no FMA contraction), OpenMP over samples, "
                          f"{cpu_s:.1f} s"}
-        max_rel = max(float(np.max(np.abs(g - r) / np.maximum(np.abs(r), 1e-6))) for g, r in zip(gpu_sample, ref))
+        max_rel = max(float(np.max(np.abs(g - r) / np.maximum(np.abs(r), 1e-6)))
+                      for g, r in zip(ens["sample"], ref))
         log(f"cpu baseline: {cpu['value']:.3e} node-days/s on {args.cpu_threads} threads; max rel err {max_rel:.2e}")
 
+    value = ens["value"]
     res = {
         "metric": "node-days/s batched+served vs plain GPU at <=1e-6 rel err; p50/p99 latency",
         "value": value,
@@ -141,7 +247,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": total / args.steps * 1e3,
+        "ms_per_step": ens["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -150,16 +256,20 @@ def main():
         "config": {"workload": f"ensemble of {S} samples of a {N}-node G={G} graph, {days:g} days, adaptive "
                                f"Cash-Karp 1e-10, daily exchange", "samples": S, "nodes": N, "groups": G,
                    "days": days, "edges": int(base.num_edges)},
+        "graph_replay_node_days_per_s": ens["graph_value"],
         "plain_gpu_node_days_per_s": plain_value,
-        "batched_vs_plain": value / plain_value,
+        "batched_vs_plain": value / plain_value if plain_value else None,
         "max_rel_err_vs_oracle": max_rel,
         "roofline": {"bound": "mfma", "roof": "FP64 vector ALU (78.6 TFLOP/s = FP64 peak of MI355X)",
                      "kernel": f"node_phase<G={G},CashKarp54>", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                     "avg_launch_ms": node_ms, "launches": pinfo.node_launches,
-                     "flops_per_launch": flops_per_launch},
+                     "avg_launch_ms": node_ms, "launches": launches, "flops_per_launch": flops_per_launch},
+        "serving": serving,
         "cpu_baseline": cpu,
     }
+    if serving:
+        best = max(s["node_days_per_s"] for s in serving)
+        res["served_vs_plain"] = best / plain_value if plain_value else None
     if cpu:
         res["vs_cpu"] = value / cpu["value"]
     print(json.dumps(res), flush=True)

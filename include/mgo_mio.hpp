@@ -1,0 +1,1746 @@
+// mgo_mio.hpp - the mio:: call shapes of MEmilio's osecir Graph-ODE path over the C ABI of mgo.h.
+//
+// Header-only C++20 layer so that code written against
+//   mio::osecir::Model / Parameters / ContactMatrixGroup,
+//   mio::Graph<mio::SimulationNode<double, mio::osecir::Simulation<double>>, mio::MobilityEdge<double>>,
+//   mio::make_mobility_sim(t0, dt, graph).advance(tmax), mio::osecir::simulate(...),
+//   mio::ParameterStudy<...>::run(create_simulation, process_result)
+// compiles against this engine with the same names, argument meanings and call order. The reference's
+// own headers need Eigen and Boost; this layer has no dependencies: matrix and vector arguments may be
+// any type with rows()/cols()/operator()(i, j) or size()/operator[](i) (Eigen types qualify), and small
+// Matrix / Vector types with the Constant/Zero factories are provided for callers without Eigen.
+//
+// Reference interfaces mirrored (paths relative to the memilio source tree):
+//   osecir parameters and model          cpp/models/ode_secir/parameters.h:46-394,400-450, model.h:56-80
+//   populations                          cpp/memilio/epidemiology/populations.h (operator[], set_difference_*)
+//   contact matrices and dampings        cpp/memilio/epidemiology/contact_matrix.h:190-194,392-400, damping.h
+//   integrator cores                     cpp/memilio/math/stepper_wrapper.h:83-91, adapt_rk.h:152-175,
+//                                        euler.h:32-63, compartments/simulation_base.h:86-89
+//   Graph, add_node, add_edge            cpp/memilio/mobility/graph.h:144-303 (edges kept sorted by
+//                                        (start, end), a repeated edge replaces the old one)
+//   SimulationNode, MobilityEdge         cpp/memilio/mobility/metapopulation_mobility_instant.h:39-101,303-363
+//   make_mobility_sim, GraphSimulation   metapopulation_mobility_instant.h:656-678, graph_simulation.h:60-120
+//   osecir::simulate                     cpp/models/ode_secir/model.h:385-390
+//   ParameterStudy::run / run_serial     cpp/memilio/compartments/parameter_studies.h:47-130,156-217,288-345
+//   interpolate_simulation_result        cpp/memilio/data/analyze_result.h:186-265
+//
+// Differences a caller sees:
+//   * Results are daily: a node's get_result() holds the values at the integer days reached (what
+//     interpolate_simulation_result would return for the reference's step-wise series) plus the current
+//     state when the simulation time is not an integer day.
+//   * A node's ContactMatrixGroup may hold several matrices if at most one of them has dampings; they
+//     are folded into one (exact in real arithmetic, ULP-level rounding differences).
+//   * All nodes of a graph share one integrator core configuration and initial step (the device runs a
+//     graph with one configuration); differing nodes raise Error(MGO_ERR_UNSUPPORTED).
+//   * ParameterStudy::run executes every simulation create_simulation returns in one device batch, then
+//     calls process_result in run order. Parameter sampling stays with create_simulation.
+//   * Errors are thrown as mgo_mio::Error (status + message) instead of the reference's asserts.
+//   * Edge result tracking (MobilityEdge save_indices / get_mobility_results) and dynamic NPIs are not
+//     provided; passing save indices is accepted and ignored.
+//
+// The names live in namespace mgo_mio; unless MGO_MIO_NO_ALIAS is defined, `mio` is an alias of it.
+#ifndef MGO_MIO_HPP
+#define MGO_MIO_HPP
+
+#include <algorithm>
+#include <array>
+#include <cfloat>
+#include <cmath>
+#include <cstddef>
+#include <cstdint>
+#include <limits>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <utility>
+#include <vector>
+
+#include "mgo.h"
+
+namespace mgo_mio
+{
+
+using ScalarType = double;
+
+// ---------------------------------------------------------------------------------------------- errors
+class Error : public std::runtime_error
+{
+public:
+    Error(mgo_status st, const std::string& msg)
+        : std::runtime_error("mgo status " + std::to_string(st) + ": " + msg)
+        , status(st)
+    {
+    }
+    mgo_status status;
+};
+
+namespace detail
+{
+inline void check(mgo_status st, const mgo_context* ctx)
+{
+    if (st != MGO_OK) {
+        throw Error(st, mgo_last_error(ctx));
+    }
+}
+
+struct ContextHolder {
+    mgo_context* ctx = nullptr;
+    mgo_device_cfg cfg{0, 0, 0};
+    ~ContextHolder()
+    {
+        if (ctx) {
+            mgo_context_destroy(ctx);
+        }
+    }
+};
+
+inline ContextHolder& context_holder()
+{
+    static ContextHolder h;
+    return h;
+}
+} // namespace detail
+
+// Device context used by every simulation of this layer (created on first use, device 0 by default).
+inline mgo_context* default_context()
+{
+    auto& h = detail::context_holder();
+    if (!h.ctx) {
+        detail::check(mgo_context_create(&h.cfg, &h.ctx), nullptr);
+    }
+    return h.ctx;
+}
+
+// Selects the device (and HBM budget, MGO_CTX_* flags) for simulations created afterwards.
+inline void set_device(int device, size_t hbm_budget_bytes = 0, int flags = 0)
+{
+    auto& h = detail::context_holder();
+    if (h.ctx) {
+        mgo_context_destroy(h.ctx);
+        h.ctx = nullptr;
+    }
+    h.cfg = mgo_device_cfg{device, hbm_budget_bytes, flags};
+}
+
+// ---------------------------------------------------------------------------------------------- indices
+class AgeGroup
+{
+public:
+    constexpr explicit AgeGroup(size_t i = 0)
+        : m_i(i)
+    {
+    }
+    constexpr size_t get() const
+    {
+        return m_i;
+    }
+    AgeGroup& operator++()
+    {
+        ++m_i;
+        return *this;
+    }
+    friend constexpr bool operator==(AgeGroup a, AgeGroup b)
+    {
+        return a.m_i == b.m_i;
+    }
+    friend constexpr auto operator<=>(AgeGroup a, AgeGroup b)
+    {
+        return a.m_i <=> b.m_i;
+    }
+
+private:
+    size_t m_i;
+};
+
+struct SimulationTime {
+    explicit SimulationTime(double t)
+        : m_t(t)
+    {
+    }
+    double get() const
+    {
+        return m_t;
+    }
+    double m_t;
+};
+
+struct DampingLevel {
+    explicit DampingLevel(int v)
+        : m_v(v)
+    {
+    }
+    int get() const
+    {
+        return m_v;
+    }
+    int m_v;
+};
+
+struct DampingType {
+    explicit DampingType(int v)
+        : m_v(v)
+    {
+    }
+    int get() const
+    {
+        return m_v;
+    }
+    int m_v;
+};
+
+// ---------------------------------------------------------------------------------------------- dense types
+// Row-major dense matrix for callers without Eigen.
+class Matrix
+{
+public:
+    Matrix() = default;
+    Matrix(size_t rows, size_t cols, double v = 0.0)
+        : m_rows(rows)
+        , m_cols(cols)
+        , m_data(rows * cols, v)
+    {
+    }
+    template <class M>
+        requires requires(const M& m) {
+            m.rows();
+            m.cols();
+            m(0, 0);
+        }
+    static Matrix from(const M& m)
+    {
+        Matrix r(size_t(m.rows()), size_t(m.cols()));
+        for (size_t i = 0; i < r.m_rows; ++i) {
+            for (size_t j = 0; j < r.m_cols; ++j) {
+                r(i, j) = double(m(i, j));
+            }
+        }
+        return r;
+    }
+    static Matrix Constant(size_t rows, size_t cols, double v)
+    {
+        return Matrix(rows, cols, v);
+    }
+    static Matrix Zero(size_t rows, size_t cols)
+    {
+        return Matrix(rows, cols, 0.0);
+    }
+    double& operator()(size_t i, size_t j)
+    {
+        return m_data[i * m_cols + j];
+    }
+    double operator()(size_t i, size_t j) const
+    {
+        return m_data[i * m_cols + j];
+    }
+    size_t rows() const
+    {
+        return m_rows;
+    }
+    size_t cols() const
+    {
+        return m_cols;
+    }
+    const double* data() const
+    {
+        return m_data.data();
+    }
+    friend bool operator==(const Matrix& a, const Matrix& b)
+    {
+        return a.m_rows == b.m_rows && a.m_cols == b.m_cols && a.m_data == b.m_data;
+    }
+
+private:
+    size_t m_rows = 0, m_cols = 0;
+    std::vector<double> m_data;
+};
+
+class Vector
+{
+public:
+    Vector() = default;
+    explicit Vector(size_t n, double v = 0.0)
+        : m_data(n, v)
+    {
+    }
+    template <class V>
+        requires requires(const V& v) {
+            v.size();
+            v[0];
+        }
+    static Vector from(const V& v)
+    {
+        Vector r(size_t(v.size()));
+        for (size_t i = 0; i < r.size(); ++i) {
+            r[i] = double(v[i]);
+        }
+        return r;
+    }
+    static Vector Constant(size_t n, double v)
+    {
+        return Vector(n, v);
+    }
+    static Vector Zero(size_t n)
+    {
+        return Vector(n, 0.0);
+    }
+    size_t size() const
+    {
+        return m_data.size();
+    }
+    double& operator[](size_t i)
+    {
+        return m_data[i];
+    }
+    double operator[](size_t i) const
+    {
+        return m_data[i];
+    }
+    double& operator()(size_t i)
+    {
+        return m_data[i];
+    }
+    double operator()(size_t i) const
+    {
+        return m_data[i];
+    }
+    const double* data() const
+    {
+        return m_data.data();
+    }
+    double* data()
+    {
+        return m_data.data();
+    }
+
+private:
+    std::vector<double> m_data;
+};
+
+namespace detail
+{
+template <class M>
+Matrix to_matrix(const M& m)
+{
+    if constexpr (std::is_same_v<M, Matrix>) {
+        return m;
+    }
+    else {
+        return Matrix::from(m);
+    }
+}
+template <class V>
+Vector to_vector(const V& v)
+{
+    if constexpr (std::is_same_v<V, Vector>) {
+        return v;
+    }
+    else if constexpr (requires { v.size(); v[0]; }) {
+        return Vector::from(v);
+    }
+    else {
+        // Eigen vectors index with operator()
+        Vector r(size_t(v.size()));
+        for (size_t i = 0; i < r.size(); ++i) {
+            r[i] = double(v(i));
+        }
+        return r;
+    }
+}
+} // namespace detail
+
+// ---------------------------------------------------------------------------------------------- contacts
+struct Damping {
+    Matrix coeffs;
+    int level = 0;
+    int type  = 0;
+    double time = 0.0;
+};
+
+// ContactMatrix: baseline B, minimum M, dampings; effective B - D(t) o (B - M).
+class ContactMatrix
+{
+public:
+    explicit ContactMatrix(size_t num_groups = 0)
+        : m_baseline(num_groups, num_groups)
+        , m_minimum(num_groups, num_groups)
+    {
+    }
+    template <class M>
+    explicit ContactMatrix(const M& baseline)
+        : m_baseline(detail::to_matrix(baseline))
+        , m_minimum(m_baseline.rows(), m_baseline.cols())
+    {
+    }
+    template <class M1, class M2>
+    ContactMatrix(const M1& baseline, const M2& minimum)
+        : m_baseline(detail::to_matrix(baseline))
+        , m_minimum(detail::to_matrix(minimum))
+    {
+    }
+    template <class M>
+    void add_damping(const M& coeffs, DampingLevel level, DampingType type, SimulationTime t)
+    {
+        m_dampings.push_back(Damping{detail::to_matrix(coeffs), level.get(), type.get(), t.get()});
+    }
+    template <class M>
+    void add_damping(const M& coeffs, SimulationTime t)
+    {
+        add_damping(coeffs, DampingLevel(0), DampingType(0), t);
+    }
+    void add_damping(double d, DampingLevel level, DampingType type, SimulationTime t)
+    {
+        add_damping(Matrix::Constant(m_baseline.rows(), m_baseline.cols(), d), level, type, t);
+    }
+    void add_damping(double d, SimulationTime t)
+    {
+        add_damping(d, DampingLevel(0), DampingType(0), t);
+    }
+    void clear_dampings()
+    {
+        m_dampings.clear();
+    }
+    const Matrix& get_baseline() const
+    {
+        return m_baseline;
+    }
+    Matrix& get_baseline()
+    {
+        return m_baseline;
+    }
+    const Matrix& get_minimum() const
+    {
+        return m_minimum;
+    }
+    Matrix& get_minimum()
+    {
+        return m_minimum;
+    }
+    const std::vector<Damping>& get_dampings() const
+    {
+        return m_dampings;
+    }
+
+private:
+    Matrix m_baseline, m_minimum;
+    std::vector<Damping> m_dampings;
+};
+
+class ContactMatrixGroup
+{
+public:
+    ContactMatrixGroup(size_t num_matrices = 1, size_t num_groups = 0)
+        : m_mats(num_matrices, ContactMatrix(num_groups))
+    {
+    }
+    ContactMatrix& operator[](size_t k)
+    {
+        return m_mats.at(k);
+    }
+    const ContactMatrix& operator[](size_t k) const
+    {
+        return m_mats.at(k);
+    }
+    size_t get_num_matrices() const
+    {
+        return m_mats.size();
+    }
+    template <class... Args>
+    void add_damping(const Args&... args)
+    {
+        for (auto& m : m_mats) {
+            m.add_damping(args...);
+        }
+    }
+    void clear_dampings()
+    {
+        for (auto& m : m_mats) {
+            m.clear_dampings();
+        }
+    }
+
+private:
+    std::vector<ContactMatrix> m_mats;
+};
+
+// ---------------------------------------------------------------------------------------------- results
+// Time series of one node: num_elements values per time point.
+template <class FP = double>
+class TimeSeries
+{
+public:
+    explicit TimeSeries(size_t num_elements = 0)
+        : m_n(num_elements)
+    {
+    }
+    size_t get_num_elements() const
+    {
+        return m_n;
+    }
+    size_t get_num_time_points() const
+    {
+        return m_t.size();
+    }
+    void add_time_point(FP t, const FP* values)
+    {
+        m_t.push_back(t);
+        m_v.insert(m_v.end(), values, values + m_n);
+    }
+    template <class V>
+        requires(!std::is_convertible_v<const V&, const FP*>)
+    void add_time_point(FP t, const V& values)
+    {
+        const Vector v = detail::to_vector(values);
+        add_time_point(t, v.data());
+    }
+    FP get_time(size_t i) const
+    {
+        return m_t.at(i);
+    }
+    FP get_last_time() const
+    {
+        return m_t.back();
+    }
+    Vector get_value(size_t i) const
+    {
+        Vector r(m_n);
+        std::copy(m_v.begin() + std::ptrdiff_t(i * m_n), m_v.begin() + std::ptrdiff_t((i + 1) * m_n), r.data());
+        return r;
+    }
+    Vector get_last_value() const
+    {
+        return get_value(m_t.size() - 1);
+    }
+    const FP* value_data(size_t i) const
+    {
+        return m_v.data() + i * m_n;
+    }
+    void clear()
+    {
+        m_t.clear();
+        m_v.clear();
+    }
+
+private:
+    size_t m_n;
+    std::vector<FP> m_t;
+    std::vector<FP> m_v;
+};
+
+// Values at the integer days ceil(t0 - abs_tol) .. floor(tN + abs_tol), linearly interpolated between the
+// series' time points (analyze_result.h:186-265).
+template <class FP>
+TimeSeries<FP> interpolate_simulation_result(const TimeSeries<FP>& ts, FP abs_tol = 1e-10)
+{
+    TimeSeries<FP> out(ts.get_num_elements());
+    const size_t np = ts.get_num_time_points();
+    if (np == 0) {
+        return out;
+    }
+    const long d0 = long(std::ceil(ts.get_time(0) - abs_tol));
+    const long d1 = long(std::floor(ts.get_last_time() + abs_tol));
+    std::vector<FP> v(ts.get_num_elements());
+    size_t k = 0;
+    for (long d = d0; d <= d1; ++d) {
+        const FP day = FP(d);
+        while (k + 1 < np && ts.get_time(k + 1) < day - abs_tol) {
+            ++k;
+        }
+        if (std::abs(ts.get_time(k) - day) <= abs_tol || k + 1 >= np) {
+            out.add_time_point(day, ts.value_data(k));
+            continue;
+        }
+        if (std::abs(ts.get_time(k + 1) - day) <= abs_tol) {
+            out.add_time_point(day, ts.value_data(k + 1));
+            continue;
+        }
+        const FP w = (day - ts.get_time(k)) / (ts.get_time(k + 1) - ts.get_time(k));
+        for (size_t i = 0; i < v.size(); ++i) {
+            v[i] = ts.value_data(k)[i] + w * (ts.value_data(k + 1)[i] - ts.value_data(k)[i]);
+        }
+        out.add_time_point(day, v.data());
+    }
+    return out;
+}
+
+// ---------------------------------------------------------------------------------------------- integrators
+// Integrator cores; each names the device integrator it selects (include/mgo.h MGO_INTEGRATOR_*).
+template <class FP = double>
+class OdeIntegratorCore
+{
+public:
+    virtual ~OdeIntegratorCore()                                 = default;
+    virtual mgo_integrator_cfg cfg() const                       = 0;
+    virtual std::unique_ptr<OdeIntegratorCore<FP>> clone() const = 0;
+};
+
+namespace detail
+{
+template <class FP, int Kind>
+class AdaptiveCore : public OdeIntegratorCore<FP>
+{
+public:
+    AdaptiveCore(FP abs_tol = 1e-10, FP rel_tol = 1e-5, FP dt_min = std::numeric_limits<FP>::min(),
+                 FP dt_max = std::numeric_limits<FP>::max())
+        : m_abs(abs_tol)
+        , m_rel(rel_tol)
+        , m_min(dt_min)
+        , m_max(dt_max)
+    {
+    }
+    void set_abs_tolerance(FP t)
+    {
+        m_abs = t;
+    }
+    void set_rel_tolerance(FP t)
+    {
+        m_rel = t;
+    }
+    void set_dt_min(FP t)
+    {
+        m_min = t;
+    }
+    void set_dt_max(FP t)
+    {
+        m_max = t;
+    }
+    FP get_dt_min() const
+    {
+        return m_min;
+    }
+    FP get_dt_max() const
+    {
+        return m_max;
+    }
+    mgo_integrator_cfg cfg() const override
+    {
+        return mgo_integrator_cfg{Kind, double(m_abs), double(m_rel), double(m_min), double(m_max)};
+    }
+
+protected:
+    FP m_abs, m_rel, m_min, m_max;
+};
+} // namespace detail
+
+// ControlledStepperWrapper<FP, runge_kutta_cash_karp54>: the reference's default core (odeint controller)
+template <class FP = double>
+class ControlledStepperWrapper : public detail::AdaptiveCore<FP, MGO_INTEGRATOR_CASH_KARP54>
+{
+public:
+    using detail::AdaptiveCore<FP, MGO_INTEGRATOR_CASH_KARP54>::AdaptiveCore;
+    std::unique_ptr<OdeIntegratorCore<FP>> clone() const override
+    {
+        return std::make_unique<ControlledStepperWrapper>(*this);
+    }
+};
+
+// RKIntegratorCore with the default Runge-Kutta-Fehlberg 4(5) tableau
+template <class FP = double>
+class RKIntegratorCore : public detail::AdaptiveCore<FP, MGO_INTEGRATOR_RKF45>
+{
+public:
+    using detail::AdaptiveCore<FP, MGO_INTEGRATOR_RKF45>::AdaptiveCore;
+    std::unique_ptr<OdeIntegratorCore<FP>> clone() const override
+    {
+        return std::make_unique<RKIntegratorCore>(*this);
+    }
+};
+
+// ExplicitStepperWrapper<FP, runge_kutta_cash_karp54>: fixed step (the simulation's dt)
+template <class FP = double>
+class ExplicitStepperWrapper : public OdeIntegratorCore<FP>
+{
+public:
+    mgo_integrator_cfg cfg() const override
+    {
+        return mgo_integrator_cfg{MGO_INTEGRATOR_CASH_KARP54_FIXED, 0.0, 0.0, 0.0, 0.0};
+    }
+    std::unique_ptr<OdeIntegratorCore<FP>> clone() const override
+    {
+        return std::make_unique<ExplicitStepperWrapper>(*this);
+    }
+};
+
+template <class FP = double>
+class EulerIntegratorCore : public OdeIntegratorCore<FP>
+{
+public:
+    mgo_integrator_cfg cfg() const override
+    {
+        return mgo_integrator_cfg{MGO_INTEGRATOR_EULER, 0.0, 0.0, 0.0, 0.0};
+    }
+    std::unique_ptr<OdeIntegratorCore<FP>> clone() const override
+    {
+        return std::make_unique<EulerIntegratorCore>(*this);
+    }
+};
+
+// ---------------------------------------------------------------------------------------------- osecir
+namespace osecir
+{
+
+enum class InfectionState
+{
+    Susceptible = 0,
+    Exposed,
+    InfectedNoSymptoms,
+    InfectedNoSymptomsConfirmed,
+    InfectedSymptoms,
+    InfectedSymptomsConfirmed,
+    InfectedSevere,
+    InfectedCritical,
+    Recovered,
+    Dead,
+    Count
+};
+
+// Per-age-group parameter values (CustomIndexArray<UncertainValue, AgeGroup>); assigning a scalar sets
+// every group.
+class AgeArray
+{
+public:
+    explicit AgeArray(size_t num_groups = 0, double v = 0.0)
+        : m_v(num_groups, v)
+    {
+    }
+    AgeArray& operator=(double v)
+    {
+        std::fill(m_v.begin(), m_v.end(), v);
+        return *this;
+    }
+    double& operator[](AgeGroup g)
+    {
+        return m_v.at(g.get());
+    }
+    double operator[](AgeGroup g) const
+    {
+        return m_v.at(g.get());
+    }
+    size_t numel() const
+    {
+        return m_v.size();
+    }
+
+private:
+    std::vector<double> m_v;
+};
+
+// parameter tags (parameters.h:46-394): scalar ones index MGO_P_*, per-age ones MGO_A_*
+#define MGO_MIO_SCALAR_TAG(Name, Index)                                                                         \
+    template <class FP = double>                                                                                \
+    struct Name {                                                                                               \
+        static constexpr int scalar_index = Index;                                                              \
+        static std::string name()                                                                               \
+        {                                                                                                       \
+            return #Name;                                                                                       \
+        }                                                                                                       \
+    };
+#define MGO_MIO_AGE_TAG(Name, Index)                                                                            \
+    template <class FP = double>                                                                                \
+    struct Name {                                                                                               \
+        static constexpr int age_index = Index;                                                                 \
+        static std::string name()                                                                               \
+        {                                                                                                       \
+            return #Name;                                                                                       \
+        }                                                                                                       \
+    };
+MGO_MIO_SCALAR_TAG(StartDay, MGO_P_START_DAY)
+MGO_MIO_SCALAR_TAG(Seasonality, MGO_P_SEASONALITY)
+MGO_MIO_SCALAR_TAG(ICUCapacity, MGO_P_ICU_CAPACITY)
+MGO_MIO_SCALAR_TAG(TestAndTraceCapacity, MGO_P_TEST_AND_TRACE_CAPACITY)
+MGO_MIO_SCALAR_TAG(TestAndTraceCapacityMaxRisk, MGO_P_TEST_AND_TRACE_MAX_RISK)
+MGO_MIO_AGE_TAG(TimeExposed, MGO_A_TIME_EXPOSED)
+MGO_MIO_AGE_TAG(TimeInfectedNoSymptoms, MGO_A_TIME_INFECTED_NO_SYMPTOMS)
+MGO_MIO_AGE_TAG(TimeInfectedSymptoms, MGO_A_TIME_INFECTED_SYMPTOMS)
+MGO_MIO_AGE_TAG(TimeInfectedSevere, MGO_A_TIME_INFECTED_SEVERE)
+MGO_MIO_AGE_TAG(TimeInfectedCritical, MGO_A_TIME_INFECTED_CRITICAL)
+MGO_MIO_AGE_TAG(TransmissionProbabilityOnContact, MGO_A_TRANSMISSION_PROBABILITY_ON_CONTACT)
+MGO_MIO_AGE_TAG(RelativeTransmissionNoSymptoms, MGO_A_RELATIVE_TRANSMISSION_NO_SYMPTOMS)
+MGO_MIO_AGE_TAG(RecoveredPerInfectedNoSymptoms, MGO_A_RECOVERED_PER_INFECTED_NO_SYMPTOMS)
+MGO_MIO_AGE_TAG(RiskOfInfectionFromSymptomatic, MGO_A_RISK_OF_INFECTION_FROM_SYMPTOMATIC)
+MGO_MIO_AGE_TAG(MaxRiskOfInfectionFromSymptomatic, MGO_A_MAX_RISK_OF_INFECTION_FROM_SYMPTOMATIC)
+MGO_MIO_AGE_TAG(SeverePerInfectedSymptoms, MGO_A_SEVERE_PER_INFECTED_SYMPTOMS)
+MGO_MIO_AGE_TAG(CriticalPerSevere, MGO_A_CRITICAL_PER_SEVERE)
+MGO_MIO_AGE_TAG(DeathsPerSevere, MGO_A_DEATHS_PER_SEVERE)
+MGO_MIO_AGE_TAG(DeathsPerCritical, MGO_A_DEATHS_PER_CRITICAL)
+#undef MGO_MIO_SCALAR_TAG
+#undef MGO_MIO_AGE_TAG
+
+template <class FP = double>
+struct ContactPatterns {
+    static std::string name()
+    {
+        return "ContactPatterns";
+    }
+};
+
+template <class FP = double>
+class Parameters
+{
+public:
+    explicit Parameters(AgeGroup num_groups)
+        : m_G(num_groups.get())
+        , m_contacts(1, num_groups.get())
+    {
+        m_scalars.fill(0.0);
+        m_scalars[MGO_P_ICU_CAPACITY]            = std::numeric_limits<double>::max();
+        m_scalars[MGO_P_TEST_AND_TRACE_CAPACITY] = std::numeric_limits<double>::max();
+        m_scalars[MGO_P_TEST_AND_TRACE_MAX_RISK] = 5.0;
+        m_ages.assign(MGO_NUM_AGE_PARAMS, AgeArray(m_G, 0.0));
+        for (int k : {MGO_A_TIME_EXPOSED, MGO_A_TIME_INFECTED_NO_SYMPTOMS, MGO_A_TIME_INFECTED_SYMPTOMS,
+                      MGO_A_TIME_INFECTED_SEVERE, MGO_A_TIME_INFECTED_CRITICAL,
+                      MGO_A_TRANSMISSION_PROBABILITY_ON_CONTACT, MGO_A_RELATIVE_TRANSMISSION_NO_SYMPTOMS,
+                      MGO_A_RISK_OF_INFECTION_FROM_SYMPTOMATIC, MGO_A_MAX_RISK_OF_INFECTION_FROM_SYMPTOMATIC}) {
+            m_ages[size_t(k)] = 1.0;
+        }
+    }
+    AgeGroup get_num_groups() const
+    {
+        return AgeGroup(m_G);
+    }
+    template <class Tag>
+    decltype(auto) get()
+    {
+        if constexpr (requires { Tag::scalar_index; }) {
+            return (m_scalars[Tag::scalar_index]);
+        }
+        else if constexpr (requires { Tag::age_index; }) {
+            return (m_ages[Tag::age_index]);
+        }
+        else {
+            static_assert(std::is_same_v<Tag, ContactPatterns<FP>>, "parameter not on the device path");
+            return (m_contacts);
+        }
+    }
+    template <class Tag>
+    decltype(auto) get() const
+    {
+        if constexpr (requires { Tag::scalar_index; }) {
+            return (std::as_const(m_scalars[Tag::scalar_index]));
+        }
+        else if constexpr (requires { Tag::age_index; }) {
+            return (std::as_const(m_ages[Tag::age_index]));
+        }
+        else {
+            static_assert(std::is_same_v<Tag, ContactPatterns<FP>>, "parameter not on the device path");
+            return (std::as_const(m_contacts));
+        }
+    }
+    template <class Tag, class V>
+    void set(const V& v)
+    {
+        get<Tag>() = v;
+    }
+    FP& get_commuter_nondetection()
+    {
+        return m_scalars[MGO_P_COMMUTER_NONDETECTION];
+    }
+    FP get_commuter_nondetection() const
+    {
+        return m_scalars[MGO_P_COMMUTER_NONDETECTION];
+    }
+    FP& get_start_commuter_detection()
+    {
+        return m_scalars[MGO_P_START_COMMUTER_DETECTION];
+    }
+    FP get_start_commuter_detection() const
+    {
+        return m_scalars[MGO_P_START_COMMUTER_DETECTION];
+    }
+    FP& get_end_commuter_detection()
+    {
+        return m_scalars[MGO_P_END_COMMUTER_DETECTION];
+    }
+    FP get_end_commuter_detection() const
+    {
+        return m_scalars[MGO_P_END_COMMUTER_DETECTION];
+    }
+    const std::array<double, MGO_NUM_SCALARS>& scalars() const
+    {
+        return m_scalars;
+    }
+    const AgeArray& age_param(int k) const
+    {
+        return m_ages[size_t(k)];
+    }
+
+private:
+    size_t m_G;
+    std::array<double, MGO_NUM_SCALARS> m_scalars;
+    std::vector<AgeArray> m_ages;
+    ContactMatrixGroup m_contacts;
+};
+
+// Compartment values indexed by {AgeGroup, InfectionState}; flat index g*10 + state.
+class Populations
+{
+public:
+    struct Index {
+        AgeGroup group;
+        InfectionState state;
+    };
+    explicit Populations(size_t num_groups = 0)
+        : m_G(num_groups)
+        , m_v(num_groups * MGO_NUM_COMPARTMENTS, 0.0)
+    {
+    }
+    size_t get_flat_index(const Index& i) const
+    {
+        return i.group.get() * MGO_NUM_COMPARTMENTS + size_t(i.state);
+    }
+    double& operator[](const Index& i)
+    {
+        return m_v.at(get_flat_index(i));
+    }
+    double operator[](const Index& i) const
+    {
+        return m_v.at(get_flat_index(i));
+    }
+    size_t numel() const
+    {
+        return m_v.size();
+    }
+    double get_total() const
+    {
+        double s = 0.0;
+        for (double x : m_v) {
+            s += x;
+        }
+        return s;
+    }
+    double get_group_total(AgeGroup g) const
+    {
+        double s = 0.0;
+        for (size_t c = 0; c < MGO_NUM_COMPARTMENTS; ++c) {
+            s += m_v[g.get() * MGO_NUM_COMPARTMENTS + c];
+        }
+        return s;
+    }
+    // sets the compartment so that all compartments sum to total (populations.h set_difference_from_total)
+    void set_difference_from_total(const Index& i, double total)
+    {
+        (*this)[i] = 0.0;
+        (*this)[i] = total - get_total();
+    }
+    // sets the compartment so that its age group sums to total
+    template <class Category = AgeGroup>
+    void set_difference_from_group_total(const Index& i, double total)
+    {
+        (*this)[i] = 0.0;
+        (*this)[i] = total - get_group_total(i.group);
+    }
+    const double* data() const
+    {
+        return m_v.data();
+    }
+
+private:
+    size_t m_G;
+    std::vector<double> m_v;
+};
+
+template <class FP = double>
+class Model
+{
+public:
+    explicit Model(int num_agegroups)
+        : populations(size_t(num_agegroups))
+        , parameters(AgeGroup(size_t(num_agegroups)))
+    {
+    }
+    size_t get_num_groups() const
+    {
+        return parameters.get_num_groups().get();
+    }
+    Populations populations;
+    Parameters<FP> parameters;
+};
+
+template <class FP = double>
+class Simulation;
+
+} // namespace osecir
+
+// ---------------------------------------------------------------------------------------------- graph
+template <class T>
+struct Node {
+    template <class... Args>
+    Node(int node_id, Args&&... args)
+        : id(node_id)
+        , property(std::forward<Args>(args)...)
+    {
+    }
+    int id;
+    T property;
+};
+
+template <class T>
+struct Edge {
+    template <class... Args>
+    Edge(size_t start, size_t end, Args&&... args)
+        : start_node_idx(start)
+        , end_node_idx(end)
+        , property(std::forward<Args>(args)...)
+    {
+    }
+    size_t start_node_idx;
+    size_t end_node_idx;
+    T property;
+};
+
+template <class NodePropertyT, class EdgePropertyT>
+class Graph
+{
+public:
+    using NodeProperty = NodePropertyT;
+    using EdgeProperty = EdgePropertyT;
+
+    template <class... Args>
+    Node<NodePropertyT>& add_node(int id, Args&&... args)
+    {
+        m_nodes.emplace_back(id, std::forward<Args>(args)...);
+        return m_nodes.back();
+    }
+    // edges stay sorted by (start, end); adding an existing edge replaces it (graph.h:231-241)
+    template <class... Args>
+    Edge<EdgePropertyT>& add_edge(size_t start, size_t end, Args&&... args)
+    {
+        if (start >= m_nodes.size() || end >= m_nodes.size()) {
+            throw Error(MGO_ERR_OUT_OF_RANGE, "add_edge: node index out of range");
+        }
+        auto it = std::lower_bound(m_edges.begin(), m_edges.end(), std::make_pair(start, end),
+                                   [](const Edge<EdgePropertyT>& e, const std::pair<size_t, size_t>& k) {
+                                       return std::make_pair(e.start_node_idx, e.end_node_idx) < k;
+                                   });
+        if (it != m_edges.end() && it->start_node_idx == start && it->end_node_idx == end) {
+            *it = Edge<EdgePropertyT>(start, end, std::forward<Args>(args)...);
+            return *it;
+        }
+        return *m_edges.insert(it, Edge<EdgePropertyT>(start, end, std::forward<Args>(args)...));
+    }
+    std::vector<Node<NodePropertyT>>& nodes()
+    {
+        return m_nodes;
+    }
+    const std::vector<Node<NodePropertyT>>& nodes() const
+    {
+        return m_nodes;
+    }
+    std::vector<Edge<EdgePropertyT>>& edges()
+    {
+        return m_edges;
+    }
+    const std::vector<Edge<EdgePropertyT>>& edges() const
+    {
+        return m_edges;
+    }
+
+private:
+    std::vector<Node<NodePropertyT>> m_nodes;
+    std::vector<Edge<EdgePropertyT>> m_edges;
+};
+
+// MobilityParameters: one coefficient per compartment (10*G), MobilityEdge owns them
+template <class FP = double>
+class MobilityParameters
+{
+public:
+    template <class V>
+    explicit MobilityParameters(const V& coeffs)
+        : m_coeffs(detail::to_vector(coeffs))
+    {
+    }
+    const Vector& get_coefficients() const
+    {
+        return m_coeffs;
+    }
+    Vector& get_coefficients()
+    {
+        return m_coeffs;
+    }
+
+private:
+    Vector m_coeffs;
+};
+
+template <class FP = double>
+class MobilityEdge
+{
+public:
+    explicit MobilityEdge(const MobilityParameters<FP>& p)
+        : m_params(p)
+    {
+    }
+    template <class V>
+        requires(!std::is_same_v<std::decay_t<V>, MobilityParameters<FP>>)
+    explicit MobilityEdge(const V& coeffs)
+        : m_params(coeffs)
+    {
+    }
+    // save_indices select commuter sums tracked by the reference; not tracked here
+    template <class V, class Idx>
+    MobilityEdge(const V& coeffs, const Idx&)
+        : m_params(coeffs)
+    {
+    }
+    const MobilityParameters<FP>& get_parameters() const
+    {
+        return m_params;
+    }
+
+private:
+    MobilityParameters<FP> m_params;
+};
+
+namespace osecir
+{
+// Simulation of one model; results are daily values (see the header comment).
+template <class FP>
+class Simulation
+{
+public:
+    explicit Simulation(const Model<FP>& model, FP t0 = 0.0, FP dt = 0.1)
+        : m_model(model)
+        , m_core(std::make_unique<ControlledStepperWrapper<FP>>())
+        , m_result(model.populations.numel())
+        , m_t0(t0)
+        , m_dt(dt)
+    {
+        m_result.add_time_point(t0, model.populations.data());
+    }
+    Simulation(const Simulation& o)
+        : m_model(o.m_model)
+        , m_core(o.m_core->clone())
+        , m_result(o.m_result)
+        , m_t0(o.m_t0)
+        , m_dt(o.m_dt)
+    {
+    }
+    Simulation(Simulation&&)            = default;
+    Simulation& operator=(Simulation&&) = default;
+    Simulation& operator=(const Simulation& o)
+    {
+        if (this != &o) {
+            *this = Simulation(o);
+        }
+        return *this;
+    }
+
+    void set_integrator_core(std::unique_ptr<OdeIntegratorCore<FP>>&& core)
+    {
+        m_core = std::move(core);
+    }
+    OdeIntegratorCore<FP>& get_integrator_core()
+    {
+        return *m_core;
+    }
+    const OdeIntegratorCore<FP>& get_integrator_core() const
+    {
+        return *m_core;
+    }
+    Model<FP>& get_model()
+    {
+        return m_model;
+    }
+    const Model<FP>& get_model() const
+    {
+        return m_model;
+    }
+    TimeSeries<FP>& get_result()
+    {
+        return m_result;
+    }
+    const TimeSeries<FP>& get_result() const
+    {
+        return m_result;
+    }
+    FP get_t0() const
+    {
+        return m_t0;
+    }
+    FP& get_dt()
+    {
+        return m_dt;
+    }
+    FP get_dt() const
+    {
+        return m_dt;
+    }
+
+    // Simulation::advance(tmax) of a stand-alone model: runs from t0 to tmax on the device in one call
+    // (osecir::simulate semantics); a second call with a larger tmax is not supported.
+    TimeSeries<FP>& advance(FP tmax);
+
+private:
+    Model<FP> m_model;
+    std::unique_ptr<OdeIntegratorCore<FP>> m_core;
+    TimeSeries<FP> m_result;
+    FP m_t0, m_dt;
+    bool m_advanced = false;
+};
+} // namespace osecir
+
+template <class FP, class Sim>
+class SimulationNode
+{
+public:
+    template <class... Args>
+        requires(!(sizeof...(Args) == 1 && (std::is_same_v<std::decay_t<Args>, SimulationNode> && ...)))
+    explicit SimulationNode(Args&&... args)
+        : m_sim(std::forward<Args>(args)...)
+    {
+    }
+    Sim& get_simulation()
+    {
+        return m_sim;
+    }
+    const Sim& get_simulation() const
+    {
+        return m_sim;
+    }
+    TimeSeries<FP>& get_result()
+    {
+        return m_sim.get_result();
+    }
+    const TimeSeries<FP>& get_result() const
+    {
+        return m_sim.get_result();
+    }
+    Vector get_last_state() const
+    {
+        return m_sim.get_result().get_last_value();
+    }
+
+private:
+    Sim m_sim;
+};
+
+template <class FP = double>
+using OsecirGraph = Graph<SimulationNode<FP, osecir::Simulation<FP>>, MobilityEdge<FP>>;
+
+namespace detail
+{
+// mgo_graph_desc arrays of a model graph
+struct GraphArrays {
+    int n = 0, G = 0;
+    std::vector<double> scalars, ages, cbase, cmin, dtimes, dcoeffs, pops, ecoeffs;
+    std::vector<int32_t> doffs, dlevels, dtypes, estart, eend;
+    mgo_integrator_cfg integrator{};
+    double dt_node = 0.1;
+    mgo_graph_desc desc{};
+
+    template <class FP>
+    void add_model(const osecir::Model<FP>& m)
+    {
+        const int g = int(m.get_num_groups());
+        if (n == 0) {
+            G = g;
+            doffs.push_back(0);
+        }
+        else if (g != G) {
+            throw Error(MGO_ERR_INVALID_VALUE, "all nodes of a graph need the same number of age groups");
+        }
+        if (G < 1 || G > MGO_MAX_GROUPS) {
+            throw Error(MGO_ERR_UNSUPPORTED, "number of age groups outside 1.." + std::to_string(MGO_MAX_GROUPS));
+        }
+        const auto& p = m.parameters;
+        scalars.insert(scalars.end(), p.scalars().begin(), p.scalars().end());
+        for (int k = 0; k < MGO_NUM_AGE_PARAMS; ++k) {
+            for (int a = 0; a < G; ++a) {
+                ages.push_back(p.age_param(k)[AgeGroup(size_t(a))]);
+            }
+        }
+        // fold the contact matrix group into one matrix: at most one matrix may carry dampings
+        const ContactMatrixGroup& cg = p.template get<osecir::ContactPatterns<FP>>();
+        int damped                   = -1;
+        for (size_t k = 0; k < cg.get_num_matrices(); ++k) {
+            if (!cg[k].get_dampings().empty()) {
+                if (damped >= 0) {
+                    throw Error(MGO_ERR_UNSUPPORTED, "more than one damped contact matrix in a node");
+                }
+                damped = int(k);
+            }
+        }
+        std::vector<double> B(size_t(G) * G, 0.0), M(size_t(G) * G, 0.0);
+        for (size_t k = 0; k < cg.get_num_matrices(); ++k) {
+            const Matrix& b  = cg[k].get_baseline();
+            const Matrix& mn = cg[k].get_minimum();
+            if (b.rows() != size_t(G) || b.cols() != size_t(G) || mn.rows() != size_t(G) || mn.cols() != size_t(G)) {
+                throw Error(MGO_ERR_INVALID_VALUE, "contact matrix shape differs from the number of age groups");
+            }
+            for (int i = 0; i < G; ++i) {
+                for (int j = 0; j < G; ++j) {
+                    B[size_t(i) * G + j] += b(size_t(i), size_t(j));
+                    // undamped matrices add their baseline to M so that B - M keeps only the damped part
+                    M[size_t(i) * G + j] += (cg.get_num_matrices() == 1 || int(k) == damped)
+                                                ? mn(size_t(i), size_t(j))
+                                                : b(size_t(i), size_t(j));
+                }
+            }
+        }
+        if (cg.get_num_matrices() > 1 && damped < 0) {
+            M.assign(M.size(), 0.0);
+            for (size_t k = 0; k < cg.get_num_matrices(); ++k) {
+                for (int i = 0; i < G; ++i) {
+                    for (int j = 0; j < G; ++j) {
+                        M[size_t(i) * G + j] += cg[k].get_minimum()(size_t(i), size_t(j));
+                    }
+                }
+            }
+        }
+        cbase.insert(cbase.end(), B.begin(), B.end());
+        cmin.insert(cmin.end(), M.begin(), M.end());
+        if (damped >= 0) {
+            for (const Damping& d : cg[size_t(damped)].get_dampings()) {
+                if (d.coeffs.rows() != size_t(G) || d.coeffs.cols() != size_t(G)) {
+                    throw Error(MGO_ERR_INVALID_VALUE, "damping shape differs from the number of age groups");
+                }
+                dtimes.push_back(d.time);
+                dlevels.push_back(d.level);
+                dtypes.push_back(d.type);
+                dcoeffs.insert(dcoeffs.end(), d.coeffs.data(), d.coeffs.data() + size_t(G) * G);
+            }
+        }
+        doffs.push_back(int32_t(dtimes.size()));
+        pops.insert(pops.end(), m.populations.data(), m.populations.data() + m.populations.numel());
+        ++n;
+    }
+
+    template <class FP>
+    void add_core(const OdeIntegratorCore<FP>& core, double dt)
+    {
+        const mgo_integrator_cfg c = core.cfg();
+        if (n == 1) {
+            integrator = c;
+            dt_node    = dt;
+        }
+        else if (c.kind != integrator.kind || c.abs_tol != integrator.abs_tol || c.rel_tol != integrator.rel_tol ||
+                 c.dt_min != integrator.dt_min || c.dt_max != integrator.dt_max || dt != dt_node) {
+            throw Error(MGO_ERR_UNSUPPORTED, "nodes of one graph need the same integrator core and initial step");
+        }
+    }
+
+    void add_edge(size_t s, size_t e, const Vector& c)
+    {
+        if (c.size() != size_t(G) * MGO_NUM_COMPARTMENTS) {
+            throw Error(MGO_ERR_INVALID_VALUE, "mobility coefficients need 10 * G values");
+        }
+        estart.push_back(int32_t(s));
+        eend.push_back(int32_t(e));
+        ecoeffs.insert(ecoeffs.end(), c.data(), c.data() + c.size());
+    }
+
+    const mgo_graph_desc& finish()
+    {
+        desc                  = mgo_graph_desc{};
+        desc.num_nodes        = n;
+        desc.num_groups       = G;
+        desc.node_scalars     = scalars.data();
+        desc.age_params       = ages.data();
+        desc.contact_baseline = cbase.data();
+        desc.contact_minimum  = cmin.data();
+        desc.damping_offsets  = doffs.data();
+        desc.damping_times    = dtimes.empty() ? nullptr : dtimes.data();
+        desc.damping_levels   = dlevels.empty() ? nullptr : dlevels.data();
+        desc.damping_types    = dtypes.empty() ? nullptr : dtypes.data();
+        desc.damping_coeffs   = dcoeffs.empty() ? nullptr : dcoeffs.data();
+        desc.populations      = pops.data();
+        desc.num_edges        = int32_t(estart.size());
+        desc.edge_start       = estart.empty() ? nullptr : estart.data();
+        desc.edge_end         = eend.empty() ? nullptr : eend.data();
+        desc.edge_coeffs      = ecoeffs.empty() ? nullptr : ecoeffs.data();
+        desc.edge_coeff_len   = G * MGO_NUM_COMPARTMENTS;
+        return desc;
+    }
+};
+
+template <class FP>
+GraphArrays graph_arrays(const OsecirGraph<FP>& g)
+{
+    GraphArrays a;
+    if (g.nodes().empty()) {
+        throw Error(MGO_ERR_INVALID_VALUE, "graph without nodes");
+    }
+    for (const auto& node : g.nodes()) {
+        const auto& sim = node.property.get_simulation();
+        a.add_model(sim.get_model());
+        a.add_core(sim.get_integrator_core(), double(sim.get_dt()));
+    }
+    for (const auto& e : g.edges()) {
+        a.add_edge(e.start_node_idx, e.end_node_idx, e.property.get_parameters().get_coefficients());
+    }
+    a.finish();
+    return a;
+}
+
+// daily values [day][node][10G] -> each node's result series (plus the state at t when t is not a day)
+template <class FP>
+void fill_results(OsecirGraph<FP>& g, const std::vector<double>& daily, int ndays, int day0, FP t,
+                  const std::vector<double>* state)
+{
+    const size_t n  = g.nodes().size();
+    const size_t nv = n ? g.nodes()[0].property.get_result().get_num_elements() : 0;
+    for (size_t v = 0; v < n; ++v) {
+        TimeSeries<FP>& ts = g.nodes()[v].property.get_result();
+        ts.clear();
+        for (int d = 0; d < ndays; ++d) {
+            ts.add_time_point(FP(day0 + d), daily.data() + (size_t(d) * n + v) * nv);
+        }
+        if (state && (ndays == 0 || std::abs(double(day0 + ndays - 1) - double(t)) > 1e-10)) {
+            ts.add_time_point(t, state->data() + v * nv);
+        }
+    }
+}
+} // namespace detail
+
+// GraphSimulation over an osecir mobility graph (GraphSimulation<Graph<SimulationNode<Simulation>,
+// MobilityEdge>>); the graph's parameters and state live on the device from the first advance on.
+template <class FP, class GraphT>
+class GraphSimulation
+{
+public:
+    GraphSimulation(FP t0, FP dt, GraphT&& g)
+        : m_graph(std::move(g))
+        , m_t(t0)
+        , m_t0(t0)
+        , m_dt(dt)
+    {
+    }
+    GraphSimulation(FP t0, FP dt, const GraphT& g)
+        : m_graph(g)
+        , m_t(t0)
+        , m_t0(t0)
+        , m_dt(dt)
+    {
+    }
+    GraphSimulation(GraphSimulation&& o) noexcept
+        : m_graph(std::move(o.m_graph))
+        , m_t(o.m_t)
+        , m_t0(o.m_t0)
+        , m_dt(o.m_dt)
+        , m_horizon(o.m_horizon)
+        , m_id(std::exchange(o.m_id, 0))
+    {
+    }
+    GraphSimulation(const GraphSimulation&)            = delete;
+    GraphSimulation& operator=(const GraphSimulation&) = delete;
+    ~GraphSimulation()
+    {
+        if (m_id) {
+            mgo_graph_destroy(default_context(), m_id);
+        }
+    }
+
+    // Largest tmax that later advance calls will use (sizes the device's daily output); optional.
+    void set_horizon(FP tmax)
+    {
+        m_horizon = tmax;
+    }
+
+    void advance(FP tmax)
+    {
+        mgo_context* ctx = default_context();
+        if (!m_id) {
+            m_arrays              = detail::graph_arrays(m_graph);
+            mgo_sim_cfg cfg{};
+            cfg.t0                = double(m_t0);
+            cfg.dt_graph          = double(m_dt);
+            cfg.dt_node           = m_arrays.dt_node;
+            cfg.horizon           = double(std::max(tmax, m_horizon));
+            cfg.integrator        = m_arrays.integrator;
+            m_cfg                 = cfg;
+            detail::check(mgo_graph_create(ctx, &m_arrays.desc, &cfg, &m_id), ctx);
+        }
+        else if (double(tmax) > m_cfg.horizon + 1e-10) {
+            throw Error(MGO_ERR_UNSUPPORTED, "advance beyond the horizon of the first advance; call set_horizon");
+        }
+        detail::check(mgo_graph_advance(ctx, m_id, double(tmax)), ctx);
+        double t = 0;
+        detail::check(mgo_graph_get_time(ctx, m_id, &t), ctx);
+        m_t            = FP(t);
+        const size_t n = m_graph.nodes().size();
+        const int nv   = m_arrays.G * MGO_NUM_COMPARTMENTS;
+        const int cap  = int(std::floor(m_cfg.horizon + 1e-10) - std::ceil(m_cfg.t0 - 1e-10)) + 1;
+        std::vector<double> daily(size_t(std::max(cap, 1)) * n * size_t(nv));
+        int32_t nd = 0;
+        detail::check(mgo_graph_read_daily(ctx, m_id, cap, daily.data(), &nd), ctx);
+        std::vector<double> state(n * size_t(nv));
+        detail::check(mgo_graph_read_state(ctx, m_id, state.data()), ctx);
+        detail::fill_results(m_graph, daily, nd, int(std::ceil(m_cfg.t0 - 1e-10)), m_t, &state);
+    }
+
+    FP get_t() const
+    {
+        return m_t;
+    }
+    FP get_dt() const
+    {
+        return m_dt;
+    }
+    GraphT& get_graph() &
+    {
+        return m_graph;
+    }
+    const GraphT& get_graph() const&
+    {
+        return m_graph;
+    }
+    GraphT&& get_graph() &&
+    {
+        return std::move(m_graph);
+    }
+
+    // used by ParameterStudy to run many simulations in one device batch
+    detail::GraphArrays& batch_arrays()
+    {
+        m_arrays = detail::graph_arrays(m_graph);
+        return m_arrays;
+    }
+    void set_batch_result(const std::vector<double>& daily, int ndays, int day0, FP t)
+    {
+        m_t = t;
+        detail::fill_results(m_graph, daily, ndays, day0, t, static_cast<const std::vector<double>*>(nullptr));
+    }
+
+private:
+    GraphT m_graph;
+    FP m_t, m_t0, m_dt;
+    FP m_horizon = 0;
+    mgo_graph_id m_id = 0;
+    detail::GraphArrays m_arrays;
+    mgo_sim_cfg m_cfg{};
+};
+
+// make_mobility_sim(t0, dt, graph): daily exchange with dt = 0.5
+template <class FP>
+GraphSimulation<FP, OsecirGraph<FP>> make_mobility_sim(FP t0, FP dt, OsecirGraph<FP>&& g)
+{
+    return GraphSimulation<FP, OsecirGraph<FP>>(t0, dt, std::move(g));
+}
+template <class FP>
+GraphSimulation<FP, OsecirGraph<FP>> make_mobility_sim(FP t0, FP dt, const OsecirGraph<FP>& g)
+{
+    return GraphSimulation<FP, OsecirGraph<FP>>(t0, dt, g);
+}
+
+namespace osecir
+{
+template <class FP>
+TimeSeries<FP>& Simulation<FP>::advance(FP tmax)
+{
+    if (m_advanced) {
+        throw Error(MGO_ERR_UNSUPPORTED, "a stand-alone Simulation advances once from t0 on the device");
+    }
+    mgo_context* ctx = default_context();
+    detail::GraphArrays a;
+    a.add_model(m_model);
+    a.add_core(*m_core, double(m_dt));
+    const mgo_graph_desc& d = a.finish();
+    mgo_sim_cfg cfg{};
+    cfg.t0         = double(m_t0);
+    cfg.dt_graph   = 0.5;
+    cfg.dt_node    = double(m_dt);
+    cfg.horizon    = double(tmax);
+    cfg.integrator = a.integrator;
+    const int cap  = int(std::floor(double(tmax) + 1e-10) - std::ceil(double(m_t0) - 1e-10)) + 1;
+    const size_t nv = m_model.populations.numel();
+    std::vector<double> out(size_t(std::max(cap, 1)) * nv);
+    int32_t nd = 0;
+    detail::check(mgo_simulate_single(ctx, &d, &cfg, double(tmax), cap, out.data(), &nd), ctx);
+    m_result.clear();
+    for (int k = 0; k < nd; ++k) {
+        m_result.add_time_point(FP(std::ceil(double(m_t0) - 1e-10) + k), out.data() + size_t(k) * nv);
+    }
+    m_advanced = true;
+    return m_result;
+}
+
+// osecir::simulate(t0, tmax, dt, model[, core]): daily values of one model (model.h:385-390)
+template <class FP>
+TimeSeries<FP> simulate(FP t0, FP tmax, FP dt, const Model<FP>& model,
+                        std::unique_ptr<OdeIntegratorCore<FP>>&& integrator_core = nullptr)
+{
+    Simulation<FP> sim(model, t0, dt);
+    if (integrator_core) {
+        sim.set_integrator_core(std::move(integrator_core));
+    }
+    sim.advance(tmax);
+    return sim.get_result();
+}
+} // namespace osecir
+
+template <class FP, class Sim, class E>
+std::vector<TimeSeries<FP>> interpolate_simulation_result(const Graph<SimulationNode<FP, Sim>, E>& g,
+                                                          FP abs_tol = 1e-10)
+{
+    std::vector<TimeSeries<FP>> r;
+    for (const auto& n : g.nodes()) {
+        r.push_back(interpolate_simulation_result(n.property.get_result(), abs_tol));
+    }
+    return r;
+}
+
+// ---------------------------------------------------------------------------------------------- study
+namespace detail
+{
+template <class T>
+struct is_osecir_graph_sim : std::false_type {
+};
+template <class FP>
+struct is_osecir_graph_sim<GraphSimulation<FP, OsecirGraph<FP>>> : std::true_type {
+};
+
+// every graph simulation advanced to tmax as one device batch (mgo_ensemble_run)
+template <class FP>
+void run_graph_batch(std::vector<GraphSimulation<FP, OsecirGraph<FP>>>& sims, FP tmax)
+{
+    if (sims.empty()) {
+        return;
+    }
+    mgo_context* ctx = default_context();
+    std::vector<mgo_graph_desc> descs;
+    std::vector<mgo_sim_cfg> cfgs;
+    std::vector<double> tm;
+    std::vector<size_t> offs;
+    size_t total = 0;
+    for (auto& s : sims) {
+        GraphArrays& a = s.batch_arrays();
+        mgo_sim_cfg c{};
+        c.t0         = double(s.get_t());
+        c.dt_graph   = double(s.get_dt());
+        c.dt_node    = a.dt_node;
+        c.horizon    = double(tmax);
+        c.integrator = a.integrator;
+        descs.push_back(a.desc);
+        cfgs.push_back(c);
+        tm.push_back(double(tmax));
+        offs.push_back(total);
+        const int nd = int(std::floor(double(tmax) + 1e-10) - std::ceil(c.t0 - 1e-10)) + 1;
+        total += size_t(std::max(nd, 0)) * size_t(a.n) * size_t(a.G) * MGO_NUM_COMPARTMENTS;
+    }
+    mgo_ensemble_desc ed{int32_t(sims.size()), descs.data(), cfgs.data(), tm.data()};
+    std::vector<double> out(std::max<size_t>(total, 1));
+    check(mgo_ensemble_run(ctx, &ed, out.data(), nullptr), ctx);
+    for (size_t i = 0; i < sims.size(); ++i) {
+        const int day0 = int(std::ceil(cfgs[i].t0 - 1e-10));
+        const int nd   = int(std::floor(double(tmax) + 1e-10)) - day0 + 1;
+        const size_t sz = size_t(std::max(nd, 0)) * size_t(descs[i].num_nodes) * size_t(descs[i].num_groups) *
+                          MGO_NUM_COMPARTMENTS;
+        std::vector<double> part(out.begin() + std::ptrdiff_t(offs[i]), out.begin() + std::ptrdiff_t(offs[i] + sz));
+        sims[i].set_batch_result(part, std::max(nd, 0), day0, tmax);
+    }
+}
+} // namespace detail
+
+template <class ParameterType, typename TimeType = double, typename StepType = TimeType>
+class ParameterStudy
+{
+public:
+    using Parameters = ParameterType;
+    using Time       = TimeType;
+    using Step       = StepType;
+
+    ParameterStudy(const Parameters& parameters, Time t0, Time tmax, Step dt, size_t num_runs)
+        : m_parameters(parameters)
+        , m_num_runs(num_runs)
+        , m_t0(t0)
+        , m_tmax(tmax)
+        , m_dt(dt)
+    {
+    }
+
+    // create_simulation(const Parameters&, Time t0, Step dt, size_t run_idx) -> Sim
+    // process_simulation_result(Sim&&, size_t run_idx) -> R or void
+    template <class CreateSimulationFunction, class ProcessSimulationResultFunction>
+    auto run(CreateSimulationFunction&& create_simulation, ProcessSimulationResultFunction&& process_simulation_result)
+    {
+        return run_serial(std::forward<CreateSimulationFunction>(create_simulation),
+                          std::forward<ProcessSimulationResultFunction>(process_simulation_result));
+    }
+    template <class CreateSimulationFunction>
+    auto run(CreateSimulationFunction&& create_simulation)
+    {
+        return run_serial(std::forward<CreateSimulationFunction>(create_simulation));
+    }
+
+    template <class CreateSimulationFunction, class ProcessSimulationResultFunction>
+    auto run_serial(CreateSimulationFunction&& create_simulation,
+                    ProcessSimulationResultFunction&& process_simulation_result)
+    {
+        using Sim = std::decay_t<std::invoke_result_t<CreateSimulationFunction, const Parameters&, Time, Step, size_t>>;
+        using R   = std::decay_t<std::invoke_result_t<ProcessSimulationResultFunction, Sim&&, size_t>>;
+        std::vector<Sim> sims;
+        sims.reserve(m_num_runs);
+        for (size_t i = 0; i < m_num_runs; ++i) {
+            sims.push_back(create_simulation(std::as_const(m_parameters), m_t0, m_dt, i));
+        }
+        if constexpr (detail::is_osecir_graph_sim<Sim>::value) {
+            detail::run_graph_batch(sims, m_tmax);
+        }
+        else {
+            for (auto& s : sims) {
+                s.advance(m_tmax);
+            }
+        }
+        if constexpr (std::is_void_v<R>) {
+            for (size_t i = 0; i < sims.size(); ++i) {
+                process_simulation_result(std::move(sims[i]), i);
+            }
+        }
+        else {
+            std::vector<R> out;
+            out.reserve(sims.size());
+            for (size_t i = 0; i < sims.size(); ++i) {
+                out.push_back(process_simulation_result(std::move(sims[i]), i));
+            }
+            return out;
+        }
+    }
+    template <class CreateSimulationFunction>
+    auto run_serial(CreateSimulationFunction&& create_simulation)
+    {
+        using Sim = std::decay_t<std::invoke_result_t<CreateSimulationFunction, const Parameters&, Time, Step, size_t>>;
+        return run_serial(std::forward<CreateSimulationFunction>(create_simulation), [](Sim&& s, size_t) {
+            return std::move(s);
+        });
+    }
+
+    size_t get_num_runs() const
+    {
+        return m_num_runs;
+    }
+    Time get_tmax() const
+    {
+        return m_tmax;
+    }
+    Time get_t0() const
+    {
+        return m_t0;
+    }
+    Step get_dt() const
+    {
+        return m_dt;
+    }
+    const Parameters& get_parameters() const
+    {
+        return m_parameters;
+    }
+    Parameters& get_parameters()
+    {
+        return m_parameters;
+    }
+
+private:
+    Parameters m_parameters;
+    size_t m_num_runs;
+    Time m_t0, m_tmax;
+    Step m_dt;
+};
+
+} // namespace mgo_mio
+
+#ifndef MGO_MIO_NO_ALIAS
+namespace mio = mgo_mio;
+#endif
+
+#endif // MGO_MIO_HPP

@@ -424,20 +424,27 @@ void build_edges(const mgo_graph_desc* g, EdgeStructHost& es, std::vector<double
     const int n  = g->num_nodes;
     const int G  = g->num_groups;
     const int nv = G * NC;
-    std::map<std::pair<int, int>, int> last; // (start, end) -> edge index of the last occurrence
+    // (start, end) keys sorted; a repeated key keeps its last occurrence (Graph::add_edge replaces)
+    std::vector<std::pair<uint64_t, int>> keys(size_t(std::max(g->num_edges, 0)));
     for (int e = 0; e < g->num_edges; ++e) {
-        last[{g->edge_start[e], g->edge_end[e]}] = e;
+        keys[size_t(e)] = {(uint64_t(uint32_t(g->edge_start[e])) << 32) | uint32_t(g->edge_end[e]), e};
     }
+    std::sort(keys.begin(), keys.end());
     struct E {
         int s, d;
         double cval;
         int cidx;
     };
     std::vector<E> edges;
-    edges.reserve(last.size());
+    edges.reserve(keys.size());
     const int len = (g->edge_coeff_len == 1) ? 1 : nv;
-    for (auto& kv : last) {
-        const double* c = g->edge_coeffs + size_t(kv.second) * len;
+    for (size_t k = 0; k < keys.size(); ++k) {
+        if (k + 1 < keys.size() && keys[k + 1].first == keys[k].first) {
+            continue; // a later occurrence of the same (start, end) follows
+        }
+        const int src   = int(keys[k].first >> 32);
+        const int dst   = int(uint32_t(keys[k].first));
+        const double* c = g->edge_coeffs + size_t(keys[k].second) * len;
         bool any_pos = false, uniform = true;
         for (int k = 0; k < len; ++k) {
             any_pos = any_pos || (c[k] > 0.0);
@@ -446,7 +453,7 @@ void build_edges(const mgo_graph_desc* g, EdgeStructHost& es, std::vector<double
         if (!any_pos) {
             continue;
         }
-        E ed{kv.first.first, kv.first.second, c[0], -1};
+        E ed{src, dst, c[0], -1};
         if (!uniform) {
             ed.cidx = int(cvec.size() / size_t(nv));
             cvec.insert(cvec.end(), c, c + nv);

@@ -157,3 +157,60 @@ class GraphSim:
         if self.id.value:
             self.ctx.lib.mgo_graph_destroy(self.ctx.ptr, self.id)
             self.id = C.c_int64(0)
+
+
+class Server:
+    """Continuous-batching request loop of a context (mgo_serve_*)."""
+
+    def __init__(self, ctx, max_resident=0, hbm_budget_bytes=0):
+        self.ctx = ctx
+        cfg = capi.ServeCfg(max_resident, hbm_budget_bytes)
+        ctx.check(ctx.lib.mgo_serve_start(ctx.ptr, C.byref(cfg)))
+        self.requests = {}
+        self.finished = {}   # polled, not yet taken: ticket -> ResultInfo
+        self.running = True
+
+    def submit(self, spec, cfg, tmax):
+        req = capi.Request(spec.desc(), cfg, tmax)
+        t = C.c_int64(0)
+        self.ctx.check(self.ctx.lib.mgo_serve_submit(self.ctx.ptr, C.byref(req), C.byref(t)))
+        self.requests[t.value] = (spec, cfg, tmax)
+        return t.value
+
+    def update(self, ticket, scale, t_start=-1.0, level=0, type=0):
+        u = capi.DampingUpdate(scale, t_start, level, type)
+        self.ctx.check(self.ctx.lib.mgo_serve_update(self.ctx.ptr, ticket, C.byref(u)))
+
+    def poll(self, capacity=256):
+        arr = (capi.ResultInfo * capacity)()
+        n = C.c_int32(0)
+        self.ctx.check(self.ctx.lib.mgo_serve_poll(self.ctx.ptr, arr, capacity, C.byref(n)))
+        return [arr[k] for k in range(n.value)]
+
+    def take(self, ticket):
+        """Daily values [day][node][10G] of a finished request."""
+        spec, cfg, tmax = self.requests.pop(ticket)
+        self.finished.pop(ticket, None)
+        nd = spec.num_days(cfg.t0, tmax)
+        out = np.zeros((nd, spec.num_nodes, spec.state_size()))
+        self.ctx.check(self.ctx.lib.mgo_serve_take(self.ctx.ptr, ticket, capi.dptr(out), out.size))
+        return out
+
+    def wait(self, tickets, timeout_s=60.0):
+        """Poll until every ticket in tickets has finished; returns {ticket: ResultInfo}."""
+        import time
+        want = set(tickets)
+        t_end = time.monotonic() + timeout_s
+        while want - set(self.finished):
+            for r in self.poll():
+                self.finished[r.ticket] = r
+            if want - set(self.finished):
+                if time.monotonic() > t_end:
+                    raise TimeoutError(f"requests {sorted(want - set(self.finished))} not finished in {timeout_s} s")
+                time.sleep(0.0005)
+        return {t: self.finished[t] for t in want}
+
+    def stop(self):
+        if self.running:
+            self.ctx.check(self.ctx.lib.mgo_serve_stop(self.ctx.ptr))
+            self.running = False

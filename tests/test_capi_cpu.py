@@ -49,3 +49,15 @@ def test_context_create_without_device_fails_cleanly():
     st = lib.mgo_context_create(C.byref(cfg), C.byref(p))
     assert st == capi.ERR_DEVICE
     assert not p.value
+
+
+def test_cpp_shim_host_checks():
+    """The C++ mio:: call-shape layer builds, links the engine and the oracle, and passes its host-only
+    checks (edge order and replacement, interpolation, populations); without a device it stops there."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(__file__), "cpp", "test_mio_shim")
+    assert os.path.exists(exe), "tests/cpp/test_mio_shim is built by __graft_entry__.build()"
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert "host checks: ok" in r.stdout, r.stdout
+    assert r.returncode == 0, r.stdout

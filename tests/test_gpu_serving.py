@@ -1,0 +1,124 @@
+"""Serving loop (mgo_serve_*) on the device: a request's result is the same alone and in any mix of
+other requests (bitwise), matches the oracle to the parity bars, and a scenario update applied to a
+resident request equals the oracle run with the same damping added to every node's contact matrix
+(ContactMatrix::add_damping, cpp/memilio/epidemiology/contact_matrix.h:190-194)."""
+import numpy as np
+import pytest
+
+from memilio_amd import capi, synthetic
+from memilio_amd.engine import Context, MgoError, Server
+from tests import setups
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_err(a, b):
+    assert a.shape == b.shape, (a.shape, b.shape)
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-6))) if a.size else 0.0
+
+
+def _requests():
+    """Mixed classes: (G, integrator) differ, sizes differ, one single-node model, one fixed-step run."""
+    a = synthetic.graph(num_nodes=60, G=3, seed=11, mean_degree=8.0)
+    b = synthetic.graph(num_nodes=25, G=6, seed=12, mean_degree=5.0)
+    c = synthetic.single_model(seed=13, G=2)
+    d = synthetic.graph(num_nodes=90, G=1, seed=14, mean_degree=10.0)
+    e = synthetic.graph(num_nodes=40, G=3, seed=15, mean_degree=6.0)
+    return [
+        (a, synthetic.ck_cfg(1e-8), 30.0, 1e-6),
+        (b, synthetic.fixed_cfg(0.1), 21.0, 1e-12),
+        (c, setups.rkf45_cfg(1e-8, 1e-8, 1e-6, 1.0), 45.0, 1e-6),
+        (d, synthetic.ck_cfg(1e-10), 17.0, 1e-6),
+        (e, synthetic.ck_cfg(1e-6), 33.0, 1e-6),
+    ]
+
+
+def _oracle_daily(oracle, spec, cfg, tmax):
+    ref, _ = oracle.graph_run(spec, cfg, tmax)
+    return ref
+
+
+def test_serve_alone_vs_mix(oracle):
+    reqs = _requests()
+    with Context() as ctx:
+        srv = Server(ctx)
+        t = srv.submit(*reqs[0][:3])
+        srv.wait([t])
+        alone = srv.take(t)
+        # the same request in a mix, submitted third; the others arrive around it
+        order = [1, 3, 0, 2, 4]
+        tickets = {i: srv.submit(*reqs[i][:3]) for i in order}
+        infos = srv.wait(tickets.values())
+        mixed = {i: srv.take(tickets[i]) for i in order}
+        srv.stop()
+    for i, tk in tickets.items():
+        assert infos[tk].status == capi.OK
+        assert infos[tk].admit_ms >= infos[tk].submit_ms
+        assert infos[tk].done_ms >= infos[tk].admit_ms
+    assert np.array_equal(alone, mixed[0]), "result differs alone vs in a mix"
+    for i, (spec, cfg, tmax, tol) in enumerate(reqs):
+        ref = _oracle_daily(oracle, spec, cfg, tmax)
+        assert rel_err(mixed[i], ref) < tol, i
+
+
+def test_serve_queueing_and_budget(oracle):
+    """max_resident = 2: requests queue and are admitted as others retire; results unchanged."""
+    reqs = _requests()
+    with Context() as ctx:
+        srv = Server(ctx, max_resident=2)
+        tickets = [srv.submit(*r[:3]) for r in reqs]
+        infos = srv.wait(tickets)
+        outs = [srv.take(tk) for tk in tickets]
+        srv.stop()
+    admits = sorted(infos[tk].admit_ms for tk in tickets)
+    dones = sorted(infos[tk].done_ms for tk in tickets)
+    # the third admission waits for the first retirement
+    assert admits[2] >= dones[0]
+    for (spec, cfg, tmax, tol), out in zip(reqs, outs):
+        assert rel_err(out, _oracle_daily(oracle, spec, cfg, tmax)) < tol
+
+
+def test_serve_damping_update(oracle):
+    """Two scenario updates queued before admission (the request waits behind a long one with
+    max_resident = 1) equal the oracle with the same dampings on every node from the start."""
+    blocker = synthetic.graph(num_nodes=400, G=6, seed=21)
+    spec = synthetic.graph(num_nodes=50, G=3, seed=22, mean_degree=6.0)
+    cfg, tmax = synthetic.fixed_cfg(0.1), 60.0
+    ups = [(0.4, 20.0, 0, 0), (0.8, 35.0, 1, 0)]
+    with Context() as ctx:
+        srv = Server(ctx, max_resident=1)
+        tb = srv.submit(blocker, synthetic.ck_cfg(1e-10), 200.0)
+        tk = srv.submit(spec, cfg, tmax)
+        for s, ts, lev, typ in ups:
+            srv.update(tk, s, ts, lev, typ)
+        infos = srv.wait([tb, tk], timeout_s=120.0)
+        assert infos[tk].admit_ms >= infos[tb].done_ms
+        srv.take(tb)
+        got = srv.take(tk)
+        srv.stop()
+    damped = synthetic.graph(num_nodes=50, G=3, seed=22, mean_degree=6.0)
+    damped.set_dampings([[(ts, lev, typ, 1.0 - s) for s, ts, lev, typ in ups]] * damped.num_nodes)
+    ref = _oracle_daily(oracle, damped, cfg, tmax)
+    undamped = _oracle_daily(oracle, spec, cfg, tmax)
+    assert rel_err(got, ref) < 1e-12
+    assert rel_err(got, undamped) > 1e-3   # the update changed the course
+
+
+def test_serve_errors():
+    with Context() as ctx:
+        srv = Server(ctx)
+        spec = synthetic.single_model(seed=1, G=2)
+        tk = srv.submit(spec, synthetic.ck_cfg(1e-8), 5.0)
+        with pytest.raises(MgoError) as e:
+            srv.update(tk, 1.5)
+        assert e.value.status == capi.ERR_INVALID_VALUE
+        with pytest.raises(MgoError) as e:
+            srv.update(tk + 1000, 0.5)
+        assert e.value.status == capi.ERR_NOT_FOUND
+        srv.wait([tk])
+        out = srv.take(tk)
+        assert out.shape == (6, 1, 20)
+        with pytest.raises(MgoError) as e:
+            Server(ctx)   # already serving
+        assert e.value.status == capi.ERR_INVALID_VALUE
+        srv.stop()

@@ -5,7 +5,7 @@ These pin the oracle before it is trusted as the checker of the device kernels.
 import numpy as np
 import pytest
 
-from memilio_amd import capi
+from memilio_amd import capi, synthetic
 from tests import setups
 
 
@@ -123,3 +123,12 @@ def test_adaptive_step_size_contract(oracle):
         dts = np.diff(t)
         assert np.all(dts[:-1] >= 0.01 - 1e-15) and np.all(dts <= 0.5 + 1e-15)
         assert t[-1] == pytest.approx(10.0, abs=1e-12)
+
+
+def test_fixup_scenario_triggers(oracle):
+    """The GPU fix-up tests rely on this set-up driving the return fix-up; keep it doing so."""
+    spec = setups.fixup_graph()
+    for cfg in (synthetic.fixed_cfg(0.1), synthetic.ck_cfg(1e-10)):
+        _, st = oracle.graph_run(spec, cfg, 20.0)
+        assert st.negative_fixups > 0
+        assert st.returns > 0
