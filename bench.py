@@ -49,67 +49,66 @@ def node_flops(steps, attempts, G):
     return steps * rhs_flops(G) + attempts * (5 * rhs_flops(G) + 540 * G)
 
 
-def ensemble_bench(args, specs, cfgs, tmaxs):
-    """Timed ensemble steps (events around every node / edge launch) and one graph-replayed run."""
+def ensemble_bench(args, ctx, specs, cfgs, tmaxs):
+    """One hipGraph-replayed run (one graph launch per half-day cycle), then the timed steps with events
+    around every node / edge launch."""
     S, N, days = len(specs), specs[0].num_nodes, tmaxs[0]
     node_days = S * N * days
     out = {}
-    with Context(flags=capi.CTX_PROFILE_KERNELS) as ctx:
-        t = time.perf_counter()
-        batch = Batch(ctx, specs, cfgs, tmaxs)
-        log(f"batch resident in HBM in {time.perf_counter() - t:.1f}s")
-        for w in range(args.warmup):
-            batch.reset()
-            info = batch.run()
-            log(f"warmup {w}: {info.device_ms:.1f} ms device")
-        times, infos = [], []
-        for k in range(args.steps):
-            batch.reset()
-            t0 = time.perf_counter()
-            info = batch.run()
-            dt = time.perf_counter() - t0
-            times.append(dt)
-            infos.append(info)
-            log(f"step {k}: {dt * 1e3:.1f} ms wall, {info.device_ms:.1f} ms device, node kernel "
-                f"{info.node_kernel_ms:.1f} ms / {info.node_launches} launches, edge kernels "
-                f"{info.edge_kernel_ms:.1f} ms, steps {info.steps}, attempts {info.attempts}, fix-up passes "
-                f"{info.fixup_passes}, replays {info.exact_replays}")
-        total = sum(times)
-        out["value"] = node_days * args.steps / total
-        out["ms_per_step"] = total / args.steps * 1e3
-        out["infos"] = infos
-        out["sample"] = batch.read_daily(0, min(args.cpu_samples, S)) if not args.no_cpu else None
-        batch.close()
-    with Context() as gctx:
-        gb = Batch(gctx, specs, cfgs, tmaxs)
-        gb.run()               # capture + warm-up
-        gb.reset()
-        t0 = time.perf_counter()
-        ginfo = gb.run()
-        gs = time.perf_counter() - t0
-        gb.close()
+    t = time.perf_counter()
+    batch = Batch(ctx, specs, cfgs, tmaxs)
+    log(f"batch resident in HBM in {time.perf_counter() - t:.1f}s")
+    ctx.set_flags(0)
+    batch.run()               # capture + warm-up
+    batch.reset()
+    t0 = time.perf_counter()
+    ginfo = batch.run()
+    gs = time.perf_counter() - t0
     out["graph_value"] = node_days / gs
     log(f"hipGraph-replayed run: {gs * 1e3:.1f} ms wall ({ginfo.device_ms:.1f} ms device) -> "
         f"{out['graph_value']:.3e} node-days/s")
+    ctx.set_flags(capi.CTX_PROFILE_KERNELS)
+    for w in range(args.warmup):
+        batch.reset()
+        info = batch.run()
+        log(f"warmup {w}: {info.device_ms:.1f} ms device")
+    times, infos = [], []
+    for k in range(args.steps):
+        batch.reset()
+        t0 = time.perf_counter()
+        info = batch.run()
+        dt = time.perf_counter() - t0
+        times.append(dt)
+        infos.append(info)
+        log(f"step {k}: {dt * 1e3:.1f} ms wall, {info.device_ms:.1f} ms device, node kernel "
+            f"{info.node_kernel_ms:.1f} ms / {info.node_launches} launches, edge kernels "
+            f"{info.edge_kernel_ms:.1f} ms, steps {info.steps}, attempts {info.attempts}, fix-up passes "
+            f"{info.fixup_passes}, replays {info.exact_replays}")
+    ctx.set_flags(0)
+    total = sum(times)
+    out["value"] = node_days * args.steps / total
+    out["ms_per_step"] = total / args.steps * 1e3
+    out["infos"] = infos
+    out["sample"] = batch.read_daily(0, min(args.cpu_samples, S)) if not args.no_cpu else None
+    batch.close()
     return out
 
 
-def plain_bench(base, cfg, days):
-    with Context() as ctx:
-        plain = GraphSim(ctx, base, cfg, horizon=days)
-        plain.advance(1.0)   # warm-up (graph capture, first launches)
-        plain.close()
-        plain = GraphSim(ctx, base, cfg, horizon=days)
-        t0 = time.perf_counter()
-        plain.advance(days)
-        plain_s = time.perf_counter() - t0
-        plain.close()
+def plain_bench(ctx, base, cfg, days):
+    plain = GraphSim(ctx, base, cfg, horizon=days)
+    plain.advance(1.0)   # warm-up (graph capture, first launches)
+    plain.close()
+    plain = GraphSim(ctx, base, cfg, horizon=days)
+    t0 = time.perf_counter()
+    plain.advance(days)
+    plain_s = time.perf_counter() - t0
+    plain.close()
     value = base.num_nodes * days / plain_s
     log(f"plain single graph: {plain_s * 1e3:.1f} ms -> {value:.3e} node-days/s")
     return value
 
 
-def serving_bench(args):
+def serving_bench(args, ctx):
     """Poisson request streams over one serving loop per rate; requests drawn from the uneven-batch
     distribution. Latency = completion - submission on the context clock."""
     t = time.perf_counter()
@@ -123,41 +122,40 @@ def serving_bench(args):
         is_update = rng.random(n_arr) < 0.1
         picks = rng.integers(0, len(pool), n_arr)
         scales = rng.uniform(0.3, 1.0, n_arr)
-        with Context() as ctx:
-            srv = Server(ctx)
-            live, tickets, updates = [], [], 0
-            meta = {}
-            t_start = time.perf_counter()
-            t_next = t_start
-            for k in range(n_arr):
-                t_next += gaps[k]
-                while True:
-                    now = time.perf_counter()
-                    if now >= t_next:
-                        break
-                    time.sleep(min(t_next - now, 0.002))
-                if is_update[k]:
-                    for r in srv.poll():
-                        srv.finished[r.ticket] = r
-                    live = [tk for tk in live if tk not in srv.finished]
-                    if live:
-                        tk = live[int(rng.integers(0, len(live)))]
-                        try:
-                            srv.update(tk, float(scales[k]))
-                            updates += 1
-                        except Exception:
-                            pass   # finished between poll and update
-                    continue
-                spec, horizon, tol = pool[picks[k]]
-                tk = srv.submit(spec, synthetic.ck_cfg(tol), horizon)
-                tickets.append(tk)
-                live.append(tk)
-                meta[tk] = spec.num_nodes * horizon
-            infos = srv.wait(tickets, timeout_s=600.0)
-            t_end = time.perf_counter()
-            for tk in tickets:
-                srv.take(tk)
-            srv.stop()
+        srv = Server(ctx)
+        live, tickets, updates = [], [], 0
+        meta = {}
+        t_start = time.perf_counter()
+        t_next = t_start
+        for k in range(n_arr):
+            t_next += gaps[k]
+            while True:
+                now = time.perf_counter()
+                if now >= t_next:
+                    break
+                time.sleep(min(t_next - now, 0.002))
+            if is_update[k]:
+                for r in srv.poll():
+                    srv.finished[r.ticket] = r
+                live = [tk for tk in live if tk not in srv.finished]
+                if live:
+                    tk = live[int(rng.integers(0, len(live)))]
+                    try:
+                        srv.update(tk, float(scales[k]))
+                        updates += 1
+                    except Exception:
+                        pass   # finished between poll and update
+                continue
+            spec, horizon, tol = pool[picks[k]]
+            tk = srv.submit(spec, synthetic.ck_cfg(tol), horizon)
+            tickets.append(tk)
+            live.append(tk)
+            meta[tk] = spec.num_nodes * horizon
+        infos = srv.wait(tickets, timeout_s=600.0)
+        t_end = time.perf_counter()
+        for tk in tickets:
+            srv.take(tk)
+        srv.stop()
         bad = [tk for tk in tickets if infos[tk].status != capi.OK]
         lat = np.array([infos[tk].done_ms - infos[tk].submit_ms for tk in tickets])
         first = min(infos[tk].submit_ms for tk in tickets)
@@ -206,7 +204,8 @@ def main():
     tmaxs = [days] * S
     log(f"inputs generated in {time.perf_counter() - t:.1f}s, {base.num_edges} edges")
 
-    ens = ensemble_bench(args, specs, cfgs, tmaxs)
+    ctx = Context()   # one context (one stream) for every measurement of this process
+    ens = ensemble_bench(args, ctx, specs, cfgs, tmaxs)
     infos = ens["infos"]
     node_ms_sum = sum(i.node_kernel_ms for i in infos)
     launches = sum(i.node_launches for i in infos)
@@ -218,8 +217,9 @@ def main():
     log(f"node kernel: {launches} launches, avg {node_ms:.3f} ms, {achieved:.2f} TFLOP/s; edge phase avg "
         f"{edge_ms:.3f} ms per cycle")
 
-    plain_value = None if args.no_plain else plain_bench(base, cfg, days)
-    serving = None if args.no_serve else serving_bench(args)
+    plain_value = None if args.no_plain else plain_bench(ctx, base, cfg, days)
+    serving = None if args.no_serve else serving_bench(args, ctx)
+    ctx.close()
 
     cpu = None
     max_rel = None

@@ -164,6 +164,8 @@ typedef struct mgo_device_cfg {
 
 mgo_status mgo_context_create(const mgo_device_cfg* cfg, mgo_context** out);
 mgo_status mgo_context_destroy(mgo_context* ctx);
+/* Replaces the MGO_CTX_* flags; they apply to every batch run and graph advance started afterwards. */
+mgo_status mgo_context_set_flags(mgo_context* ctx, int32_t flags);
 /* Last error message of the context (thread-local copy, valid until the next call on that context). */
 const char* mgo_last_error(const mgo_context* ctx);
 /* Version string of the library. */

@@ -272,15 +272,22 @@ struct ClassHost {
     std::vector<int> insts; // global instance indices
     int num_nodes = 0;
     // device buffers
-    DevBuf y, snap, ypre, fac, rc, lcoef, cbase, cdelta, nodep, dtime, dmat, tnode, dtnode, steps, attempts, out,
+    DevBuf y, snap, ypre, rc, lcoef, cbase, cdelta, nodep, dtime, dmat, tnode, dtnode, steps, attempts, out,
         ipool, epool, cvec, estructs, inst_ids;
-    DevBuf fx[2], fx_pool[2];          // fix-up records of the return passes
+    DevBuf fx_rec, fx_pool, fx_dirty;  // fix-up records of the return passes
     DevBuf y0;                         // initial state, for reset
     std::vector<double> tnode0, dtnode0; // initial clocks, for reset
     size_t out_doubles = 0;
     DevBuf seg_dev, block_seg, block_first; // launch tables (one segment = the whole class)
     SegDev seg_host{};                       // host copy of the segment descriptor
     LaunchDev launch{};
+
+    // fix-up record versions and dirty marks to "none" (tag -1, below every epoch)
+    void clear_fixup_records(hipStream_t s)
+    {
+        hip_check(hipMemsetAsync(fx_rec.p, 0xff, fx_rec.bytes, s), "clear fix-up records");
+        hip_check(hipMemsetAsync(fx_dirty.p, 0xff, fx_dirty.bytes, s), "clear fix-up marks");
+    }
 };
 
 // blocks of a segment of n nodes with G age groups (BLOCK/WAVE wavefronts of 64/G nodes per block)
@@ -734,11 +741,10 @@ struct Batch {
         up(C.y0, y);
         C.tnode0  = tnode;
         C.dtnode0 = dtnode;
-        C.snap.alloc(y.size() * sizeof(double));
+        C.snap.alloc(N * G * SNAP_STRIDE * sizeof(double));
         C.ypre.alloc(y.size() * sizeof(double));
-        C.fac.alloc(N * G * sizeof(double));
         C.rc.alloc(N * G * RC_STRIDE * sizeof(double));
-        device_bytes += C.snap.bytes + C.ypre.bytes + C.fac.bytes + C.rc.bytes;
+        device_bytes += C.snap.bytes + C.ypre.bytes + C.rc.bytes;
         up(C.lcoef, lcoef);
         up(C.cbase, cbase);
         up(C.cdelta, cdelta);
@@ -780,11 +786,11 @@ struct Batch {
         }
         up(C.estructs, estructs);
         up(C.inst_ids, inst_ids);
-        for (int k = 0; k < 2; ++k) {
-            C.fx[k].alloc(size_t(std::max<int64_t>(fx_total, 1)) * sizeof(int32_t));
-            C.fx_pool[k].alloc(size_t(std::max<int64_t>(pool_slots, 1)) * FX_SLOTS * G * NC * sizeof(double));
-            device_bytes += C.fx[k].bytes + C.fx_pool[k].bytes;
-        }
+        C.fx_rec.alloc(size_t(std::max<int64_t>(fx_total, 1)) * 2 * sizeof(FxRec));
+        C.fx_pool.alloc(size_t(std::max<int64_t>(pool_slots, 1)) * FX_SLOTS * G * NC * sizeof(double));
+        C.fx_dirty.alloc(2 * std::max<size_t>(N, 1) * sizeof(int32_t));
+        device_bytes += C.fx_rec.bytes + C.fx_pool.bytes + C.fx_dirty.bytes;
+        C.clear_fixup_records(stream);
         for (int ii : C.insts) {
             idev[size_t(ii)].out = C.out.as<double>() + inst[size_t(ii)].out_base;
         }
@@ -804,7 +810,6 @@ struct Batch {
         d.y         = C.y.as<double>();
         d.snap      = C.snap.as<double>();
         d.ypre      = C.ypre.as<double>();
-        d.fac       = C.fac.as<double>();
         d.rc        = C.rc.as<double>();
         d.lcoef     = C.lcoef.as<double>();
         d.cbase     = C.cbase.as<double>();
@@ -815,10 +820,9 @@ struct Batch {
         d.steps     = C.steps.as<uint32_t>();
         d.attempts  = C.attempts.as<uint32_t>();
         d.estructs  = C.estructs.as<EdgeStructDev>();
-        for (int k = 0; k < 2; ++k) {
-            d.fx[k]      = C.fx[k].as<int32_t>();
-            d.fx_pool[k] = C.fx_pool[k].as<double>();
-        }
+        d.fx_rec    = C.fx_rec.as<FxRec>();
+        d.fx_pool   = C.fx_pool.as<double>();
+        d.fx_dirty  = C.fx_dirty.as<int32_t>();
         C.seg_host = d;
         std::vector<SegDev> segs{d};
         std::vector<int32_t> bseg, bfirst;
@@ -987,6 +991,7 @@ struct Batch {
                       "reset dt");
             hip_check(hipMemsetAsync(C.steps.p, 0, C.steps.bytes, stream), "reset steps");
             hip_check(hipMemsetAsync(C.attempts.p, 0, C.attempts.bytes, stream), "reset attempts");
+            C.clear_fixup_records(stream);
         }
         hip_check(hipMemcpyAsync(inst_dev.p, inst0.data(), inst0.size() * sizeof(InstDev), hipMemcpyHostToDevice,
                                  stream),
@@ -1165,6 +1170,14 @@ Batch* find_batch(std::unordered_map<int64_t, std::unique_ptr<Batch>>& m, int64_
 
 #include "mgo_serve.inc"
 
+// run-time options of a batch from the context flags (read at every run)
+static void apply_flags(Batch& b, int32_t flags)
+{
+    b.use_graph        = !(flags & MGO_CTX_NO_GRAPH_CAPTURE);
+    b.profile          = (flags & MGO_CTX_PROFILE_KERNELS) != 0;
+    b.sequential_fixup = (flags & MGO_CTX_SEQUENTIAL_FIXUP) != 0;
+}
+
 extern "C" {
 
 const char* mgo_version(void)
@@ -1196,6 +1209,16 @@ mgo_status mgo_context_create(const mgo_device_cfg* cfg, mgo_context** out)
     ctx->hbm_budget = (cfg && cfg->hbm_budget_bytes) ? cfg->hbm_budget_bytes : size_t(double(free_b) * 0.9);
     *out            = ctx.release();
     return MGO_OK;
+}
+
+mgo_status mgo_context_set_flags(mgo_context* ctx, int32_t flags)
+{
+    return guarded(ctx, [&] {
+        if (flags & ~(MGO_CTX_NO_GRAPH_CAPTURE | MGO_CTX_PROFILE_KERNELS | MGO_CTX_SEQUENTIAL_FIXUP)) {
+            fail(MGO_ERR_INVALID_VALUE, "unknown context flags 0x%x", unsigned(flags));
+        }
+        ctx->flags = flags;
+    });
 }
 
 mgo_status mgo_context_destroy(mgo_context* ctx)
@@ -1247,9 +1270,7 @@ mgo_status mgo_graph_create(mgo_context* ctx, const mgo_graph_desc* graph, const
         std::vector<RunSpec> runs(1);
         runs[0] = RunSpec{graph, *cfg, cfg->horizon, false, {}};
         auto b  = std::make_unique<Batch>();
-        b->use_graph = !(ctx->flags & MGO_CTX_NO_GRAPH_CAPTURE);
-        b->profile   = (ctx->flags & MGO_CTX_PROFILE_KERNELS) != 0;
-        b->sequential_fixup = (ctx->flags & MGO_CTX_SEQUENTIAL_FIXUP) != 0;
+        apply_flags(*b, ctx->flags);
         b->build(runs, ctx->stream);
         *out = ctx->next_id++;
         ctx->graphs[*out] = std::move(b);
@@ -1266,6 +1287,7 @@ mgo_status mgo_graph_advance(mgo_context* ctx, mgo_graph_id id, double tmax)
         if (day_floor(tmax) > b->inst[0].day0 + b->inst[0].ndays - 1 && b->inst[0].ndays >= 0) {
             fail(MGO_ERR_OUT_OF_RANGE, "tmax %g beyond the horizon given at creation", tmax);
         }
+        apply_flags(*b, ctx->flags);
         b->set_tmax(0, tmax);
         b->run(nullptr);
         b->check_status();
@@ -1367,9 +1389,7 @@ static void build_ensemble(mgo_context* ctx, const mgo_ensemble_desc* desc, Batc
         }
         runs[size_t(r)] = RunSpec{&desc->graphs[r], desc->cfgs[r], desc->tmax[r], false, {}};
     }
-    b.use_graph = !(ctx->flags & MGO_CTX_NO_GRAPH_CAPTURE);
-    b.profile   = (ctx->flags & MGO_CTX_PROFILE_KERNELS) != 0;
-    b.sequential_fixup = (ctx->flags & MGO_CTX_SEQUENTIAL_FIXUP) != 0;
+    apply_flags(b, ctx->flags);
     b.build(runs, ctx->stream);
     for (int r = 0; r < desc->num_runs; ++r) {
         b.inst[size_t(r)].tmax = desc->tmax[r];
@@ -1453,6 +1473,7 @@ mgo_status mgo_batch_run(mgo_context* ctx, mgo_batch_id id, mgo_run_info* info)
 {
     return guarded(ctx, [&] {
         Batch* b          = find_batch(ctx->batches, id);
+        apply_flags(*b, ctx->flags);
         const int64_t cyc = b->cycles_needed();
         double ms         = 0;
         b->run(&ms);

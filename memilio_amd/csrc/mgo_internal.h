@@ -6,10 +6,11 @@
 // launch covers the segments of one class through a block -> segment table; kernels are specialised
 // per (G, integrator). Within a segment:
 //   lane slot   = node * G + g   (one GPU lane per (node, age group); 64/G nodes per wavefront)
-//   y, snap, ypre       [node][g][10]          state, departure snapshot, pre-return state
+//   y, ypre             [node][g][10]          state, pre-return state
+//   snap                [node][g][SNAP_STRIDE] departure row: snapshot (10), ISy mobility factor, commuter
+//                                              nondetection factor at the departure time
 //   lcoef               [node][g][NUM_LC]      per-lane rate coefficients (see LaneCoef)
 //   cbase, cdelta       [node][g][G]           contact row B_g., B_g. - M_g.
-//   fac                 [node][g]              ISy mobility factor of the departure snapshot
 //   rc                  [node][g][RC_STRIDE]   return coefficients of the node as a destination, frozen at
 //                                              the departure time (see RetCoef)
 //   nodep               [node]                 scalar parameters + instance index
@@ -70,6 +71,13 @@ enum RetCoef : int {
     RC_STRIDE = 12
 };
 
+// departure row of a lane: the snapshot of SimulationNode::m_last_state, the ISy mobility factor
+// (get_mobility_factors) and the commuter nondetection factor (test_commuters) at the departure time;
+// 96 bytes, read with 16-byte loads
+constexpr int SNAP_FAC    = 10;
+constexpr int SNAP_ND     = 11;
+constexpr int SNAP_STRIDE = 12;
+
 // node flags
 enum : int32_t {
     NF_TTC_ACTIVE  = 1, // TestAndTraceCapacity < DBL_MAX: the risk smoother depends on the state
@@ -119,21 +127,30 @@ struct InstDev {
     int32_t replays;   // return phases resolved by the sequential replay so far
     int32_t exact_end; // single-model run: the last step ends exactly at tmax
     // negative-value fix-up iteration of the current return phase (see return_walk)
-    int64_t fx_base;      // first in-edge record of this instance in ClassDev::fx[*]
-    int64_t fx_pool_base; // first slot of this instance in ClassDev::fx_pool[*] (doubles)
+    int64_t fx_base;      // first in-edge record pair of this instance in SegDev::fx_rec (in pairs)
+    int64_t fx_pool_base; // first double of this instance's slots in SegDev::fx_pool
     int32_t fx_pass;      // pass being run (0 = the first return walk)
     int32_t fx_state;     // 0 first pass, 1 iterating, 2 sequential replay needed, 3 settled
-    int32_t fx_changed;   // set by a pass whose fix-up records differ from the previous pass
-    int32_t fx_count[2];  // slots used by the records of the even / odd pass
+    int32_t fx_changed;   // set by a pass in which a fix-up record changed
+    int32_t fx_count;     // pool slots used in this return phase
     int32_t fx_passes;    // extra passes run so far (statistics)
     int32_t fx_sequential; // resolve fix-ups with the sequential replay only
     int32_t seg;          // segment (index into LaunchDev::segs of its class) holding the nodes
-    int32_t pad2;
+    int32_t fx_epoch;     // record tag of pass 0 of the current return phase (grows by FX_EPOCH_STEP)
 };
 
-// fix-up records: slots per instance and pass parity; passes before falling back to the sequential replay
-constexpr int FX_SLOTS      = 512;
-constexpr int FX_MAX_PASSES = 4;
+// Fix-up records. Each in-edge keeps two tagged versions {tag, slot}; tag = fx_epoch + pass of the pass
+// that wrote it, slot = pool slot of the fixed traveler vector or -1 (unfixed). A pass p reads the newest
+// version tagged below fx_epoch + p and overwrites the older one, so readers of the previous pass's
+// records never race with writers of the current pass. Tags of earlier return phases are below fx_epoch
+// and read as "no record" (unfixed).
+struct alignas(8) FxRec {
+    int32_t tag;
+    int32_t slot;
+};
+constexpr int FX_SLOTS      = 1024; // pool slots per instance and return phase
+constexpr int FX_MAX_PASSES = 8;    // passes before falling back to the sequential replay
+constexpr int FX_EPOCH_STEP = 16;   // > FX_MAX_PASSES
 
 // CSR edge structure of one graph (node indices local to the instance). In-edges of node v are sorted
 // by source; in_split[v] = number of in-edges whose source < v, so walking
@@ -166,7 +183,6 @@ struct SegDev {
     double* y;
     double* snap;
     double* ypre;
-    double* fac;
     double* rc;
     const double* lcoef;
     const double* cbase;
@@ -177,8 +193,10 @@ struct SegDev {
     uint32_t* steps;
     uint32_t* attempts;
     const EdgeStructDev* estructs; // indexed by InstDev::estruct
-    int32_t* fx[2];                // fix-up record per in-edge entry: slot or -1, even / odd pass
-    double* fx_pool[2];            // fixed traveler vectors [slot][G][10], even / odd pass
+    FxRec* fx_rec;                 // [in-edge][2] tagged fix-up record versions
+    double* fx_pool;               // fixed traveler vectors [slot][G][10]
+    int32_t* fx_dirty;             // [2][node] tag of the pass that changed a record of the node's out-edges
+                                   // (plane = pass parity)
 };
 
 // One kernel launch over the segments of a class.

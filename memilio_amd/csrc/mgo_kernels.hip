@@ -8,9 +8,13 @@
 //
 // Kernels per half-day cycle (GraphSimulation::advance, cpp/memilio/mobility/graph_simulation.h:101-120):
 //   clock_kernel       per instance: step size, edge action and output day of this cycle
-//   node_phase<G,K>    per node lane: SystemIntegrator::advance to t+dt with integrator K
-//   edge_phase<G>      per node lane: the node's departures or returns of all incident edges, walked in
-//                      the reference's edge order (deterministic segmented sum, no atomics)
+//   node_phase<G,K>    per node lane: SystemIntegrator::advance to t+dt with integrator K; writes the
+//                      departure row (departure cycles) or the pre-return state and the return
+//                      coefficients (return cycles)
+//   depart_phase<G>    per node lane: the node's departures over all incident edges, walked in the
+//                      reference's edge order (deterministic segmented sum, no atomics)
+//   return_phase<G>    per node lane: the returns with the negative-value fix-up, in parallel passes over
+//                      the nodes whose inputs changed (fixup_check between passes)
 //   replay_phase<G>    per flagged instance: sequential return pass with the negative-value fix-up
 #include <hip/hip_runtime.h>
 
@@ -368,6 +372,12 @@ __device__ __forceinline__ const SegDev* load_lane(const LaunchDev& ld, Lane<G>&
     return c;
 }
 
+__device__ __forceinline__ double detect_nondetection(const NodeP& np, double t)
+{
+    // test_commuters (model.h:717-742): nondetection applies in [start, end)
+    return (t >= np.det_start && t < np.det_end) ? np.nondetect : 1.0;
+}
+
 // mobility factor of the ISy compartment of the lane's age group (model.h:687-715)
 template <int G>
 __device__ __forceinline__ double mobility_factor(const Lane<G>& L, const double (&y)[NC])
@@ -394,9 +404,8 @@ __global__ void clock_kernel(InstDev* inst, int n)
     I.c_out_day = -1;
     I.fx_pass   = 0;
     I.fx_state  = 0;
-    I.fx_changed  = 0;
-    I.fx_count[0] = 0;
-    I.fx_count[1] = 0;
+    I.fx_changed = 0;
+    I.fx_count   = 0;
     if (I.status != 0 || !(I.t < I.tmax)) {
         return;
     }
@@ -434,6 +443,9 @@ __global__ void clock_kernel(InstDev* inst, int n)
             }
         }
         I.toggle ^= 1;
+    }
+    if (I.c_action == 2) {
+        I.fx_epoch += FX_EPOCH_STEP; // fix-up records of earlier return phases read as unfixed
     }
     I.t             = tn;
     const double dn = rint(tn);
@@ -491,6 +503,24 @@ __global__ void __launch_bounds__(BLOCK) node_phase(LaunchDev ld, InstDev* __res
     int next_day     = max(0, int(floor(I.c_t0 + 1e-10)) + 1 - I.day0);
     const double tde = I.c_t1 - 1e-10;
 
+    if (I.c_action == 2) {
+        // return coefficients of this node as a destination: its state after the departures, at the
+        // departure time (the `pop` of calculate_mobility_returns, metapopulation_mobility_instant.h:460-471)
+        double c_icr, c_d;
+        const double kappa = force_of_infection<G>(c, L, y, I.c_tdep, c_icr, c_d);
+        double* rc         = c.rc + (size_t(L.node) * G + L.age) * RC_STRIDE;
+        rc[RC_KAPPA]       = kappa;
+        rc[RC_E]           = L.c[LC_E];
+        rc[RC_INS_ISY]     = L.c[LC_INS_ISY];
+        rc[RC_INS_R]       = L.c[LC_INS_R];
+        rc[RC_ISY_ISEV]    = L.c[LC_ISY_ISEV];
+        rc[RC_ISY_R]       = L.c[LC_ISY_R];
+        rc[RC_ISEV_ICR]    = c_icr;
+        rc[RC_ISEV_R]      = L.c[LC_ISEV_R];
+        rc[RC_ISEV_D]      = c_d;
+        rc[RC_ICR_D]       = L.c[LC_ICR_D];
+        rc[RC_ICR_R]       = L.c[LC_ICR_R];
+    }
     bool active  = fabs((tmax - t) / (tmax - t0)) > eps;
     bool need_k1 = true;
     bool fail = true, valid = true;
@@ -640,12 +670,15 @@ __global__ void __launch_bounds__(BLOCK) node_phase(LaunchDev ld, InstDev* __res
     }
     const size_t off = (size_t(L.node) * G + L.age) * NC;
     if (I.c_action == 1) {
-        // SimulationNode::m_last_state: departures and mobility factors use this snapshot
+        // departure row: SimulationNode::m_last_state, the ISy mobility factor and the commuter
+        // nondetection factor at the departure time
+        double* row = c.snap + (size_t(L.node) * G + L.age) * SNAP_STRIDE;
 #pragma unroll
         for (int i = 0; i < NC; ++i) {
-            c.snap[off + i] = y[i];
+            row[i] = y[i];
         }
-        c.fac[size_t(L.node) * G + L.age] = mobility_factor<G>(L, y);
+        row[SNAP_FAC] = mobility_factor<G>(L, y);
+        row[SNAP_ND]  = (L.np.flags & NF_DETECTION) ? detect_nondetection(L.np, I.c_t1) : 1.0;
     }
     else if (I.c_action == 2) {
 #pragma unroll
@@ -663,33 +696,66 @@ __global__ void __launch_bounds__(BLOCK) node_phase(LaunchDev ld, InstDev* __res
 }
 
 // ------------------------------------------------------------------------------- edge phase
-__device__ __forceinline__ double detect_nondetection(const NodeP& np, double t)
+// Lane -> (segment, node, age group) without loading the lane's coefficients (edge kernels need none).
+template <int G>
+__device__ __forceinline__ const SegDev* lane_map(const LaunchDev& ld, int& node, int& age, int& base)
 {
-    // test_commuters (model.h:717-742): nondetection applies in [start, end)
-    return (t >= np.det_start && t < np.det_end) ? np.nondetect : 1.0;
+    constexpr int NPW = WAVE / G;
+    const int lb      = xcd_block();
+    if (lb >= ld.num_blocks) {
+        return nullptr;
+    }
+    const int seg = ld.block_seg[lb];
+    if (seg < 0) {
+        return nullptr;
+    }
+    const SegDev* c = ld.segs + seg;
+    const int wave  = (lb - ld.block_first[seg]) * (BLOCK / WAVE) + int(threadIdx.x) / WAVE;
+    const int lane  = int(threadIdx.x) % WAVE;
+    const int grp   = lane / G;
+    age             = lane - grp * G;
+    node            = wave * NPW + grp;
+    base            = grp * G;
+    if (grp >= NPW || node >= c->num_nodes) {
+        return nullptr;
+    }
+    return c;
 }
 
-// traveler vector of one edge from its source snapshot (metapopulation_mobility_instant.h:608-615)
-__device__ __forceinline__ void traveler_vector(const double* cvec, const EdgeEnt& e, const double* snap_src,
-                                                double fac_src, double nd_src, int G, int age, double (&m)[NC])
+// departure row of lane slot `lane` (16-byte loads)
+__device__ __forceinline__ void load_row(const double* snap, size_t lane, double (&r)[SNAP_STRIDE])
+{
+    const double2* p = reinterpret_cast<const double2*>(snap + lane * SNAP_STRIDE);
+#pragma unroll
+    for (int i = 0; i < SNAP_STRIDE / 2; ++i) {
+        const double2 v = p[i];
+        r[2 * i]        = v.x;
+        r[2 * i + 1]    = v.y;
+    }
+}
+
+// traveler vector of one edge from its source's departure row (metapopulation_mobility_instant.h:608-615:
+// snapshot * coefficients * mobility factors, then test_commuters' nondetection on INS and ISy if with_nd)
+__device__ __forceinline__ void traveler_vector(const double* cvec, const EdgeEnt& e, const double (&row)[SNAP_STRIDE],
+                                                int G, int age, bool with_nd, double (&m)[NC])
 {
     if (e.cidx < 0) {
 #pragma unroll
         for (int i = 0; i < NC; ++i) {
-            m[i] = snap_src[i] * e.cval;
+            m[i] = row[i] * e.cval;
         }
     }
     else {
         const double* cv = cvec + size_t(e.cidx) * G * NC + size_t(age) * NC;
 #pragma unroll
         for (int i = 0; i < NC; ++i) {
-            m[i] = snap_src[i] * cv[i];
+            m[i] = row[i] * cv[i];
         }
     }
-    m[ISY] = m[ISY] * fac_src;
-    if (nd_src != 1.0) {
-        m[ISY] *= nd_src;
-        m[INS] *= nd_src;
+    m[ISY] = m[ISY] * row[SNAP_FAC];
+    if (with_nd && row[SNAP_ND] != 1.0) {
+        m[ISY] *= row[SNAP_ND];
+        m[INS] *= row[SNAP_ND];
     }
 }
 
@@ -747,100 +813,204 @@ __device__ __forceinline__ bool fixup_negative(const double (&y)[NC], double (&m
     return neg;
 }
 
-// Return walk of one node (returns at c_t1 of the travelers that left at c_tdep,
-// metapopulation_mobility_instant.h:568-603), starting from the pre-return state and visiting the node's
-// edges in the reference's global (start, end) order.
-//
-// The fix-up of edge (s, d) depends on d's state at that point of the sequence, which includes d's
-// earlier out-edges (d, x) with their own fix-ups decided at x. Pass p applies the fix-ups of the node's
-// in-edges locally and takes the out-edge vectors from the records of pass p-1 (pass 0: unfixed). The
-// dependencies follow the global edge order (a DAG), so the records reach the reference's sequential
-// values after at most (longest dependency chain + 1) passes; a pass whose records equal the previous
-// ones reproduces the sequential result exactly.
+// Departures at t_dep = c_t1 (metapopulation_mobility_instant.h:605-621): every edge (s, d) adds its
+// traveler vector to d and subtracts it from s, edges in the global (start, end) order. Node v sees its
+// edges as in[s < v], out[all], in[s > v]; the lane keeps its age group's compartments in registers and
+// walks that sequence (a deterministic segmented sum in the reference's order, no atomics).
 template <int G>
-__device__ __forceinline__ void return_walk(const SegDev& c, Lane<G>& L, InstDev& I, int pass)
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) depart_phase(LaunchDev ld,
+                                                                                       InstDev* __restrict__ inst)
 {
+    int node, age, base;
+    const SegDev* cp = lane_map<G>(ld, node, age, base);
+    if (!cp) {
+        return;
+    }
+    const SegDev& c = *cp;
+    InstDev& I      = inst[c.nodep[node].inst];
+    if (!I.c_active || I.c_action != 1) {
+        return;
+    }
     const EdgeStructDev& es = c.estructs[I.estruct];
-    const int v             = L.node - I.node_base;
+    const int v             = node - I.node_base;
     const int in_b = es.in_ptr[v], in_e = es.in_ptr[v + 1];
     const int in_s          = in_b + es.split[v];
     const int out_b = es.out_ptr[v], out_e = es.out_ptr[v + 1];
     const EdgeEnt* ine      = es.in_ent;
     const EdgeEnt* oute     = es.out_ent;
     const double* cvec      = es.cvec;
-    const size_t lane_off   = (size_t(L.node) * G + L.age);
-    const int cur = pass & 1, prv = cur ^ 1;
-    int32_t* fxc            = c.fx[cur] + I.fx_base;
-    const int32_t* fxp      = c.fx[prv] + I.fx_base;
-    double* poolc           = c.fx_pool[cur] + I.fx_pool_base;
-    const double* poolp     = c.fx_pool[prv] + I.fx_pool_base;
+    const size_t lane_off   = size_t(node) * G + age;
+    const size_t src_base   = size_t(I.node_base) * G + age;
+    double* yp              = c.y + lane_off * NC;
+    double y[NC];
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+        y[i] = yp[i];
+    }
+    auto in_range = [&](int k, int end) {
+        for (; k < end; ++k) {
+            const EdgeEnt e = ine[k];
+            double r[SNAP_STRIDE];
+            load_row(c.snap, src_base + size_t(e.nbr) * G, r);
+            double m[NC];
+            traveler_vector(cvec, e, r, G, age, true, m);
+#pragma unroll
+            for (int i = 0; i < NC; ++i) {
+                y[i] += m[i];
+            }
+        }
+    };
+    in_range(in_b, in_s);
+    if (out_b < out_e) {
+        double sv[SNAP_STRIDE];
+        load_row(c.snap, lane_off, sv);
+        const double nd_v = sv[SNAP_ND];
+        for (int k = out_b; k < out_e; ++k) {
+            double m[NC];
+            traveler_vector(cvec, oute[k], sv, G, age, false, m);
+            if (nd_v != 1.0) {
+                // test_commuters on the source: detected commuters stay home as confirmed cases
+                y[INS] -= m[INS] * (1 - nd_v);
+                y[INSC] += m[INS] * (1 - nd_v);
+                y[ISY] -= m[ISY] * (1 - nd_v);
+                y[ISYC] += m[ISY] * (1 - nd_v);
+                m[ISY] *= nd_v;
+                m[INS] *= nd_v;
+            }
+#pragma unroll
+            for (int i = 0; i < NC; ++i) {
+                y[i] -= m[i];
+            }
+        }
+    }
+    in_range(in_s, in_e);
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+        yp[i] = y[i];
+    }
+    if (I.c_out_day >= 0) {
+        double* op = out_ptr(c, I, node, age, G, I.c_out_day);
+#pragma unroll
+        for (int i = 0; i < NC; ++i) {
+            op[i] = y[i];
+        }
+    }
+}
+
+// newest fix-up record version of an in-edge written in [lo, hi] (slot, -1 = unfixed or none)
+__device__ __forceinline__ int32_t record_slot(const FxRec* r, int32_t lo, int32_t hi)
+{
+    const FxRec a = r[0];
+    const FxRec b = r[1];
+    const bool va = a.tag >= lo && a.tag <= hi;
+    const bool vb = b.tag >= lo && b.tag <= hi;
+    if (va && (!vb || a.tag > b.tag)) {
+        return a.slot;
+    }
+    return vb ? b.slot : -1;
+}
+
+// Return walk of one node (returns at c_t1 of the travelers that left at c_tdep,
+// metapopulation_mobility_instant.h:568-603) from the pre-return state, visiting the node's edges in the
+// reference's global (start, end) order.
+//
+// The fix-up of edge (s, d) depends on d's state at that point of the sequence, which includes d's
+// earlier out-edges (d, x) with their own fix-ups decided at x. Pass p applies the fix-ups of the node's
+// in-edges locally and takes the out-edge vectors from the records as of pass p-1 (pass 0: unfixed). The
+// dependencies follow the global edge order (a DAG), so the records reach the reference's sequential
+// values after at most (longest dependency chain + 1) passes; a pass in which no record changes
+// reproduces the sequential result exactly. A node whose out-edge records did not change in pass p-1
+// would repeat its walk of pass p-1 unchanged, so pass p only re-walks the nodes marked dirty then.
+template <int G, bool FIRST>
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) return_phase(LaunchDev ld,
+                                                                                       InstDev* __restrict__ inst)
+{
+    int node, age, base;
+    const SegDev* cp = lane_map<G>(ld, node, age, base);
+    if (!cp) {
+        return;
+    }
+    const SegDev& c = *cp;
+    InstDev& I      = inst[c.nodep[node].inst];
+    if (!I.c_active || I.c_action != 2) {
+        return;
+    }
+    const int pass = FIRST ? 0 : I.fx_pass;
+    if (FIRST ? (I.fx_state != 0) : (I.fx_state != 1 || pass == 0)) {
+        return;
+    }
+    const int32_t tag = I.fx_epoch + pass;
+    // marks of pass p-1 are in plane (p-1)&1; this pass writes plane p&1
+    int32_t* dirty_out = c.fx_dirty + size_t(pass & 1) * c.num_nodes;
+    if (!FIRST && c.fx_dirty[size_t((pass - 1) & 1) * c.num_nodes + node] != tag - 1) {
+        return;
+    }
+    const EdgeStructDev& es = c.estructs[I.estruct];
+    const int v             = node - I.node_base;
+    const int in_b = es.in_ptr[v], in_e = es.in_ptr[v + 1];
+    const int in_s          = in_b + es.split[v];
+    const int out_b = es.out_ptr[v], out_e = es.out_ptr[v + 1];
+    const EdgeEnt* ine      = es.in_ent;
+    const EdgeEnt* oute     = es.out_ent;
+    const double* cvec      = es.cvec;
+    const size_t lane_off   = size_t(node) * G + age;
+    const size_t src_base   = size_t(I.node_base) * G + age;
+    FxRec* recs             = c.fx_rec + 2 * I.fx_base;
+    double* pool            = c.fx_pool + I.fx_pool_base;
+    const int32_t lo        = I.fx_epoch;
+    const double h          = I.c_step;
+    const double* rc_v      = c.rc + lane_off * RC_STRIDE;
 
     double y[NC];
 #pragma unroll
     for (int i = 0; i < NC; ++i) {
         y[i] = c.ypre[lane_off * NC + i];
     }
-    double sv[NC];
-#pragma unroll
-    for (int i = 0; i < NC; ++i) {
-        sv[i] = c.snap[lane_off * NC + i];
-    }
-    const double fac_v = c.fac[lane_off];
-    const double td    = I.c_tdep;
-    const double h     = I.c_step;
-    const double nd_v  = (L.np.flags & NF_DETECTION) ? detect_nondetection(L.np, td) : 1.0;
-    double rcl[RC_STRIDE];
-#pragma unroll
-    for (int k = 0; k < RC_STRIDE; ++k) {
-        rcl[k] = c.rc[lane_off * RC_STRIDE + k];
-    }
     bool changed = false;
     auto in_edge = [&](int k) {
         const EdgeEnt e = ine[k];
-        const int s     = I.node_base + e.nbr;
-        const NodeP& ns = c.nodep[s];
-        const double nd = (ns.flags & NF_DETECTION) ? detect_nondetection(ns, td) : 1.0;
+        double row[SNAP_STRIDE];
+        load_row(c.snap, src_base + size_t(e.nbr) * G, row);
         double m[NC];
-        traveler_vector(cvec, e, c.snap + (size_t(s) * G + L.age) * NC, c.fac[size_t(s) * G + L.age], nd, G, L.age, m);
-        return_euler(rcl, h, m);
-        const bool neg = fixup_negative(y, m);
-        const bool any = group_any<G>(neg, L.base);
-        int slot       = -1;
-        if (any) {
-            if (L.age == 0) {
-                slot = atomicAdd(&I.fx_count[cur], 1);
-            }
-            slot = __shfl(slot, L.base);
-            if (slot < FX_SLOTS) {
+        traveler_vector(cvec, e, row, G, age, true, m);
+        return_euler(rc_v, h, m);
+        const bool any = group_any<G>(fixup_negative(y, m), base);
+        const int32_t ps = FIRST ? -1 : record_slot(recs + 2 * k, lo, tag - 1);
+        bool diff        = (ps >= 0) != any;
+        if (any && !diff) {
 #pragma unroll
-                for (int i = 0; i < NC; ++i) {
-                    poolc[(size_t(slot) * G + L.age) * NC + i] = m[i];
+            for (int i = 0; i < NC; ++i) {
+                diff = diff || (pool[(size_t(ps) * G + age) * NC + i] != m[i]);
+            }
+        }
+        diff = group_any<G>(diff, base);
+        if (diff) {
+            int32_t slot = -1;
+            if (any) {
+                if (age == 0) {
+                    slot = atomicAdd(&I.fx_count, 1);
                 }
-            }
-            else {
-                slot = -1;
-                if (L.age == 0) {
-                    I.fx_state = 2; // record pool full: resolve this phase with the sequential replay
-                }
-            }
-        }
-        if (L.age == 0) {
-            fxc[k] = slot;
-        }
-        if (pass == 0) {
-            changed = changed || any;
-        }
-        else {
-            const int ps = fxp[k];
-            if ((ps >= 0) != any) {
-                changed = true;
-            }
-            else if (any) {
+                slot = __shfl(slot, base);
+                if (slot < FX_SLOTS) {
 #pragma unroll
-                for (int i = 0; i < NC; ++i) {
-                    changed = changed || (poolp[(size_t(ps) * G + L.age) * NC + i] != m[i]);
+                    for (int i = 0; i < NC; ++i) {
+                        pool[(size_t(slot) * G + age) * NC + i] = m[i];
+                    }
+                }
+                else {
+                    slot = -1;
+                    if (age == 0) {
+                        I.fx_state = 2; // record pool full: resolve this phase with the sequential replay
+                    }
                 }
             }
+            if (age == 0) {
+                FxRec* r        = recs + 2 * k;
+                const int older = (r[0].tag <= r[1].tag) ? 0 : 1;
+                r[older]        = FxRec{tag, slot};
+                dirty_out[I.node_base + e.nbr] = tag; // the source re-walks in the next pass
+            }
+            changed = true;
         }
 #pragma unroll
         for (int i = 0; i < NC; ++i) {
@@ -850,30 +1020,33 @@ __device__ __forceinline__ void return_walk(const SegDev& c, Lane<G>& L, InstDev
     for (int k = in_b; k < in_s; ++k) {
         in_edge(k);
     }
-    for (int k = out_b; k < out_e; ++k) {
-        const EdgeEnt e = oute[k];
-        double m[NC];
-        const int ps = (pass > 0) ? fxp[e.peer] : -1;
-        if (ps >= 0) {
+    if (out_b < out_e) {
+        double sv[SNAP_STRIDE];
+        load_row(c.snap, lane_off, sv);
+        for (int k = out_b; k < out_e; ++k) {
+            const EdgeEnt e = oute[k];
+            double m[NC];
+            const int32_t ps = FIRST ? -1 : record_slot(recs + 2 * e.peer, lo, tag - 1);
+            if (ps >= 0) {
+#pragma unroll
+                for (int i = 0; i < NC; ++i) {
+                    m[i] = pool[(size_t(ps) * G + age) * NC + i];
+                }
+            }
+            else {
+                traveler_vector(cvec, e, sv, G, age, true, m);
+                return_euler(c.rc + (src_base + size_t(e.nbr) * G) * RC_STRIDE, h, m);
+            }
 #pragma unroll
             for (int i = 0; i < NC; ++i) {
-                m[i] = poolp[(size_t(ps) * G + L.age) * NC + i];
+                y[i] += m[i];
             }
-        }
-        else {
-            const int d = I.node_base + e.nbr;
-            traveler_vector(cvec, e, sv, fac_v, nd_v, G, L.age, m);
-            return_euler(c.rc + (size_t(d) * G + L.age) * RC_STRIDE, h, m);
-        }
-#pragma unroll
-        for (int i = 0; i < NC; ++i) {
-            y[i] += m[i];
         }
     }
     for (int k = in_s; k < in_e; ++k) {
         in_edge(k);
     }
-    if (changed) {
+    if (changed && age == 0) {
         I.fx_changed = 1; // every writer stores 1
     }
     double* yp = c.y + lane_off * NC;
@@ -882,111 +1055,7 @@ __device__ __forceinline__ void return_walk(const SegDev& c, Lane<G>& L, InstDev
         yp[i] = y[i];
     }
     if (I.c_out_day >= 0) {
-        double* op = out_ptr(c, I, L.node, L.age, G, I.c_out_day);
-#pragma unroll
-        for (int i = 0; i < NC; ++i) {
-            op[i] = y[i];
-        }
-    }
-}
-
-template <int G>
-__global__ void __launch_bounds__(BLOCK) edge_phase(LaunchDev ld, InstDev* __restrict__ inst)
-{
-    Lane<G> L;
-    const SegDev* cp = load_lane<G>(ld, L);
-    if (!cp) {
-        return;
-    }
-    const SegDev& c = *cp;
-    InstDev& I = inst[L.np.inst];
-    if (!I.c_active || I.c_action == 0) {
-        return;
-    }
-    if (I.c_action == 2) {
-        return_walk<G>(c, L, I, 0);
-        return;
-    }
-    const EdgeStructDev& es = c.estructs[I.estruct];
-    const int v             = L.node - I.node_base;
-    const int in_b = es.in_ptr[v], in_e = es.in_ptr[v + 1];
-    const int in_s          = in_b + es.split[v];
-    const int out_b = es.out_ptr[v], out_e = es.out_ptr[v + 1];
-    const EdgeEnt* ine      = es.in_ent;
-    const EdgeEnt* oute     = es.out_ent;
-    const double* cvec      = es.cvec;
-    const size_t lane_off   = (size_t(L.node) * G + L.age);
-    double* yp              = c.y + lane_off * NC;
-    double y[NC];
-#pragma unroll
-    for (int i = 0; i < NC; ++i) {
-        y[i] = yp[i];
-    }
-    double sv[NC];
-#pragma unroll
-    for (int i = 0; i < NC; ++i) {
-        sv[i] = c.snap[lane_off * NC + i];
-    }
-    const double fac_v = c.fac[lane_off];
-
-    // departures at t = c_t1 (:605-621): to.y += m, from.y -= m, edges in (start, end) order
-    const double td   = I.c_t1;
-    const double nd_v = (L.np.flags & NF_DETECTION) ? detect_nondetection(L.np, td) : 1.0;
-    auto in_edge      = [&](int k) {
-        const EdgeEnt e = ine[k];
-        const int s     = I.node_base + e.nbr;
-        const NodeP& ns = c.nodep[s];
-        const double nd = (ns.flags & NF_DETECTION) ? detect_nondetection(ns, td) : 1.0;
-        double m[NC];
-        traveler_vector(cvec, e, c.snap + (size_t(s) * G + L.age) * NC, c.fac[size_t(s) * G + L.age], nd, G, L.age, m);
-#pragma unroll
-        for (int i = 0; i < NC; ++i) {
-            y[i] += m[i];
-        }
-    };
-    for (int k = in_b; k < in_s; ++k) {
-        in_edge(k);
-    }
-    for (int k = out_b; k < out_e; ++k) {
-        double m[NC];
-        traveler_vector(cvec, oute[k], sv, fac_v, 1.0, G, L.age, m);
-        if (nd_v != 1.0) {
-            y[INS] -= m[INS] * (1 - nd_v);
-            y[INSC] += m[INS] * (1 - nd_v);
-            y[ISY] -= m[ISY] * (1 - nd_v);
-            y[ISYC] += m[ISY] * (1 - nd_v);
-            m[ISY] *= nd_v;
-            m[INS] *= nd_v;
-        }
-#pragma unroll
-        for (int i = 0; i < NC; ++i) {
-            y[i] -= m[i];
-        }
-    }
-    for (int k = in_s; k < in_e; ++k) {
-        in_edge(k);
-    }
-    // return coefficients of this node as a destination, from its post-exchange state at t_dep
-    double c_icr, c_d;
-    const double kappa = force_of_infection<G>(c, L, y, td, c_icr, c_d);
-    double* rc         = c.rc + lane_off * RC_STRIDE;
-    rc[RC_KAPPA]       = kappa;
-    rc[RC_E]           = L.c[LC_E];
-    rc[RC_INS_ISY]     = L.c[LC_INS_ISY];
-    rc[RC_INS_R]       = L.c[LC_INS_R];
-    rc[RC_ISY_ISEV]    = L.c[LC_ISY_ISEV];
-    rc[RC_ISY_R]       = L.c[LC_ISY_R];
-    rc[RC_ISEV_ICR]    = c_icr;
-    rc[RC_ISEV_R]      = L.c[LC_ISEV_R];
-    rc[RC_ISEV_D]      = c_d;
-    rc[RC_ICR_D]       = L.c[LC_ICR_D];
-    rc[RC_ICR_R]       = L.c[LC_ICR_R];
-#pragma unroll
-    for (int i = 0; i < NC; ++i) {
-        yp[i] = y[i];
-    }
-    if (I.c_out_day >= 0) {
-        double* op = out_ptr(c, I, L.node, L.age, G, I.c_out_day);
+        double* op = out_ptr(c, I, node, age, G, I.c_out_day);
 #pragma unroll
         for (int i = 0; i < NC; ++i) {
             op[i] = y[i];
@@ -1016,25 +1085,7 @@ __global__ void fixup_check(LaunchDev ld, InstDev* inst)
     I.fx_state   = 1;
     I.fx_pass    = I.fx_pass + 1;
     I.fx_changed = 0;
-    I.fx_count[I.fx_pass & 1] = 0;
     I.fx_passes += 1;
-}
-
-// One more return pass for the instances that are iterating.
-template <int G>
-__global__ void __launch_bounds__(BLOCK) return_pass(LaunchDev ld, InstDev* __restrict__ inst)
-{
-    Lane<G> L;
-    const SegDev* cp = load_lane<G>(ld, L);
-    if (!cp) {
-        return;
-    }
-    const SegDev& c = *cp;
-    InstDev& I = inst[L.np.inst];
-    if (!I.c_active || I.c_action != 2 || I.fx_state != 1) {
-        return;
-    }
-    return_walk<G>(c, L, I, I.fx_pass);
 }
 
 // ------------------------------------------------------------------------------- exact replay
@@ -1065,19 +1116,18 @@ __global__ void __launch_bounds__(WAVE) replay_phase(LaunchDev ld, InstDev* __re
     const int32_t* seq      = es.seq;
     const EdgeEnt* oute     = es.out_ent;
     const double* cvec      = es.cvec;
-    const double td        = I.c_tdep;
-    const double h         = I.c_step;
+    const double h          = I.c_step;
     if (a < G) {
         for (int k = 0; k < es.num_edges; ++k) {
             const int s       = I.node_base + seq[3 * k];
             const int d       = I.node_base + seq[3 * k + 1];
             const EdgeEnt e   = oute[seq[3 * k + 2]];
-            const NodeP& ns   = c.nodep[s];
-            const double nd   = (ns.flags & NF_DETECTION) ? detect_nondetection(ns, td) : 1.0;
             const size_t soff = size_t(s) * G + a;
             const size_t doff = size_t(d) * G + a;
+            double row[SNAP_STRIDE];
+            load_row(c.snap, soff, row);
             double m[NC];
-            traveler_vector(cvec, e, c.snap + soff * NC, c.fac[soff], nd, G, a, m);
+            traveler_vector(cvec, e, row, G, a, true, m);
             return_euler(c.rc + doff * RC_STRIDE, h, m);
             double* yd = c.y + doff * NC;
             double* ys = c.y + soff * NC;
@@ -1137,12 +1187,13 @@ template <int G>
 static hipError_t launch_edge_g(const LaunchDev& c, InstDev* inst, hipStream_t s)
 {
     const int nb = c.num_blocks;
-    hipLaunchKernelGGL((edge_phase<G>), dim3(nb), dim3(BLOCK), 0, s, c, inst);
+    hipLaunchKernelGGL((depart_phase<G>), dim3(nb), dim3(BLOCK), 0, s, c, inst);
+    hipLaunchKernelGGL((return_phase<G, true>), dim3(nb), dim3(BLOCK), 0, s, c, inst);
     hipError_t err = hipGetLastError();
     const int ni   = (c.num_inst + 255) / 256;
     for (int p = 1; p < FX_MAX_PASSES && err == hipSuccess; ++p) {
         hipLaunchKernelGGL(fixup_check, dim3(ni), dim3(256), 0, s, c, inst);
-        hipLaunchKernelGGL((return_pass<G>), dim3(nb), dim3(BLOCK), 0, s, c, inst);
+        hipLaunchKernelGGL((return_phase<G, false>), dim3(nb), dim3(BLOCK), 0, s, c, inst);
         err = hipGetLastError();
     }
     if (err != hipSuccess) {

@@ -39,6 +39,10 @@ class Context:
     def __exit__(self, *a):
         self.close()
 
+    def set_flags(self, flags):
+        """MGO_CTX_* flags for the runs started afterwards."""
+        self.check(self.lib.mgo_context_set_flags(self.ptr, flags))
+
     def clock_ms(self):
         return self.lib.mgo_clock_ms(self.ptr)
 

@@ -1,0 +1,526 @@
+"""Python mirror of `memilio.simulation.osecir` for the Graph-ODE path, backed by the HIP engine.
+
+Reference bindings mirrored (pycode/memilio-simulation/memilio/simulation/bindings/models/osecir.cpp,
+bindings/mobility/graph_simulation.h:35-52, bindings/compartments/parameter_studies.h:92-124):
+InfectionState, Model (parameters by name, populations indexed by (AgeGroup, InfectionState),
+apply_constraints), ModelGraph, MobilityGraph, MobilitySimulation(graph, t0, dt).advance(tmax),
+GraphParameterStudy(graph, t0, tmax, dt, num_runs).run([handle]), simulate(t0, tmax, dt, model),
+interpolate_simulation_result.
+
+Differences a caller sees: results are daily (the values interpolate_simulation_result would give for the
+reference's step-wise series); GraphParameterStudy.run takes an optional `sample(graph, run_idx)` callback
+in place of the reference's draw_sample over parameter distributions, and runs every sampled graph in one
+device batch; a MobilitySimulation advances up to the horizon of its first advance unless `horizon` is
+given. Every solve runs on the device; a missing engine library raises.
+"""
+import copy
+import enum
+
+import numpy as np
+
+from .. import capi
+from ..engine import Context
+from ..graphspec import GraphSpec, NC
+from . import (AgeGroup, ContactMatrix, ContactMatrixGroup, MobilityParameters, TimeSeries,
+               UncertainContactMatrix, UncertainValue, interpolate_simulation_result)
+
+__all__ = ["InfectionState", "Model", "ModelGraph", "MobilityGraph", "MobilitySimulation", "GraphParameterStudy",
+           "Simulation", "simulate", "interpolate_simulation_result", "ControlledStepperWrapper",
+           "RKIntegratorCore", "ExplicitStepperWrapper", "EulerIntegratorCore"]
+
+
+class InfectionState(enum.IntEnum):
+    Susceptible = 0
+    Exposed = 1
+    InfectedNoSymptoms = 2
+    InfectedNoSymptomsConfirmed = 3
+    InfectedSymptoms = 4
+    InfectedSymptomsConfirmed = 5
+    InfectedSevere = 6
+    InfectedCritical = 7
+    Recovered = 8
+    Dead = 9
+    Count = 10
+
+
+_SCALARS = {
+    "StartDay": capi.P_START_DAY,
+    "Seasonality": capi.P_SEASONALITY,
+    "ICUCapacity": capi.P_ICU_CAPACITY,
+    "TestAndTraceCapacity": capi.P_TEST_AND_TRACE_CAPACITY,
+    "TestAndTraceCapacityMaxRisk": capi.P_TEST_AND_TRACE_MAX_RISK,
+}
+_AGES = {
+    "TimeExposed": capi.A_TIME_EXPOSED,
+    "TimeInfectedNoSymptoms": capi.A_TIME_INFECTED_NO_SYMPTOMS,
+    "TimeInfectedSymptoms": capi.A_TIME_INFECTED_SYMPTOMS,
+    "TimeInfectedSevere": capi.A_TIME_INFECTED_SEVERE,
+    "TimeInfectedCritical": capi.A_TIME_INFECTED_CRITICAL,
+    "TransmissionProbabilityOnContact": capi.A_TRANSMISSION_PROBABILITY_ON_CONTACT,
+    "RelativeTransmissionNoSymptoms": capi.A_RELATIVE_TRANSMISSION_NO_SYMPTOMS,
+    "RecoveredPerInfectedNoSymptoms": capi.A_RECOVERED_PER_INFECTED_NO_SYMPTOMS,
+    "RiskOfInfectionFromSymptomatic": capi.A_RISK_OF_INFECTION_FROM_SYMPTOMATIC,
+    "MaxRiskOfInfectionFromSymptomatic": capi.A_MAX_RISK_OF_INFECTION_FROM_SYMPTOMATIC,
+    "SeverePerInfectedSymptoms": capi.A_SEVERE_PER_INFECTED_SYMPTOMS,
+    "CriticalPerSevere": capi.A_CRITICAL_PER_SEVERE,
+    "DeathsPerSevere": capi.A_DEATHS_PER_SEVERE,
+    "DeathsPerCritical": capi.A_DEATHS_PER_CRITICAL,
+}
+
+
+# ----------------------------------------------------------------------------------------- integrators
+class ControlledStepperWrapper:
+    """Adaptive Cash-Karp 5(4) with odeint's controller: the reference's default core."""
+    kind = capi.INTEGRATOR_CASH_KARP54
+
+    def __init__(self, abs_tol=1e-10, rel_tol=1e-5, dt_min=capi.DBL_MIN, dt_max=capi.DBL_MAX):
+        self.abs_tol, self.rel_tol, self.dt_min, self.dt_max = abs_tol, rel_tol, dt_min, dt_max
+
+    def cfg(self):
+        return capi.integrator_cfg(self.kind, self.abs_tol, self.rel_tol, self.dt_min, self.dt_max)
+
+
+class RKIntegratorCore(ControlledStepperWrapper):
+    """RKIntegratorCore with the default Runge-Kutta-Fehlberg 4(5) tableau."""
+    kind = capi.INTEGRATOR_RKF45
+
+
+class ExplicitStepperWrapper:
+    """Fixed-step Cash-Karp (the simulation's dt)."""
+
+    def cfg(self):
+        return capi.integrator_cfg(capi.INTEGRATOR_CASH_KARP54_FIXED, 0.0, 0.0, 0.0, 0.0)
+
+
+class EulerIntegratorCore:
+    def cfg(self):
+        return capi.integrator_cfg(capi.INTEGRATOR_EULER, 0.0, 0.0, 0.0, 0.0)
+
+
+_ctx = None
+
+
+def _context():
+    """Process-wide device context (device 0), created on first use."""
+    global _ctx
+    if _ctx is None:
+        _ctx = Context()
+    return _ctx
+
+
+# ----------------------------------------------------------------------------------------- model
+class _AgeParameter:
+    """Per-age-group values of one parameter: p[AgeGroup(i)] = v, p[AgeGroup(i)].value."""
+
+    def __init__(self, arr):
+        self._a = arr
+
+    def __getitem__(self, g):
+        i = int(g.get() if isinstance(g, AgeGroup) else g)
+        return UncertainValue(lambda: float(self._a[i]), lambda v: self._a.__setitem__(i, v))
+
+    def __setitem__(self, g, v):
+        i = int(g.get() if isinstance(g, AgeGroup) else g)
+        self._a[i] = float(v.value if isinstance(v, UncertainValue) else v)
+
+    def get_num_elements(self):
+        return len(self._a)
+
+    def numel(self):
+        return len(self._a)
+
+
+class Parameters:
+    """osecir::Parameters by name (parameters.h:46-394, 400-450)."""
+
+    def __init__(self, num_groups):
+        object.__setattr__(self, "_G", int(num_groups))
+        from ..graphspec import default_age_params, default_scalars
+        object.__setattr__(self, "_sc", default_scalars())
+        object.__setattr__(self, "_age", default_age_params(self._G))
+        object.__setattr__(self, "ContactPatterns",
+                           UncertainContactMatrix(ContactMatrixGroup(1, self._G)))
+
+    def __getattr__(self, name):
+        if name in _SCALARS:
+            k = _SCALARS[name]
+            return UncertainValue(lambda: float(self._sc[k]), lambda v: self._sc.__setitem__(k, v))
+        if name in _AGES:
+            return _AgeParameter(self._age[_AGES[name]])
+        raise AttributeError(f"osecir parameter {name!r} is not on the device path")
+
+    def __setattr__(self, name, v):
+        if name in _SCALARS:
+            self._sc[_SCALARS[name]] = float(v.value if isinstance(v, UncertainValue) else v)
+        elif name in _AGES:
+            self._age[_AGES[name]][:] = np.asarray(v, dtype=np.float64)
+        elif name == "ContactPatterns":
+            object.__setattr__(self, name, v if isinstance(v, UncertainContactMatrix) else UncertainContactMatrix(v))
+        else:
+            raise AttributeError(f"osecir parameter {name!r} is not on the device path")
+
+    def get_num_groups(self):
+        return self._G
+
+    # commuter detection (parameters.h:417-450)
+    def get_commuter_nondetection(self):
+        return float(self._sc[capi.P_COMMUTER_NONDETECTION])
+
+    def set_commuter_nondetection(self, v):
+        self._sc[capi.P_COMMUTER_NONDETECTION] = float(v)
+
+    def get_start_commuter_detection(self):
+        return float(self._sc[capi.P_START_COMMUTER_DETECTION])
+
+    def set_start_commuter_detection(self, v):
+        self._sc[capi.P_START_COMMUTER_DETECTION] = float(v)
+
+    def get_end_commuter_detection(self):
+        return float(self._sc[capi.P_END_COMMUTER_DETECTION])
+
+    def set_end_commuter_detection(self, v):
+        self._sc[capi.P_END_COMMUTER_DETECTION] = float(v)
+
+    def __deepcopy__(self, memo):
+        p = Parameters(self._G)
+        object.__setattr__(p, "_sc", self._sc.copy())
+        object.__setattr__(p, "_age", self._age.copy())
+        object.__setattr__(p, "ContactPatterns", copy.deepcopy(self.ContactPatterns, memo))
+        return p
+
+    def apply_constraints(self):
+        """Clamp parameters to admissible values as osecir::Parameters::apply_constraints
+        (parameters.h:465-612): seasonality outside [0, 0.5] -> 0, negative capacities -> 0, stay times
+        below 0.1 -> 0.1, RelativeTransmissionNoSymptoms < 0 -> 0, the other probabilities outside [0, 1]
+        -> 0 (MaxRiskOfInfectionFromSymptomatic is not checked). Returns True if anything was corrected."""
+        corrected = False
+        sc, a = self._sc, self._age
+        if not 0.0 <= sc[capi.P_SEASONALITY] <= 0.5:
+            sc[capi.P_SEASONALITY] = 0.0
+            corrected = True
+        for k in (capi.P_ICU_CAPACITY, capi.P_TEST_AND_TRACE_CAPACITY):
+            if sc[k] < 0.0:
+                sc[k] = 0.0
+                corrected = True
+        if sc[capi.P_TEST_AND_TRACE_MAX_RISK] < 0.0:
+            sc[capi.P_TEST_AND_TRACE_MAX_RISK] = 0.0
+            corrected = True
+        for k in (capi.A_TIME_EXPOSED, capi.A_TIME_INFECTED_NO_SYMPTOMS, capi.A_TIME_INFECTED_SYMPTOMS,
+                  capi.A_TIME_INFECTED_SEVERE, capi.A_TIME_INFECTED_CRITICAL):
+            bad = a[k] < 0.1
+            if bad.any():
+                a[k][bad] = 0.1
+                corrected = True
+        bad = a[capi.A_RELATIVE_TRANSMISSION_NO_SYMPTOMS] < 0.0
+        if bad.any():
+            a[capi.A_RELATIVE_TRANSMISSION_NO_SYMPTOMS][bad] = 0.0
+            corrected = True
+        for k in (capi.A_TRANSMISSION_PROBABILITY_ON_CONTACT, capi.A_RECOVERED_PER_INFECTED_NO_SYMPTOMS,
+                  capi.A_RISK_OF_INFECTION_FROM_SYMPTOMATIC, capi.A_SEVERE_PER_INFECTED_SYMPTOMS,
+                  capi.A_CRITICAL_PER_SEVERE, capi.A_DEATHS_PER_SEVERE, capi.A_DEATHS_PER_CRITICAL):
+            bad = (a[k] < 0.0) | (a[k] > 1.0)
+            if bad.any():
+                a[k][bad] = 0.0
+                corrected = True
+        return corrected
+
+
+class Populations:
+    """Compartment values indexed by (AgeGroup, InfectionState); flat index g * 10 + state."""
+
+    def __init__(self, num_groups):
+        self._G = int(num_groups)
+        self._v = np.zeros((self._G, NC))
+
+    @staticmethod
+    def _idx(key):
+        g, s = key
+        return int(g.get() if isinstance(g, AgeGroup) else g), int(s)
+
+    def __getitem__(self, key):
+        g, s = self._idx(key)
+        return UncertainValue(lambda: float(self._v[g, s]), lambda v: self._v.__setitem__((g, s), v))
+
+    def __setitem__(self, key, v):
+        g, s = self._idx(key)
+        self._v[g, s] = float(v.value if isinstance(v, UncertainValue) else v)
+
+    def numel(self):
+        return self._v.size
+
+    def get_num_compartments(self):
+        return self._v.size
+
+    def get_flat_index(self, key):
+        g, s = self._idx(key)
+        return g * NC + s
+
+    def get_total(self):
+        return float(self._v.sum())
+
+    def get_group_total_AgeGroup(self, g):
+        return float(self._v[int(g.get() if isinstance(g, AgeGroup) else g)].sum())
+
+    def set_difference_from_total(self, key, total):
+        self[key] = 0.0
+        self[key] = total - self.get_total()
+
+    def set_difference_from_group_total_AgeGroup(self, key, total):
+        g, _ = self._idx(key)
+        self[key] = 0.0
+        self[key] = total - float(self._v[g].sum())
+
+    def get_compartments(self):
+        return self._v.ravel().copy()
+
+
+class Model:
+    """osecir::Model: parameters and initial populations."""
+
+    def __init__(self, num_agegroups):
+        self.parameters = Parameters(num_agegroups)
+        self.populations = Populations(num_agegroups)
+
+    def apply_constraints(self):
+        return self.parameters.apply_constraints()
+
+    def check_constraints(self):
+        test = copy.deepcopy(self)
+        return test.apply_constraints()
+
+    @property
+    def num_groups(self):
+        return self.parameters.get_num_groups()
+
+
+# ----------------------------------------------------------------------------------------- graph
+class _Node:
+    def __init__(self, id, prop):
+        self.id = id
+        self.property = prop
+
+
+class _Edge:
+    def __init__(self, start, end, prop):
+        self.start_node_idx = start
+        self.end_node_idx = end
+        self.property = prop
+
+
+class _Graph:
+    def __init__(self):
+        self._nodes, self._edges = [], []
+
+    @property
+    def num_nodes(self):
+        return len(self._nodes)
+
+    @property
+    def num_edges(self):
+        return len(self._edges)
+
+    def get_node(self, i):
+        return self._nodes[i]
+
+    def get_edge(self, i):
+        return self._edges[i]
+
+    def get_num_out_edges(self, i):
+        return sum(1 for e in self._edges if e.start_node_idx == i)
+
+    def get_out_edge(self, i, k):
+        return [e for e in self._edges if e.start_node_idx == i][k]
+
+    def add_edge(self, start, end, coeffs):
+        """Edges stay sorted by (start, end); an existing edge is replaced (graph.h:231-241)."""
+        if not (0 <= start < self.num_nodes and 0 <= end < self.num_nodes):
+            raise IndexError("add_edge: node index out of range")
+        prop = coeffs if isinstance(coeffs, MobilityParameters) else MobilityParameters(coeffs)
+        for k, e in enumerate(self._edges):
+            if (e.start_node_idx, e.end_node_idx) == (start, end):
+                self._edges[k] = _Edge(start, end, prop)
+                return
+            if (e.start_node_idx, e.end_node_idx) > (start, end):
+                self._edges.insert(k, _Edge(start, end, prop))
+                return
+        self._edges.append(_Edge(start, end, prop))
+
+
+class ModelGraph(_Graph):
+    """Graph<osecir::Model, MobilityParameters>: the parameter graph of a study."""
+
+    def add_node(self, id, model):
+        self._nodes.append(_Node(id, copy.deepcopy(model)))
+
+
+class SimulationNode:
+    """SimulationNode<osecir::Simulation>: the model, its initial time and step, and its result."""
+
+    def __init__(self, model, t0=0.0, dt=0.1):
+        self.model = copy.deepcopy(model)
+        self.t0 = float(t0)
+        self.dt = float(dt)
+        self.result = TimeSeries(model.populations.numel())
+        self.result.add_time_point(self.t0, model.populations.get_compartments())
+
+
+class MobilityGraph(_Graph):
+    """Graph<SimulationNode<osecir::Simulation>, MobilityEdge>."""
+
+    def add_node(self, id, model, t0=0.0, dt=0.1):
+        self._nodes.append(_Node(id, SimulationNode(model, t0, dt)))
+
+
+def _spec_from(models, edges):
+    """GraphSpec (mgo_graph_desc arrays) of a list of models and (start, end, coeffs) edges."""
+    G = models[0].num_groups
+    if any(m.num_groups != G for m in models):
+        raise ValueError("all nodes of a graph need the same number of age groups")
+    spec = GraphSpec(len(models), G)
+    per_node_damp = []
+    any_damp = False
+    minima = np.zeros((len(models), G, G))
+    for v, m in enumerate(models):
+        p = m.parameters
+        spec.node_scalars[v] = p._sc
+        spec.age_params[v] = p._age
+        spec.populations[v] = m.populations._v
+        grp = p.ContactPatterns.cont_freq_mat
+        damped = [k for k in range(grp.num_matrices) if grp[k].dampings]
+        if len(damped) > 1:
+            raise ValueError("more than one damped contact matrix in a node is not supported on the device")
+        B = sum(grp[k].baseline for k in range(grp.num_matrices))
+        if damped:
+            k0 = damped[0]
+            M = grp[k0].minimum + sum(grp[k].baseline for k in range(grp.num_matrices) if k != k0)
+        else:
+            M = sum(grp[k].minimum for k in range(grp.num_matrices))
+        spec.contact_baseline[v] = B
+        minima[v] = M
+        lst = []
+        if damped:
+            for d in grp[damped[0]].dampings:
+                c = np.broadcast_to(d.coeffs, (G, G)) if d.coeffs.size == 1 else d.coeffs
+                lst.append((d.time, d.level, d.type, c))
+            any_damp = True
+        per_node_damp.append(lst)
+    if minima.any():
+        spec.contact_minimum = minima
+    if any_damp:
+        spec.set_dampings(per_node_damp)
+    if edges:
+        s = np.array([e[0] for e in edges], np.int32)
+        d = np.array([e[1] for e in edges], np.int32)
+        c = np.stack([np.broadcast_to(np.asarray(e[2], np.float64), (NC * G,)) for e in edges])
+        spec.set_edges(s, d, c)
+    return spec
+
+
+def _integrator_of(graph, integrator):
+    if integrator is not None:
+        return integrator.cfg()
+    return ControlledStepperWrapper().cfg()
+
+
+def _fill_results(graph, daily, day0, t_end=None, state=None):
+    for v in range(graph.num_nodes):
+        node = graph.get_node(v).property
+        ts = TimeSeries(node.result.get_num_elements())
+        for k in range(daily.shape[0]):
+            ts.add_time_point(day0 + k, daily[k, v])
+        if state is not None and (daily.shape[0] == 0 or abs(day0 + daily.shape[0] - 1 - t_end) > 1e-10):
+            ts.add_time_point(t_end, state[v])
+        node.result = ts
+
+
+class MobilitySimulation:
+    """make_mobility_sim(t0, dt, graph): GraphSimulation with daily exchange for dt = 0.5."""
+
+    def __init__(self, graph, t0=0.0, dt=1.0, integrator=None, horizon=None):
+        self.graph = graph
+        self.t = float(t0)
+        self.t0 = float(t0)
+        self.dt = float(dt)
+        self._integrator = integrator
+        self._horizon = horizon
+        self._sim = None
+
+    def advance(self, tmax):
+        from ..engine import GraphSim
+        if self._sim is None:
+            nodes = [self.graph.get_node(v).property for v in range(self.graph.num_nodes)]
+            dts = {n.dt for n in nodes}
+            if len(dts) != 1:
+                raise ValueError("nodes of one graph need the same initial step on the device")
+            edges = [(e.start_node_idx, e.end_node_idx, e.property.coefficients) for e in self.graph._edges]
+            self._spec = _spec_from([n.model for n in nodes], edges)
+            cfg = capi.sim_cfg(t0=self.t0, dt_graph=self.dt, dt_node=dts.pop(),
+                               integrator=_integrator_of(self.graph, self._integrator))
+            hz = max(float(tmax), float(self._horizon or 0.0))
+            self._sim = GraphSim(_context(), self._spec, cfg, horizon=hz)
+        self._sim.advance(float(tmax))
+        self.t = self._sim.get_t()
+        _fill_results(self.graph, self._sim.daily(), int(np.ceil(self.t0 - 1e-10)), self.t, self._sim.state())
+
+
+class Simulation:
+    """osecir::Simulation of one model; advance(tmax) runs t0 -> tmax on the device (once)."""
+
+    def __init__(self, model, t0=0.0, dt=0.1, integrator=None):
+        self.model = copy.deepcopy(model)
+        self.t0, self.dt = float(t0), float(dt)
+        self.integrator = integrator
+        self.result = TimeSeries(model.populations.numel())
+        self.result.add_time_point(self.t0, model.populations.get_compartments())
+
+    def advance(self, tmax):
+        spec = _spec_from([self.model], [])
+        cfg = capi.sim_cfg(t0=self.t0, dt_node=self.dt, horizon=float(tmax),
+                           integrator=(self.integrator or ControlledStepperWrapper()).cfg())
+        daily = _context().simulate_single(spec, cfg, float(tmax))[:, 0, :]
+        d0 = int(np.ceil(self.t0 - 1e-10))
+        self.result = TimeSeries(self.model.populations.numel(), [d0 + k for k in range(daily.shape[0])], daily)
+        return self.result
+
+
+def simulate(t0, tmax, dt, model, integrator=None):
+    """osecir::simulate(t0, tmax, dt, model[, core]) -> daily TimeSeries (model.h:385-390)."""
+    return Simulation(model, t0, dt, integrator).advance(tmax)
+
+
+class GraphParameterStudy:
+    """ParameterStudy over a ModelGraph (parameter_studies.h:156-217, bindings :92-124): every run builds
+    the simulation graph with node step = graph step = dt (make_sampled_graph_simulation), after the
+    optional sample(graph, run_idx) callback; all runs execute as one device batch (mgo_ensemble_run)."""
+
+    def __init__(self, graph, t0, tmax, dt, num_runs):
+        self.parameters = graph
+        self.t0, self.tmax, self.dt = float(t0), float(tmax), float(dt)
+        self.num_runs = int(num_runs)
+
+    def run(self, handle_result_func=None, sample=None, integrator=None):
+        graphs, specs, cfgs = [], [], []
+        for run in range(self.num_runs):
+            g = copy.deepcopy(self.parameters)
+            if sample is not None:
+                sample(g, run)
+            sg = MobilityGraph()
+            for v in range(g.num_nodes):
+                nd = g.get_node(v)
+                sg.add_node(nd.id, nd.property, self.t0, self.dt)
+            for e in g._edges:
+                sg.add_edge(e.start_node_idx, e.end_node_idx, e.property)
+            edges = [(e.start_node_idx, e.end_node_idx, e.property.coefficients) for e in sg._edges]
+            specs.append(_spec_from([sg.get_node(v).property.model for v in range(sg.num_nodes)], edges))
+            cfgs.append(capi.sim_cfg(t0=self.t0, dt_graph=self.dt, dt_node=self.dt,
+                                     integrator=(integrator or ControlledStepperWrapper()).cfg()))
+            graphs.append(sg)
+        outs, _ = _context().ensemble_run(specs, cfgs, [self.tmax] * self.num_runs)
+        d0 = int(np.ceil(self.t0 - 1e-10))
+        for sg, daily in zip(graphs, outs):
+            _fill_results(sg, daily, d0)
+        if handle_result_func is None:
+            return graphs
+        for run, sg in enumerate(graphs):
+            handle_result_func(sg, run)
+        return None

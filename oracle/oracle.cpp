@@ -233,7 +233,7 @@ struct Model {
     // FlowModel::get_derivatives(pop, y, t, dydt) (cpp/memilio/compartments/flow_model.h:138-144)
     virtual void get_derivatives(const double* pop, const double* y, double t, double* dydt) const = 0;
     // get_mobility_factors customisation point (metapopulation_mobility_instant.h:507-519): default ones
-    virtual void mobility_factors(const double* y, double* factors) const
+    virtual void mobility_factors(const double* /*y*/, double* factors) const
     {
         std::fill(factors, factors + size(), 1.0);
     }

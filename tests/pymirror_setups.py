@@ -1,0 +1,102 @@
+"""Set-ups written with the Python mirror API the way the reference's own Python examples and tests
+write them (pycode/examples/simulation/ode_secir_mobility.py:37-100,
+pycode/memilio-simulation/tests/test_parameter_study.py:14-52)."""
+import numpy as np
+
+import memilio_amd.simulation as mio
+import memilio_amd.simulation.osecir as osecir
+
+
+def mobility_example_graph(t0=0.0):
+    """Two regions of the reference's ode_secir_mobility.py example."""
+    model = osecir.Model(1)
+    A0 = mio.AgeGroup(0)
+    model.parameters.TimeExposed[A0] = 3.2
+    model.parameters.TimeInfectedNoSymptoms[A0] = 2.
+    model.parameters.TimeInfectedSymptoms[A0] = 6
+    model.parameters.TimeInfectedSevere[A0] = 12
+    model.parameters.TimeInfectedCritical[A0] = 8
+    model.parameters.ContactPatterns.cont_freq_mat[0].baseline = np.r_[0.5]
+    model.parameters.ContactPatterns.cont_freq_mat[0].add_damping(mio.Damping(np.r_[0.3], t=0.3))
+    model.parameters.TransmissionProbabilityOnContact[A0] = 1.0
+    model.parameters.RelativeTransmissionNoSymptoms[A0] = 0.67
+    model.parameters.RecoveredPerInfectedNoSymptoms[A0] = 0.09
+    model.parameters.RiskOfInfectionFromSymptomatic[A0] = 0.25
+    model.parameters.SeverePerInfectedSymptoms[A0] = 0.2
+    model.parameters.CriticalPerSevere[A0] = 0.25
+    model.parameters.DeathsPerCritical[A0] = 0.3
+    graph = osecir.MobilityGraph()
+    S = osecir.InfectionState
+    model.populations[A0, S.Exposed] = 100
+    model.populations[A0, S.InfectedNoSymptoms] = 50
+    model.populations[A0, S.InfectedSymptoms] = 50
+    model.populations[A0, S.InfectedSevere] = 20
+    model.populations[A0, S.InfectedCritical] = 10
+    model.populations[A0, S.Recovered] = 10
+    model.populations[A0, S.Dead] = 0
+    model.populations.set_difference_from_group_total_AgeGroup((A0, S.Susceptible), 10000)
+    model.apply_constraints()
+    graph.add_node(id=0, model=model, t0=t0)
+    for s in S:
+        if s != S.Count:
+            model.populations[A0, s] = 0
+    model.populations.set_difference_from_group_total_AgeGroup((A0, S.Susceptible), 2000)
+    model.apply_constraints()
+    graph.add_node(id=1, model=model, t0=t0)
+    coeffs = 0.1 * np.ones(model.populations.numel())
+    coeffs[S.Dead] = 0
+    params = mio.MobilityParameters(coeffs)
+    graph.add_edge(0, 1, params)
+    graph.add_edge(1, 0, params)
+    return graph
+
+
+def study_model(G=2, seed=0):
+    """A G-group model like the reference's parameter-study test set-up, with a damping at t = 30."""
+    rng = np.random.default_rng(seed)
+    model = osecir.Model(G)
+    S = osecir.InfectionState
+    for g in range(G):
+        A = mio.AgeGroup(g)
+        model.parameters.TimeExposed[A] = 3.2
+        model.parameters.TimeInfectedNoSymptoms[A] = 2.
+        model.parameters.TimeInfectedSymptoms[A] = 6
+        model.parameters.TimeInfectedSevere[A] = 12
+        model.parameters.TimeInfectedCritical[A] = 8
+        model.parameters.TransmissionProbabilityOnContact[A] = 0.3 + 0.2 * rng.random()
+        model.parameters.RecoveredPerInfectedNoSymptoms[A] = 0.09
+        model.parameters.RiskOfInfectionFromSymptomatic[A] = 0.25
+        model.parameters.SeverePerInfectedSymptoms[A] = 0.2
+        model.parameters.CriticalPerSevere[A] = 0.25
+        model.parameters.DeathsPerCritical[A] = 0.3
+        model.populations[A, S.Exposed] = 100
+        model.populations[A, S.InfectedNoSymptoms] = 50
+        model.populations[A, S.InfectedSymptoms] = 50
+        model.populations[A, S.InfectedSevere] = 20
+        model.populations[A, S.InfectedCritical] = 10
+        model.populations[A, S.Recovered] = 10
+        model.populations.set_difference_from_group_total_AgeGroup((A, S.Susceptible), 10000 + 1000 * g)
+    model.parameters.ContactPatterns.cont_freq_mat[0] = mio.ContactMatrix(0.5 + rng.random((G, G)))
+    model.parameters.ContactPatterns.cont_freq_mat.add_damping(mio.Damping(0.7 * np.ones((G, G)), 30.0))
+    model.parameters.Seasonality.value = 0.2
+    model.parameters.TestAndTraceCapacity.value = 35
+    model.apply_constraints()
+    return model
+
+
+def study_graph(n=5, G=2):
+    graph = osecir.ModelGraph()
+    for v in range(n):
+        graph.add_node(v, study_model(G, seed=v))
+    for v in range(n):
+        graph.add_edge(v, (v + 1) % n, 0.01 * np.ones(10 * G))
+        graph.add_edge((v + 1) % n, v, 0.02 * np.ones(10 * G))
+    return graph
+
+
+def scale_contacts(graph, run):
+    """sample(graph, run_idx): each run scales node contacts by its own factor."""
+    rng = np.random.default_rng(100 + run)
+    for v in range(graph.num_nodes):
+        cm = graph.get_node(v).property.parameters.ContactPatterns.cont_freq_mat[0]
+        cm.baseline = cm.baseline * rng.uniform(0.8, 1.2)

@@ -1,0 +1,74 @@
+"""The Python mirror of memilio.simulation.osecir on the device: the reference's Python mobility example,
+a GraphParameterStudy batch and osecir.simulate, each against the oracle on the same descriptor."""
+import numpy as np
+import pytest
+
+import memilio_amd.simulation as mio
+import memilio_amd.simulation.osecir as osecir
+from memilio_amd import capi
+from tests import pymirror_setups as ps
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_err(a, b):
+    assert a.shape == b.shape, (a.shape, b.shape)
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-6)))
+
+
+def _spec(graph):
+    nodes = [graph.get_node(v).property for v in range(graph.num_nodes)]
+    edges = [(e.start_node_idx, e.end_node_idx, e.property.coefficients) for e in graph._edges]
+    return osecir._spec_from([n.model for n in nodes], edges)
+
+
+def _results(graph):
+    """[day][node][10G] of the node results."""
+    return np.stack([np.stack(graph.get_node(v).property.result.values) for v in range(graph.num_nodes)], axis=1)
+
+
+def test_mobility_simulation_example(oracle):
+    graph = ps.mobility_example_graph()
+    sim = osecir.MobilitySimulation(graph, 0.0, dt=0.5)
+    sim.advance(50)
+    assert sim.t == 50
+    r0 = osecir.interpolate_simulation_result(sim.graph.get_node(0).property.result)
+    assert r0.as_ndarray().shape == (11, 51)
+    cfg = capi.sim_cfg(t0=0.0, dt_graph=0.5, dt_node=0.1)
+    ref, _ = oracle.graph_run(_spec(graph), cfg, 50.0)
+    assert rel_err(_results(sim.graph), ref) < 1e-6
+
+
+def test_graph_parameter_study(oracle):
+    graph = ps.study_graph(n=5, G=2)
+    study = osecir.GraphParameterStudy(graph, t0=1, tmax=30, dt=0.5, num_runs=6)
+    seen = []
+
+    def handle(g, run_idx):
+        seen.append(run_idx)
+        assert g.get_node(0).property.result.get_time(0) == 1
+        assert g.get_node(0).property.result.get_last_time() == 30
+        sampled = ps.study_graph(n=5, G=2)
+        ps.scale_contacts(sampled, run_idx)
+        sg = osecir.MobilityGraph()
+        for v in range(sampled.num_nodes):
+            sg.add_node(v, sampled.get_node(v).property, 1.0, 0.5)
+        for e in sampled._edges:
+            sg.add_edge(e.start_node_idx, e.end_node_idx, e.property)
+        cfg = capi.sim_cfg(t0=1.0, dt_graph=0.5, dt_node=0.5)
+        ref, _ = oracle.graph_run(_spec(sg), cfg, 30.0)
+        assert rel_err(_results(g), ref) < 1e-6
+
+    study.run(handle, sample=ps.scale_contacts)
+    assert seen == list(range(6))
+
+
+def test_simulate_single_model(oracle):
+    model = ps.study_model(G=3, seed=4)
+    core = osecir.RKIntegratorCore(1e-8, 1e-8, 1e-6, 1.0)
+    ts = osecir.simulate(0.0, 40.0, 0.3, model, core)
+    assert ts.get_num_time_points() == 41 and ts.get_last_time() == 40
+    spec = osecir._spec_from([model], [])
+    cfg = capi.sim_cfg(t0=0.0, dt_node=0.3, integrator=core.cfg())
+    ref, _ = oracle.simulate_daily(spec, cfg, 40.0)
+    assert rel_err(np.stack(ts.values), ref) < 1e-6

@@ -1,0 +1,74 @@
+"""Host logic of the Python mirror of memilio.simulation.osecir: parameter and population access, graph
+edge order, apply_constraints as the reference, and the descriptor it hands to the engine (checked by
+running the oracle on it against the same set-up built directly)."""
+import numpy as np
+
+import memilio_amd.simulation as mio
+import memilio_amd.simulation.osecir as osecir
+from memilio_amd import capi
+from tests import pymirror_setups as ps
+
+
+def test_parameters_and_populations():
+    m = osecir.Model(2)
+    A1 = mio.AgeGroup(1)
+    m.parameters.TimeExposed[A1] = 3.5
+    m.parameters.StartDay = 60
+    m.parameters.Seasonality.value = 0.2
+    assert m.parameters.TimeExposed[A1].value == 3.5
+    assert m.parameters.TimeExposed[mio.AgeGroup(0)].value == 1.0      # reference default
+    assert m.parameters.StartDay.value == 60 and m.parameters.Seasonality.value == 0.2
+    assert m.parameters.TestAndTraceCapacityMaxRisk.value == 5.0
+    m.populations[A1, osecir.InfectionState.Exposed] = 7
+    m.populations.set_difference_from_total((mio.AgeGroup(0), osecir.InfectionState.Susceptible), 100)
+    assert m.populations[mio.AgeGroup(0), osecir.InfectionState.Susceptible].value == 93
+    assert m.populations.get_flat_index((A1, osecir.InfectionState.Exposed)) == 11
+    assert m.populations.numel() == 20
+
+
+def test_apply_constraints_like_reference():
+    m = osecir.Model(1)
+    A = mio.AgeGroup(0)
+    m.parameters.TimeExposed[A] = 0.01
+    m.parameters.Seasonality.value = 0.7
+    m.parameters.RelativeTransmissionNoSymptoms[A] = 1.5     # only < 0 is corrected
+    m.parameters.MaxRiskOfInfectionFromSymptomatic[A] = 2.0  # not checked
+    m.parameters.DeathsPerCritical[A] = 1.2
+    assert m.apply_constraints()
+    assert m.parameters.TimeExposed[A].value == 0.1
+    assert m.parameters.Seasonality.value == 0.0
+    assert m.parameters.RelativeTransmissionNoSymptoms[A].value == 1.5
+    assert m.parameters.MaxRiskOfInfectionFromSymptomatic[A].value == 2.0
+    assert m.parameters.DeathsPerCritical[A].value == 0.0
+    assert not m.apply_constraints()
+
+
+def test_graph_edges_sorted_and_replaced():
+    g = osecir.ModelGraph()
+    for v in range(3):
+        g.add_node(v, osecir.Model(1))
+    g.add_edge(2, 0, np.ones(10))
+    g.add_edge(0, 1, np.ones(10))
+    g.add_edge(0, 1, 2 * np.ones(10))
+    assert g.num_nodes == 3 and g.num_edges == 2
+    assert g.get_num_out_edges(0) == 1 and g.get_num_out_edges(1) == 0
+    assert (g.get_edge(0).start_node_idx, g.get_edge(0).end_node_idx) == (0, 1)
+    assert g.get_edge(0).property.coefficients[0] == 2.0
+
+
+def test_descriptor_matches_direct_setup(oracle):
+    """The mirror's descriptor of the reference example runs in the oracle like the same set-up written
+    against the C descriptor directly."""
+    graph = ps.mobility_example_graph()
+    nodes = [graph.get_node(v).property for v in range(graph.num_nodes)]
+    edges = [(e.start_node_idx, e.end_node_idx, e.property.coefficients) for e in graph._edges]
+    spec = osecir._spec_from([n.model for n in nodes], edges)
+    assert spec.num_nodes == 2 and spec.num_edges == 2
+    assert spec.populations[0, 0, 0] == 10000 - 240 and spec.populations[1, 0, 0] == 2000
+    assert spec.contact_baseline[0, 0, 0] == 0.5
+    assert spec.damping_times.tolist() == [0.3, 0.3]
+    assert spec.edge_coeff_matrix()[0, 9] == 0.0
+    cfg = capi.sim_cfg(t0=0.0, dt_graph=0.5, dt_node=0.1)
+    ref, _ = oracle.graph_run(spec, cfg, 10.0)
+    assert ref.shape == (11, 2, 10)
+    assert abs(ref[-1].sum() - 12000.0) < 1e-6          # population conserved
