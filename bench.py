@@ -57,16 +57,18 @@ def ensemble_bench(args, ctx, specs, cfgs, tmaxs):
     out = {}
     t = time.perf_counter()
     batch = Batch(ctx, specs, cfgs, tmaxs)
-    log(f"batch resident in HBM in {time.perf_counter() - t:.1f}s")
-    ctx.set_flags(0)
-    batch.run()               # capture + warm-up
-    batch.reset()
-    t0 = time.perf_counter()
-    ginfo = batch.run()
-    gs = time.perf_counter() - t0
-    out["graph_value"] = node_days / gs
-    log(f"hipGraph-replayed run: {gs * 1e3:.1f} ms wall ({ginfo.device_ms:.1f} ms device) -> "
-        f"{out['graph_value']:.3e} node-days/s")
+    log(f"batch resident in HBM in {time.perf_counter() - t:.1f}s (autotune included)")
+    out["graph_value"] = None
+    if not args.no_graph:
+        ctx.set_flags(0)
+        batch.run()               # capture + warm-up
+        batch.reset()
+        t0 = time.perf_counter()
+        ginfo = batch.run()
+        gs = time.perf_counter() - t0
+        out["graph_value"] = node_days / gs
+        log(f"hipGraph-replayed run: {gs * 1e3:.1f} ms wall ({ginfo.device_ms:.1f} ms device) -> "
+            f"{out['graph_value']:.3e} node-days/s")
     ctx.set_flags(capi.CTX_PROFILE_KERNELS)
     for w in range(args.warmup):
         batch.reset()
@@ -80,7 +82,7 @@ def ensemble_bench(args, ctx, specs, cfgs, tmaxs):
         dt = time.perf_counter() - t0
         times.append(dt)
         infos.append(info)
-        log(f"step {k}: {dt * 1e3:.1f} ms wall, {info.device_ms:.1f} ms device, node kernel "
+        log(f"step {k}: tile {info.tile}, {dt * 1e3:.1f} ms wall, {info.device_ms:.1f} ms device, node kernel "
             f"{info.node_kernel_ms:.1f} ms / {info.node_launches} launches, edge kernels "
             f"{info.edge_kernel_ms:.1f} ms, steps {info.steps}, attempts {info.attempts}, fix-up passes "
             f"{info.fixup_passes}, replays {info.exact_replays}")
@@ -89,6 +91,7 @@ def ensemble_bench(args, ctx, specs, cfgs, tmaxs):
     out["value"] = node_days * args.steps / total
     out["ms_per_step"] = total / args.steps * 1e3
     out["infos"] = infos
+    out["tile"] = infos[-1].tile if infos else None
     out["sample"] = batch.read_daily(0, min(args.cpu_samples, S)) if not args.no_cpu else None
     batch.close()
     return out
@@ -106,6 +109,39 @@ def plain_bench(ctx, base, cfg, days):
     value = base.num_nodes * days / plain_s
     log(f"plain single graph: {plain_s * 1e3:.1f} ms -> {value:.3e} node-days/s")
     return value
+
+
+def uneven_bench(args, ctx):
+    """Uneven batch: independent graphs with log-uniform node counts in [1, 2000], G uniform in 1..8,
+    Poisson(min(50, n-1)) out-degree, horizons U[30, 180] days, rel tol in {1e-6, 1e-8, 1e-10}; one batch,
+    one warm-up run, then timed runs (graph-replayed half-day cycles)."""
+    t = time.perf_counter()
+    pool = synthetic.uneven(args.uneven, seed=3)
+    specs = [p[0] for p in pool]
+    cfgs = [synthetic.ck_cfg(p[2]) for p in pool]
+    tmaxs = [p[1] for p in pool]
+    node_days = float(sum(sp.num_nodes * tm for sp, tm in zip(specs, tmaxs)))
+    log(f"uneven batch: {len(specs)} graphs, {sum(sp.num_nodes for sp in specs)} nodes, "
+        f"{sum(sp.num_edges for sp in specs)} edges, {node_days:.3e} node-days, generated in "
+        f"{time.perf_counter() - t:.1f}s")
+    t = time.perf_counter()
+    batch = Batch(ctx, specs, cfgs, tmaxs)
+    log(f"uneven batch resident in HBM in {time.perf_counter() - t:.1f}s")
+    batch.run()
+    times = []
+    for _ in range(args.uneven_steps):
+        batch.reset()
+        t0 = time.perf_counter()
+        info = batch.run()
+        times.append(time.perf_counter() - t0)
+        log(f"uneven step: {times[-1] * 1e3:.1f} ms wall, {info.device_ms:.1f} ms device, "
+            f"{info.half_day_cycles} cycles, fix-up passes {info.fixup_passes}, replays {info.exact_replays}")
+    batch.close()
+    value = node_days * len(times) / sum(times)
+    log(f"uneven batch: {value:.3e} node-days/s")
+    return {"workload": f"{len(specs)} independent graphs, nodes log-uniform in [1, 2000], G in 1..8, "
+                        f"horizons 30..180 days, rel tol 1e-6/1e-8/1e-10", "graphs": len(specs),
+            "node_days": node_days, "node_days_per_s": value, "ms_per_run": sum(times) / len(times) * 1e3}
 
 
 def serving_bench(args, ctx):
@@ -185,6 +221,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-graph", action="store_true", help="skip the hipGraph-replayed comparison run")
+    ap.add_argument("--tile", type=int, default=0, help="runs per node-major tile (0 = autotuned)")
+    ap.add_argument("--uneven", type=int, default=4096, help="graphs in the uneven batch (0 = skip)")
+    ap.add_argument("--uneven-steps", type=int, default=2)
     ap.add_argument("--serve-rates", type=float, nargs="*", default=[50.0, 200.0, 800.0])
     ap.add_argument("--serve-seconds", type=float, default=4.0)
     ap.add_argument("--serve-pool", type=int, default=384)
@@ -205,6 +245,7 @@ def main():
     log(f"inputs generated in {time.perf_counter() - t:.1f}s, {base.num_edges} edges")
 
     ctx = Context()   # one context (one stream) for every measurement of this process
+    ctx.set_tile(args.tile)
     ens = ensemble_bench(args, ctx, specs, cfgs, tmaxs)
     infos = ens["infos"]
     node_ms_sum = sum(i.node_kernel_ms for i in infos)
@@ -218,6 +259,7 @@ def main():
         f"{edge_ms:.3f} ms per cycle")
 
     plain_value = None if args.no_plain else plain_bench(ctx, base, cfg, days)
+    uneven = uneven_bench(args, ctx) if args.uneven > 0 else None
     serving = None if args.no_serve else serving_bench(args, ctx)
     ctx.close()
 
@@ -255,7 +297,7 @@ def main():
         "data": "synthetic (seeded generator, memilio_amd/synthetic.py)",
         "config": {"workload": f"ensemble of {S} samples of a {N}-node G={G} graph, {days:g} days, adaptive "
                                f"Cash-Karp 1e-10, daily exchange", "samples": S, "nodes": N, "groups": G,
-                   "days": days, "edges": int(base.num_edges)},
+                   "days": days, "edges": int(base.num_edges), "tile": ens["tile"]},
         "graph_replay_node_days_per_s": ens["graph_value"],
         "plain_gpu_node_days_per_s": plain_value,
         "batched_vs_plain": value / plain_value if plain_value else None,
@@ -264,6 +306,7 @@ def main():
                      "kernel": f"node_phase<G={G},CashKarp54>", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
                      "avg_launch_ms": node_ms, "launches": launches, "flops_per_launch": flops_per_launch},
+        "uneven_batch": uneven,
         "serving": serving,
         "cpu_baseline": cpu,
     }

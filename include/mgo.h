@@ -166,6 +166,11 @@ mgo_status mgo_context_create(const mgo_device_cfg* cfg, mgo_context** out);
 mgo_status mgo_context_destroy(mgo_context* ctx);
 /* Replaces the MGO_CTX_* flags; they apply to every batch run and graph advance started afterwards. */
 mgo_status mgo_context_set_flags(mgo_context* ctx, int32_t flags);
+/* Layout of ensembles created afterwards: runs that share a graph (same edge arrays and node count) are
+ * interleaved node-major in tiles of `tile` runs, so that a wavefront holds one graph node of several
+ * runs. 0 (the default) = autotuned: the first ensemble of a shape times a short trial of candidate
+ * tiles on a subset of its runs and keeps the fastest for that shape. Results do not depend on the tile. */
+mgo_status mgo_context_set_tile(mgo_context* ctx, int32_t tile);
 /* Last error message of the context (thread-local copy, valid until the next call on that context). */
 const char* mgo_last_error(const mgo_context* ctx);
 /* Version string of the library. */
@@ -222,6 +227,8 @@ typedef struct mgo_run_info {
     double node_kernel_ms;     /* summed duration of those launches */
     int32_t edge_launches;     /* edge-phase kernel launches timed */
     double edge_kernel_ms;
+    int32_t tile;              /* runs per node-major tile used for runs that share a graph */
+    int32_t reserved;
 } mgo_run_info;
 
 mgo_status mgo_ensemble_run(mgo_context* ctx, const mgo_ensemble_desc* desc, double* daily_out,

@@ -141,6 +141,8 @@ class RunInfo(C.Structure):
         ("node_kernel_ms", C.c_double),
         ("edge_launches", C.c_int32),
         ("edge_kernel_ms", C.c_double),
+        ("tile", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 
@@ -186,6 +188,7 @@ SIGNATURES = {
     "mgo_context_create": (C.c_int32, [C.POINTER(DeviceCfg), C.POINTER(C.c_void_p)]),
     "mgo_context_destroy": (C.c_int32, [C.c_void_p]),
     "mgo_context_set_flags": (C.c_int32, [C.c_void_p, C.c_int32]),
+    "mgo_context_set_tile": (C.c_int32, [C.c_void_p, C.c_int32]),
     "mgo_last_error": (C.c_char_p, [C.c_void_p]),
     "mgo_version": (C.c_char_p, []),
     "mgo_graph_create": (C.c_int32, [C.c_void_p, C.POINTER(GraphDesc), C.POINTER(SimCfg), C.POINTER(C.c_int64)]),

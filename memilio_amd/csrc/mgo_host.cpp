@@ -316,6 +316,7 @@ void block_tables(const std::vector<int>& seg_nodes, int G, std::vector<int32_t>
 struct InstHost {
     int cls;
     int node_base;
+    int node_stride = 1; // segment node of local node v = node_base + v * node_stride
     int num_nodes;
     int G;
     double t;        // host mirror of the device clock (same arithmetic)
@@ -512,6 +513,7 @@ struct Batch {
     bool sequential_fixup     = false;
     size_t device_bytes       = 0;
     std::vector<InstDev> inst0; // instance table at creation (tmax set), for reset
+    std::deque<std::vector<double>> staging; // host sources of uploads in flight during build
     // kernel timing of the last run (profile mode)
     double node_ms = 0, edge_ms = 0;
     int node_launches = 0, edge_launches = 0;
@@ -538,9 +540,12 @@ struct Batch {
 
     int inst_offset = 0; // global index of instance 0 (serving: the request's slot in the shared table)
 
-    void build(std::vector<RunSpec>& runs, hipStream_t s)
+    int tile = 1; // runs per node-major tile (see Tiling in mgo_internal.h)
+
+    void build(std::vector<RunSpec>& runs, hipStream_t s, int tile_runs = 1)
     {
         stream = s;
+        tile   = std::max(1, tile_runs);
         for (auto& r : runs) {
             validate(r.g, r.cfg);
             if (!r.single && !(r.cfg.dt_graph > 0)) {
@@ -557,8 +562,6 @@ struct Batch {
             I.cls         = ci;
             I.G           = r.g->num_groups;
             I.num_nodes   = r.g->num_nodes;
-            I.node_base   = C.num_nodes;
-            C.num_nodes += I.num_nodes;
             C.insts.push_back(int(i));
             I.t0       = r.cfg.t0;
             I.t        = r.cfg.t0;
@@ -568,6 +571,9 @@ struct Batch {
             I.day0     = day_ceil(r.cfg.t0);
             I.ndays    = std::max(0, day_floor(r.tmax) - I.day0 + 1);
         }
+        for (auto& C : classes) {
+            place_nodes(C, runs);
+        }
         std::vector<InstDev> idev(runs.size());
         for (size_t ci = 0; ci < classes.size(); ++ci) {
             build_class(ci, runs, idev);
@@ -575,6 +581,38 @@ struct Batch {
         inst_dev.upload(idev, stream);
         device_bytes += inst_dev.bytes;
         hip_check(hipStreamSynchronize(stream), "batch upload");
+        staging.clear();
+    }
+
+    // Segment node indices of a class: runs that share a graph (same edge arrays and node count) are
+    // interleaved node-major in tiles of up to `tile` runs; the others get contiguous ranges.
+    void place_nodes(ClassHost& C, const std::vector<RunSpec>& runs)
+    {
+        std::map<std::tuple<const void*, const void*, const void*, int, int, bool>, std::vector<int>> groups;
+        std::vector<std::vector<int>*> order;
+        for (int ii : C.insts) {
+            const auto* g = runs[size_t(ii)].g;
+            auto key = std::make_tuple((const void*)g->edge_start, (const void*)g->edge_end, (const void*)g->edge_coeffs,
+                                       g->num_edges, g->num_nodes, runs[size_t(ii)].single);
+            auto it = groups.find(key);
+            if (it == groups.end()) {
+                it = groups.emplace(key, std::vector<int>{}).first;
+                order.push_back(&it->second);
+            }
+            it->second.push_back(ii);
+        }
+        for (auto* grp : order) {
+            for (size_t first = 0; first < grp->size(); first += size_t(tile)) {
+                const int k = int(std::min(grp->size() - first, size_t(tile)));
+                const int n = inst[size_t((*grp)[first])].num_nodes;
+                for (int j = 0; j < k; ++j) {
+                    InstHost& I   = inst[size_t((*grp)[first + size_t(j)])];
+                    I.node_base   = C.num_nodes + j;
+                    I.node_stride = k;
+                }
+                C.num_nodes += n * k;
+            }
+        }
     }
 
     void build_class(size_t ci, std::vector<RunSpec>& runs, std::vector<InstDev>& idev)
@@ -604,7 +642,7 @@ struct Batch {
             const auto* g = r.g;
             inst_ids.push_back(ii);
             for (int v = 0; v < g->num_nodes; ++v) {
-                const size_t node = size_t(I.node_base + v);
+                const size_t node = size_t(I.node_base) + size_t(v) * size_t(I.node_stride);
                 const double* sc  = g->node_scalars + size_t(v) * MGO_NUM_SCALARS;
                 NodeP& np         = nodep[node];
                 np.start_day      = sc[MGO_P_START_DAY];
@@ -616,6 +654,7 @@ struct Batch {
                 np.det_start      = sc[MGO_P_START_COMMUTER_DETECTION];
                 np.det_end        = sc[MGO_P_END_COMMUTER_DETECTION];
                 np.inst           = inst_offset + ii;
+                np.local          = v;
                 np.flags          = 0;
                 if (np.ttc < DBL_MAX) {
                     np.flags |= NF_TTC_ACTIVE;
@@ -721,6 +760,7 @@ struct Batch {
             D.dt_min    = r.cfg.integrator.dt_min;
             D.dt_max    = r.cfg.integrator.dt_max;
             D.node_base = I.node_base;
+            D.node_stride = I.node_stride;
             D.num_nodes = I.num_nodes;
             D.day0      = I.day0;
             D.ndays     = I.ndays;
@@ -799,9 +839,25 @@ struct Batch {
             InstHost& I = inst[size_t(ii)];
             if (I.ndays > 0 && I.t0 >= double(I.day0) - 1e-10 && I.t0 <= double(I.day0)) {
                 const size_t nd = size_t(I.num_nodes) * G * NC;
-                hip_check(hipMemcpyAsync(C.out.as<double>() + I.out_base, C.y.as<double>() + size_t(I.node_base) * G * NC,
-                                         nd * sizeof(double), hipMemcpyDeviceToDevice, stream),
-                          "day0 copy");
+                if (I.node_stride == 1) {
+                    hip_check(hipMemcpyAsync(C.out.as<double>() + I.out_base,
+                                             C.y.as<double>() + size_t(I.node_base) * G * NC, nd * sizeof(double),
+                                             hipMemcpyDeviceToDevice, stream),
+                              "day0 copy");
+                }
+                else {
+                    staging.emplace_back();
+                    std::vector<double>& d0 = staging.back();
+                    d0.resize(nd);
+                    for (int v = 0; v < I.num_nodes; ++v) {
+                        const size_t node = size_t(I.node_base) + size_t(v) * size_t(I.node_stride);
+                        std::copy(y.begin() + long(node * G * NC), y.begin() + long((node + 1) * G * NC),
+                                  d0.begin() + long(size_t(v) * G * NC));
+                    }
+                    hip_check(hipMemcpyAsync(C.out.as<double>() + I.out_base, d0.data(), nd * sizeof(double),
+                                             hipMemcpyHostToDevice, stream),
+                              "day0 copy");
+                }
             }
         }
         SegDev d{};
@@ -1062,14 +1118,36 @@ struct Batch {
         hip_check(hipStreamSynchronize(stream), "read daily");
     }
 
+    // per-node values of instance i from a segment array with `width` values per node
+    template <class T>
+    void gather_nodes(size_t i, const T* src, size_t width, T* out)
+    {
+        const InstHost& I = inst[i];
+        const size_t n    = size_t(I.num_nodes);
+        if (I.node_stride == 1) {
+            hip_check(hipMemcpyAsync(out, src + size_t(I.node_base) * width, n * width * sizeof(T),
+                                     hipMemcpyDeviceToHost, stream),
+                      "read nodes");
+            hip_check(hipStreamSynchronize(stream), "read nodes");
+            return;
+        }
+        const size_t span = (n - 1) * size_t(I.node_stride) + 1;
+        std::vector<T> tmp(span * width);
+        hip_check(hipMemcpyAsync(tmp.data(), src + size_t(I.node_base) * width, tmp.size() * sizeof(T),
+                                 hipMemcpyDeviceToHost, stream),
+                  "read nodes");
+        hip_check(hipStreamSynchronize(stream), "read nodes");
+        for (size_t v = 0; v < n; ++v) {
+            std::copy(tmp.begin() + long(v * size_t(I.node_stride) * width),
+                      tmp.begin() + long((v * size_t(I.node_stride) + 1) * width), out + v * width);
+        }
+    }
+
     void read_state(size_t i, double* out)
     {
         const InstHost& I  = inst[i];
         const ClassHost& C = classes[size_t(I.cls)];
-        hip_check(hipMemcpyAsync(out, C.y.as<double>() + size_t(I.node_base) * I.G * NC,
-                                 size_t(I.num_nodes) * I.G * NC * sizeof(double), hipMemcpyDeviceToHost, stream),
-                  "read state");
-        hip_check(hipStreamSynchronize(stream), "read state");
+        gather_nodes(i, C.y.as<double>(), size_t(I.G) * NC, out);
     }
 
     void read_stats(size_t i, int64_t* steps, int64_t* attempts)
@@ -1077,13 +1155,8 @@ struct Batch {
         const InstHost& I  = inst[i];
         const ClassHost& C = classes[size_t(I.cls)];
         std::vector<uint32_t> s(size_t(I.num_nodes)), a(size_t(I.num_nodes));
-        hip_check(hipMemcpyAsync(s.data(), C.steps.as<uint32_t>() + I.node_base, s.size() * 4, hipMemcpyDeviceToHost,
-                                 stream),
-                  "read stats");
-        hip_check(hipMemcpyAsync(a.data(), C.attempts.as<uint32_t>() + I.node_base, a.size() * 4,
-                                 hipMemcpyDeviceToHost, stream),
-                  "read stats");
-        hip_check(hipStreamSynchronize(stream), "read stats");
+        gather_nodes(i, C.steps.as<uint32_t>(), 1, s.data());
+        gather_nodes(i, C.attempts.as<uint32_t>(), 1, a.data());
         for (size_t k = 0; k < s.size(); ++k) {
             if (steps) {
                 steps[k] = s[k];
@@ -1129,6 +1202,8 @@ struct mgo_context {
     std::unordered_map<int64_t, std::unique_ptr<Batch>> batches;
     std::chrono::steady_clock::time_point t_origin;
     std::unique_ptr<ServeState> serve;
+    int tile = 0;                                    // 0 = autotune
+    std::map<std::tuple<int, int, int, int, int>, int> tile_cache; // shape -> autotuned tile
 };
 
 namespace
@@ -1218,6 +1293,16 @@ mgo_status mgo_context_set_flags(mgo_context* ctx, int32_t flags)
             fail(MGO_ERR_INVALID_VALUE, "unknown context flags 0x%x", unsigned(flags));
         }
         ctx->flags = flags;
+    });
+}
+
+mgo_status mgo_context_set_tile(mgo_context* ctx, int32_t tile)
+{
+    return guarded(ctx, [&] {
+        if (tile < 0) {
+            fail(MGO_ERR_INVALID_VALUE, "tile must be >= 0");
+        }
+        ctx->tile = tile;
     });
 }
 
@@ -1377,6 +1462,87 @@ mgo_status mgo_simulate_single(mgo_context* ctx, const mgo_graph_desc* models, c
     });
 }
 
+// tmax of every run on the host mirror and in the device table; the table becomes the reset state
+static void set_run_tmax(Batch& b, const std::vector<double>& tmax, hipStream_t stream)
+{
+    std::vector<InstDev> v = b.read_inst();
+    for (size_t r = 0; r < tmax.size(); ++r) {
+        b.inst[r].tmax = tmax[r];
+        v[r].tmax      = tmax[r];
+    }
+    hip_check(hipMemcpyAsync(b.inst_dev.p, v.data(), v.size() * sizeof(InstDev), hipMemcpyHostToDevice, stream),
+              "tmax upload");
+    hip_check(hipStreamSynchronize(stream), "tmax upload");
+    b.inst0 = v;
+}
+
+// Tile autotuner. Takes the largest group of runs that share a graph; if it has at least four runs, each
+// candidate tile is timed on a trial batch of up to 256 of those runs over two simulated days (direct
+// launches, one warm-up run, then the timed run) and the fastest is kept for this shape (number of age
+// groups, integrator, nodes, edges, group size bucket) for the life of the context.
+static int autotune_tile(mgo_context* ctx, const std::vector<RunSpec>& runs)
+{
+    if (ctx->tile > 0) {
+        return ctx->tile;
+    }
+    std::map<std::tuple<const void*, const void*, const void*, int, int, int, int>, std::vector<size_t>> groups;
+    for (size_t r = 0; r < runs.size(); ++r) {
+        const auto* g = runs[r].g;
+        groups[std::make_tuple((const void*)g->edge_start, (const void*)g->edge_end, (const void*)g->edge_coeffs,
+                               g->num_edges, g->num_nodes, g->num_groups, runs[r].cfg.integrator.kind)]
+            .push_back(r);
+    }
+    const std::vector<size_t>* best = nullptr;
+    for (auto& kv : groups) {
+        if (!best || kv.second.size() > best->size()) {
+            best = &kv.second;
+        }
+    }
+    if (!best) {
+        return 1;
+    }
+    const RunSpec& r0 = runs[(*best)[0]];
+    const int bucket  = best->size() >= 256 ? 256 : int(best->size());
+    if (bucket < 4) {
+        return 1;
+    }
+    const auto key    = std::make_tuple(r0.g->num_groups, r0.cfg.integrator.kind, r0.g->num_nodes, r0.g->num_edges, bucket);
+    auto hit          = ctx->tile_cache.find(key);
+    if (hit != ctx->tile_cache.end()) {
+        return hit->second;
+    }
+    std::vector<RunSpec> trial;
+    std::vector<double> tmax;
+    for (int k = 0; k < bucket; ++k) {
+        RunSpec t = runs[(*best)[size_t(k)]];
+        t.tmax    = t.cfg.t0 + 4.0 * (t.cfg.dt_graph > 0 ? t.cfg.dt_graph : 0.5);
+        trial.push_back(t);
+        tmax.push_back(t.tmax);
+    }
+    int best_tile    = 1;
+    double best_ms   = INFINITY;
+    const int cands[] = {1, 4, 16, 64};
+    for (int c : cands) {
+        if (c > bucket) {
+            break;
+        }
+        Batch tb;
+        tb.use_graph = false;
+        tb.build(trial, ctx->stream, c);
+        set_run_tmax(tb, tmax, ctx->stream);
+        double ms = 0;
+        tb.run(&ms); // warm-up
+        tb.reset();
+        tb.run(&ms);
+        if (ms < best_ms) {
+            best_ms   = ms;
+            best_tile = c;
+        }
+    }
+    ctx->tile_cache[key] = best_tile;
+    return best_tile;
+}
+
 static void build_ensemble(mgo_context* ctx, const mgo_ensemble_desc* desc, Batch& b)
 {
     if (!desc || desc->num_runs < 1 || !desc->graphs || !desc->cfgs || !desc->tmax) {
@@ -1389,20 +1555,10 @@ static void build_ensemble(mgo_context* ctx, const mgo_ensemble_desc* desc, Batc
         }
         runs[size_t(r)] = RunSpec{&desc->graphs[r], desc->cfgs[r], desc->tmax[r], false, {}};
     }
+    const int tile = autotune_tile(ctx, runs);
     apply_flags(b, ctx->flags);
-    b.build(runs, ctx->stream);
-    for (int r = 0; r < desc->num_runs; ++r) {
-        b.inst[size_t(r)].tmax = desc->tmax[r];
-    }
-    // write all tmax values at once
-    std::vector<InstDev> v = b.read_inst();
-    for (int r = 0; r < desc->num_runs; ++r) {
-        v[size_t(r)].tmax = desc->tmax[r];
-    }
-    hip_check(hipMemcpyAsync(b.inst_dev.p, v.data(), v.size() * sizeof(InstDev), hipMemcpyHostToDevice, ctx->stream),
-              "tmax upload");
-    hip_check(hipStreamSynchronize(ctx->stream), "tmax upload");
-    b.inst0 = v;
+    b.build(runs, ctx->stream, tile);
+    set_run_tmax(b, std::vector<double>(desc->tmax, desc->tmax + desc->num_runs), ctx->stream);
 }
 
 static void fill_info(Batch& b, mgo_run_info* info, double ms, int64_t cycles)
@@ -1417,6 +1573,7 @@ static void fill_info(Batch& b, mgo_run_info* info, double ms, int64_t cycles)
         nd += double(I.num_nodes) * (I.tmax - I.t0);
     }
     info->node_days = nd;
+    info->tile      = b.tile;
     b.totals(info->steps, info->attempts);
     auto v = b.read_inst();
     for (auto& I : v) {

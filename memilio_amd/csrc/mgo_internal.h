@@ -17,6 +17,13 @@
 //   tnode, dtnode       [node]                 integrator time and carried step size
 // Instances (one graph simulation each) live in a device table; a clock kernel turns the instance
 // clock into a per-cycle record read by the node and edge kernels.
+//
+// Tiling. Runs of one class that share a graph (same edge arrays and node count, e.g. the samples of a
+// parameter study) can be interleaved node-major in tiles of T runs: local node v of the j-th run of a
+// tile sits at segment node tile_base + v * T + j (node_base = tile_base + j, node_stride = T). A
+// wavefront then holds the same graph node of 64/G consecutive runs: similar parameters (similar step
+// counts under adaptive control) and identical edge lists (uniform edge walks, adjacent gathers).
+// T = 1 is the run-major layout. Results do not depend on T.
 #pragma once
 
 #include <cstdint>
@@ -100,7 +107,7 @@ struct NodeP {
     int32_t flags;
     int32_t inst;            // instance index
     int32_t damp_cnt;        // number of damping-cache entries
-    int32_t pad;
+    int32_t local;           // node index within its instance (graph node id order)
 };
 
 // per-instance state; the c_* fields are the record of the current half-day cycle
@@ -137,6 +144,8 @@ struct InstDev {
     int32_t fx_sequential; // resolve fix-ups with the sequential replay only
     int32_t seg;          // segment (index into LaunchDev::segs of its class) holding the nodes
     int32_t fx_epoch;     // record tag of pass 0 of the current return phase (grows by FX_EPOCH_STEP)
+    int32_t node_stride;  // segment index of local node v = node_base + v * node_stride (see Tiling)
+    int32_t pad3;
 };
 
 // Fix-up records. Each in-edge keeps two tagged versions {tag, slot}; tag = fx_epoch + pass of the pass

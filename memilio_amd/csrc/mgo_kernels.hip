@@ -84,15 +84,12 @@ __device__ __forceinline__ double group_min(double v, int base)
     return s;
 }
 
+// predicate over the node's G lanes from one wavefront ballot (lanes of a node are active together)
 template <int G>
 __device__ __forceinline__ bool group_all(bool v, int base)
 {
-    int s = __shfl(int(v), base);
-#pragma unroll
-    for (int j = 1; j < G; ++j) {
-        s &= __shfl(int(v), base + j);
-    }
-    return s != 0;
+    const unsigned long long m = (~__ballot(!v) >> base) & ((1ull << G) - 1);
+    return m == ((1ull << G) - 1);
 }
 
 // Per-lane context of one node lane, loaded once per kernel.
@@ -138,15 +135,22 @@ __device__ __forceinline__ void contact_row(const SegDev& c, Lane<G>& L, double 
     }
 }
 
-// Force-of-infection coefficient of the lane's age group: season(t) * tp * sum_j C_gj(t) w_j(pop),
+// Seasonal factor 1 + k sin(pi ((StartDay + t) / 182.5 + 0.5)) of the lane's node at time t
+// (model.h:115-117).
+template <int G>
+__device__ __forceinline__ double season_at(const Lane<G>& L, double t)
+{
+    return 1.0 + L.np.seasonality * sin(MGO_PI * ((L.np.start_day + t) / 182.5 + 0.5));
+}
+
+// Force-of-infection coefficient of the lane's age group: season * tp * sum_j C_gj(t) w_j(pop),
 // w_j = divN_j * (rel_j pop[INS_j] + risk_j pop[ISy_j]) (model.h:99-145, factored over j). Also returns
 // the ICU-adjusted ISev->ICr / ISev->D coefficients (model.h:147-152, 196-204).
 template <int G>
 __device__ __forceinline__ double force_of_infection(const SegDev& c, Lane<G>& L, const double (&pop)[NC], double t,
-                                                     double& c_icr, double& c_d)
+                                                     double season, double& c_icr, double& c_d)
 {
-    const double season = 1.0 + L.np.seasonality * sin(MGO_PI * ((L.np.start_day + t) / 182.5 + 0.5));
-    double risk         = L.c[LC_RISK];
+    double risk = L.c[LC_RISK];
     if (L.np.flags & NF_TTC_ACTIVE) {
         const double ttr = group_sum<G>(L.c[LC_TTR] * pop[INS], L.base);
         risk             = smoother_cosine(ttr, L.np.ttc, L.np.ttc_hi, L.c[LC_RISK], L.c[LC_MAXRISK]);
@@ -208,10 +212,11 @@ __device__ __forceinline__ void flows_to_dydt(const double (&y)[NC], double kapp
 
 // Right-hand side of the node ODE, pop == y (osecir::Model::get_flows + FlowModel::get_derivatives)
 template <int G>
-__device__ __forceinline__ void rhs(const SegDev& c, Lane<G>& L, const double (&y)[NC], double t, double (&dy)[NC])
+__device__ __forceinline__ void rhs(const SegDev& c, Lane<G>& L, const double (&y)[NC], double t, double season,
+                                    double (&dy)[NC])
 {
     double c_icr, c_d;
-    const double kappa = force_of_infection<G>(c, L, y, t, c_icr, c_d);
+    const double kappa = force_of_infection<G>(c, L, y, t, season, c_icr, c_d);
     flows_to_dydt(y, kappa, L.c[LC_E], L.c[LC_INS_ISY], L.c[LC_INS_R], L.c[LC_ISY_ISEV], L.c[LC_ISY_R], c_icr,
                   L.c[LC_ISEV_R], c_d, L.c[LC_ICR_D], L.c[LC_ICR_R], dy);
 }
@@ -243,29 +248,29 @@ __device__ __forceinline__ void cash_karp_stages(const SegDev& c, Lane<G>& L, co
     for (int i = 0; i < NC; ++i) {
         yt[i] = y[i] + (ck::A21 * h) * k1[i];
     }
-    rhs<G>(c, L, yt, t + ck::C2 * h, k2);
+    rhs<G>(c, L, yt, t + ck::C2 * h, season_at<G>(L, t + ck::C2 * h), k2);
 #pragma unroll
     for (int i = 0; i < NC; ++i) {
         yt[i] = y[i] + (ck::A31 * h) * k1[i] + (ck::A32 * h) * k2[i];
     }
-    rhs<G>(c, L, yt, t + ck::C3 * h, k3);
+    rhs<G>(c, L, yt, t + ck::C3 * h, season_at<G>(L, t + ck::C3 * h), k3);
 #pragma unroll
     for (int i = 0; i < NC; ++i) {
         yt[i] = y[i] + (ck::A41 * h) * k1[i] + (ck::A42 * h) * k2[i] + (ck::A43 * h) * k3[i];
     }
-    rhs<G>(c, L, yt, t + ck::C4 * h, k4);
+    rhs<G>(c, L, yt, t + ck::C4 * h, season_at<G>(L, t + ck::C4 * h), k4);
 #pragma unroll
     for (int i = 0; i < NC; ++i) {
         yt[i] = y[i] + (ck::A51 * h) * k1[i] + (ck::A52 * h) * k2[i] + (ck::A53 * h) * k3[i] + (ck::A54 * h) * k4[i];
     }
-    rhs<G>(c, L, yt, t + ck::C5 * h, k5);
+    rhs<G>(c, L, yt, t + ck::C5 * h, season_at<G>(L, t + ck::C5 * h), k5);
 #pragma unroll
     for (int i = 0; i < NC; ++i) {
         yt[i] = y[i] + (ck::A61 * h) * k1[i] + (ck::A62 * h) * k2[i] + (ck::A63 * h) * k3[i] + (ck::A64 * h) * k4[i] +
                 (ck::A65 * h) * k5[i];
     }
     double k6[NC];
-    rhs<G>(c, L, yt, t + ck::C6 * h, k6);
+    rhs<G>(c, L, yt, t + ck::C6 * h, season_at<G>(L, t + ck::C6 * h), k6);
 #pragma unroll
     for (int i = 0; i < NC; ++i) {
         yo[i] = y[i] + (ck::B1 * h) * k1[i] + (ck::B3 * h) * k3[i] + (ck::B4 * h) * k4[i] + (ck::B6 * h) * k6[i];
@@ -304,28 +309,28 @@ __device__ __forceinline__ void rkf45_stages(const SegDev& c, Lane<G>& L, const 
     for (int i = 0; i < NC; ++i) {
         yt[i] = y[i] + (h * rkf::A21) * k1[i];
     }
-    rhs<G>(c, L, yt, t + rkf::C2 * h, k2);
+    rhs<G>(c, L, yt, t + rkf::C2 * h, season_at<G>(L, t + rkf::C2 * h), k2);
 #pragma unroll
     for (int i = 0; i < NC; ++i) {
         yt[i] = y[i] + (h * rkf::A31) * k1[i] + (h * rkf::A32) * k2[i];
     }
-    rhs<G>(c, L, yt, t + rkf::C3 * h, k3);
+    rhs<G>(c, L, yt, t + rkf::C3 * h, season_at<G>(L, t + rkf::C3 * h), k3);
 #pragma unroll
     for (int i = 0; i < NC; ++i) {
         yt[i] = y[i] + (h * rkf::A41) * k1[i] + (h * rkf::A42) * k2[i] + (h * rkf::A43) * k3[i];
     }
-    rhs<G>(c, L, yt, t + rkf::C4 * h, k4);
+    rhs<G>(c, L, yt, t + rkf::C4 * h, season_at<G>(L, t + rkf::C4 * h), k4);
 #pragma unroll
     for (int i = 0; i < NC; ++i) {
         yt[i] = y[i] + (h * rkf::A51) * k1[i] + (h * rkf::A52) * k2[i] + (h * rkf::A53) * k3[i] + (h * rkf::A54) * k4[i];
     }
-    rhs<G>(c, L, yt, t + rkf::C5 * h, k5);
+    rhs<G>(c, L, yt, t + rkf::C5 * h, season_at<G>(L, t + rkf::C5 * h), k5);
 #pragma unroll
     for (int i = 0; i < NC; ++i) {
         yt[i] = y[i] + (h * rkf::A61) * k1[i] + (h * rkf::A62) * k2[i] + (h * rkf::A63) * k3[i] +
                 (h * rkf::A64) * k4[i] + (h * rkf::A65) * k5[i];
     }
-    rhs<G>(c, L, yt, t + rkf::C6 * h, k6);
+    rhs<G>(c, L, yt, t + rkf::C6 * h, season_at<G>(L, t + rkf::C6 * h), k6);
 #pragma unroll
     for (int i = 0; i < NC; ++i) {
         // y + dt * (K * b_low) and dt * |K * (b_high - b_low)| (adapt_rk.h:268-271)
@@ -457,9 +462,16 @@ __global__ void clock_kernel(InstDev* inst, int n)
     }
 }
 
-__device__ __forceinline__ double* out_ptr(const SegDev& c, const InstDev& I, int node, int age, int G, int day)
+// daily output of local node v: [day][v][g][10]
+__device__ __forceinline__ double* out_ptr(const InstDev& I, int v, int age, int G, int day)
 {
-    return I.out + (size_t(day) * I.num_nodes + (node - I.node_base)) * size_t(G * NC) + size_t(age) * NC;
+    return I.out + (size_t(day) * I.num_nodes + v) * size_t(G * NC) + size_t(age) * NC;
+}
+
+// segment node of an instance's local node v
+__device__ __forceinline__ size_t seg_node(const InstDev& I, int v)
+{
+    return size_t(I.node_base) + size_t(v) * size_t(I.node_stride);
 }
 
 // ------------------------------------------------------------------------------- node phase
@@ -507,7 +519,7 @@ __global__ void __launch_bounds__(BLOCK) node_phase(LaunchDev ld, InstDev* __res
         // return coefficients of this node as a destination: its state after the departures, at the
         // departure time (the `pop` of calculate_mobility_returns, metapopulation_mobility_instant.h:460-471)
         double c_icr, c_d;
-        const double kappa = force_of_infection<G>(c, L, y, I.c_tdep, c_icr, c_d);
+        const double kappa = force_of_infection<G>(c, L, y, I.c_tdep, season_at<G>(L, I.c_tdep), c_icr, c_d);
         double* rc         = c.rc + (size_t(L.node) * G + L.age) * RC_STRIDE;
         rc[RC_KAPPA]       = kappa;
         rc[RC_E]           = L.c[LC_E];
@@ -541,7 +553,7 @@ __global__ void __launch_bounds__(BLOCK) node_phase(LaunchDev ld, InstDev* __res
                 dt         = tmax - t;
                 dt_min     = fmin(tmax - t, dt_min);
             }
-            rhs<G>(c, L, y, t, k1);
+            rhs<G>(c, L, y, t, season_at<G>(L, t), k1);
             need_k1 = false;
             fail    = true;
             valid   = true;
@@ -564,11 +576,19 @@ __global__ void __launch_bounds__(BLOCK) node_phase(LaunchDev ld, InstDev* __res
             }
             else {
                 cash_karp_stages<G>(c, L, y, k1, t, dt, yo, er, true);
-                double e = 0.0;
+                // max_i |err_i| / (abs + rel (|y_i| + dt |k1_i|)) (odeint default_error_checker): the index of
+                // the largest quotient by cross-multiplied comparison, then one division
+                double en = 0.0, ed = 1.0;
 #pragma unroll
                 for (int i = 0; i < NC; ++i) {
-                    e = fmax(e, fabs(er[i]) / (I.abs_tol + I.rel_tol * (fabs(y[i]) + dt * fabs(k1[i]))));
+                    const double n = fabs(er[i]);
+                    const double d = I.abs_tol + I.rel_tol * (fabs(y[i]) + dt * fabs(k1[i]));
+                    if (n * ed > en * d) {
+                        en = n;
+                        ed = d;
+                    }
                 }
+                double e = en / ed;
                 e = group_max<G>(e, L.base);
                 if (e > 1.0) {
                     dt *= fmax(0.9 * pow(e, -1.0 / 3.0), 0.2);
@@ -641,7 +661,7 @@ __global__ void __launch_bounds__(BLOCK) node_phase(LaunchDev ld, InstDev* __res
                 const double dd = double(next_day + I.day0);
                 if (dd > ta) {
                     const double w = (dd - ta) / (t - ta);
-                    double* op     = out_ptr(c, I, L.node, L.age, G, next_day);
+                    double* op     = out_ptr(I, L.np.local, L.age, G, next_day);
 #pragma unroll
                     for (int i = 0; i < NC; ++i) {
                         op[i] = y[i] + w * (yo[i] - y[i]);
@@ -687,7 +707,7 @@ __global__ void __launch_bounds__(BLOCK) node_phase(LaunchDev ld, InstDev* __res
         }
     }
     else if (I.c_out_day >= 0) {
-        double* op = out_ptr(c, I, L.node, L.age, G, I.c_out_day);
+        double* op = out_ptr(I, L.np.local, L.age, G, I.c_out_day);
 #pragma unroll
         for (int i = 0; i < NC; ++i) {
             op[i] = y[i];
@@ -775,12 +795,7 @@ __device__ __forceinline__ void return_euler(const double* rc, double h, double 
 template <int G>
 __device__ __forceinline__ bool group_any(bool v, int base)
 {
-    int s = __shfl(int(v), base);
-#pragma unroll
-    for (int j = 1; j < G; ++j) {
-        s |= __shfl(int(v), base + j);
-    }
-    return s != 0;
+    return ((__ballot(v) >> base) & ((1ull << G) - 1)) != 0;
 }
 
 // MobilityEdge::apply_mobility return branch fix-up (metapopulation_mobility_instant.h:576-596): every
@@ -832,7 +847,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
         return;
     }
     const EdgeStructDev& es = c.estructs[I.estruct];
-    const int v             = node - I.node_base;
+    const int v             = c.nodep[node].local;
     const int in_b = es.in_ptr[v], in_e = es.in_ptr[v + 1];
     const int in_s          = in_b + es.split[v];
     const int out_b = es.out_ptr[v], out_e = es.out_ptr[v + 1];
@@ -840,7 +855,8 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
     const EdgeEnt* oute     = es.out_ent;
     const double* cvec      = es.cvec;
     const size_t lane_off   = size_t(node) * G + age;
-    const size_t src_base   = size_t(I.node_base) * G + age;
+    const size_t src_base   = size_t(I.node_base) * G + age;   // lane of local node 0
+    const size_t src_step   = size_t(I.node_stride) * G;       // lanes between local nodes
     double* yp              = c.y + lane_off * NC;
     double y[NC];
 #pragma unroll
@@ -851,7 +867,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
         for (; k < end; ++k) {
             const EdgeEnt e = ine[k];
             double r[SNAP_STRIDE];
-            load_row(c.snap, src_base + size_t(e.nbr) * G, r);
+            load_row(c.snap, src_base + size_t(e.nbr) * src_step, r);
             double m[NC];
             traveler_vector(cvec, e, r, G, age, true, m);
 #pragma unroll
@@ -889,7 +905,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
         yp[i] = y[i];
     }
     if (I.c_out_day >= 0) {
-        double* op = out_ptr(c, I, node, age, G, I.c_out_day);
+        double* op = out_ptr(I, v, age, G, I.c_out_day);
 #pragma unroll
         for (int i = 0; i < NC; ++i) {
             op[i] = y[i];
@@ -946,7 +962,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3)))
         return;
     }
     const EdgeStructDev& es = c.estructs[I.estruct];
-    const int v             = node - I.node_base;
+    const int v             = c.nodep[node].local;
     const int in_b = es.in_ptr[v], in_e = es.in_ptr[v + 1];
     const int in_s          = in_b + es.split[v];
     const int out_b = es.out_ptr[v], out_e = es.out_ptr[v + 1];
@@ -954,7 +970,8 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3)))
     const EdgeEnt* oute     = es.out_ent;
     const double* cvec      = es.cvec;
     const size_t lane_off   = size_t(node) * G + age;
-    const size_t src_base   = size_t(I.node_base) * G + age;
+    const size_t src_base   = size_t(I.node_base) * G + age;   // lane of local node 0
+    const size_t src_step   = size_t(I.node_stride) * G;       // lanes between local nodes
     FxRec* recs             = c.fx_rec + 2 * I.fx_base;
     double* pool            = c.fx_pool + I.fx_pool_base;
     const int32_t lo        = I.fx_epoch;
@@ -970,7 +987,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3)))
     auto in_edge = [&](int k) {
         const EdgeEnt e = ine[k];
         double row[SNAP_STRIDE];
-        load_row(c.snap, src_base + size_t(e.nbr) * G, row);
+        load_row(c.snap, src_base + size_t(e.nbr) * src_step, row);
         double m[NC];
         traveler_vector(cvec, e, row, G, age, true, m);
         return_euler(rc_v, h, m);
@@ -1008,7 +1025,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3)))
                 FxRec* r        = recs + 2 * k;
                 const int older = (r[0].tag <= r[1].tag) ? 0 : 1;
                 r[older]        = FxRec{tag, slot};
-                dirty_out[I.node_base + e.nbr] = tag; // the source re-walks in the next pass
+                dirty_out[seg_node(I, e.nbr)] = tag; // the source re-walks in the next pass
             }
             changed = true;
         }
@@ -1035,7 +1052,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3)))
             }
             else {
                 traveler_vector(cvec, e, sv, G, age, true, m);
-                return_euler(c.rc + (src_base + size_t(e.nbr) * G) * RC_STRIDE, h, m);
+                return_euler(c.rc + (src_base + size_t(e.nbr) * src_step) * RC_STRIDE, h, m);
             }
 #pragma unroll
             for (int i = 0; i < NC; ++i) {
@@ -1055,7 +1072,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3)))
         yp[i] = y[i];
     }
     if (I.c_out_day >= 0) {
-        double* op = out_ptr(c, I, node, age, G, I.c_out_day);
+        double* op = out_ptr(I, v, age, G, I.c_out_day);
 #pragma unroll
         for (int i = 0; i < NC; ++i) {
             op[i] = y[i];
@@ -1105,7 +1122,7 @@ __global__ void __launch_bounds__(WAVE) replay_phase(LaunchDev ld, InstDev* __re
     // restore the pre-return state of every node
     for (int v = 0; v < I.num_nodes; ++v) {
         if (a < G) {
-            const size_t off = (size_t(I.node_base + v) * G + a) * NC;
+            const size_t off = (seg_node(I, v) * G + a) * NC;
             for (int i = 0; i < NC; ++i) {
                 c.y[off + i] = c.ypre[off + i];
             }
@@ -1119,11 +1136,9 @@ __global__ void __launch_bounds__(WAVE) replay_phase(LaunchDev ld, InstDev* __re
     const double h          = I.c_step;
     if (a < G) {
         for (int k = 0; k < es.num_edges; ++k) {
-            const int s       = I.node_base + seq[3 * k];
-            const int d       = I.node_base + seq[3 * k + 1];
             const EdgeEnt e   = oute[seq[3 * k + 2]];
-            const size_t soff = size_t(s) * G + a;
-            const size_t doff = size_t(d) * G + a;
+            const size_t soff = seg_node(I, seq[3 * k]) * G + a;
+            const size_t doff = seg_node(I, seq[3 * k + 1]) * G + a;
             double row[SNAP_STRIDE];
             load_row(c.snap, soff, row);
             double m[NC];
@@ -1151,8 +1166,8 @@ __global__ void __launch_bounds__(WAVE) replay_phase(LaunchDev ld, InstDev* __re
     }
     if (I.c_out_day >= 0 && a < G) {
         for (int v = 0; v < I.num_nodes; ++v) {
-            const size_t off = (size_t(I.node_base + v) * G + a) * NC;
-            double* op       = out_ptr(c, I, I.node_base + v, a, G, I.c_out_day);
+            const size_t off = (seg_node(I, v) * G + a) * NC;
+            double* op       = out_ptr(I, v, a, G, I.c_out_day);
             for (int i = 0; i < NC; ++i) {
                 op[i] = c.y[off + i];
             }

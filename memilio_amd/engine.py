@@ -43,6 +43,10 @@ class Context:
         """MGO_CTX_* flags for the runs started afterwards."""
         self.check(self.lib.mgo_context_set_flags(self.ptr, flags))
 
+    def set_tile(self, tile):
+        """Runs per node-major tile for ensembles created afterwards (0 = autotuned)."""
+        self.check(self.lib.mgo_context_set_tile(self.ptr, tile))
+
     def clock_ms(self):
         return self.lib.mgo_clock_ms(self.ptr)
 

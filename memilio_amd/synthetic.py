@@ -66,7 +66,7 @@ def random_edges(rng, n, mean_degree=50.0, total_outflow=0.3):
         keys[np.arange(n), np.arange(n)] = np.inf
         order = np.argsort(keys, axis=1)
         src = np.repeat(np.arange(n), deg)
-        pos = np.concatenate([np.arange(d) for d in deg])
+        pos = np.arange(src.size) - np.repeat(np.cumsum(deg) - deg, deg)
         pairs = np.unique(src * n + order[src, pos])
     else:
         pairs = np.zeros(0, np.int64)
