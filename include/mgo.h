@@ -247,6 +247,22 @@ mgo_status mgo_batch_read_daily(mgo_context* ctx, mgo_batch_id id, int32_t first
 mgo_status mgo_batch_reset(mgo_context* ctx, mgo_batch_id id);
 mgo_status mgo_batch_destroy(mgo_context* ctx, mgo_batch_id id);
 
+/* Ensemble statistics over runs first_run .. first_run+num_runs-1 of a batch, computed on the device from
+ * the resident daily outputs. The runs must have the same shape (age groups, integrator, node count,
+ * output days). Layout of a statistic: [day][node][10*G] like one run's daily output.
+ *   mgo_batch_ensemble_mean        == ensemble_mean (cpp/memilio/data/analyze_result.h:289-315): sum over
+ *                                     runs, in run order, of value / num_runs
+ *   mgo_batch_ensemble_percentiles == ensemble_percentile (analyze_result.h:316-343) for each p[q] in
+ *                                     (0, 1): the value at index floor(num_runs * p) of the sorted run values;
+ *                                     out[q][day][node][10*G]; at most 8192 runs
+ *   mgo_batch_sum_nodes            == sum_nodes (analyze_result.h:267-287): out[run][day][10*G], the nodes
+ *                                     summed in node order */
+mgo_status mgo_batch_ensemble_mean(mgo_context* ctx, mgo_batch_id id, int32_t first_run, int32_t num_runs,
+                                   double* mean_out);
+mgo_status mgo_batch_ensemble_percentiles(mgo_context* ctx, mgo_batch_id id, int32_t first_run, int32_t num_runs,
+                                          const double* p, int32_t num_p, double* out);
+mgo_status mgo_batch_sum_nodes(mgo_context* ctx, mgo_batch_id id, int32_t first_run, int32_t num_runs, double* out);
+
 /* ---------------------------------------------------------------- serving loop
  * A worker thread owned by the context runs half-day cycles over the resident requests; requests are
  * admitted as HBM budget and slots free up (continuous batching), so requests submitted while others

@@ -33,6 +33,12 @@ namespace mgo
 hipError_t launch_clock(InstDev* inst, int n, hipStream_t s);
 hipError_t launch_node(const LaunchDev& c, InstDev* inst, hipStream_t s);
 hipError_t launch_edge(const LaunchDev& c, InstDev* inst, hipStream_t s);
+hipError_t launch_ensemble_mean(const double* out, const int64_t* base, int runs, int64_t positions, double* mean,
+                                hipStream_t s);
+hipError_t launch_sum_nodes(const double* out, const int64_t* base, int runs, int days, int nodes, int width,
+                            double* sums, hipStream_t s);
+hipError_t launch_ensemble_select(const double* out, const int64_t* base, int runs, int64_t positions,
+                                  const int32_t* ranks, int num_ranks, double* sel, hipStream_t s);
 } // namespace mgo
 
 using namespace mgo;
@@ -66,11 +72,13 @@ void hip_check(hipError_t e, const char* what)
 }
 
 // First-fit sub-allocator over one device allocation (serving: per-request buffers without hipMalloc).
+// first-fit device memory arena (serving); alloc / free are thread-safe
 struct Arena {
     char* base  = nullptr;
     size_t size = 0;
     size_t used = 0;
     std::map<size_t, size_t> free_; // offset -> length
+    std::mutex mu;
     void init(size_t bytes)
     {
         size = bytes;
@@ -88,6 +96,7 @@ struct Arena {
     }
     void* alloc(size_t bytes)
     {
+        std::lock_guard<std::mutex> lk(mu);
         bytes = (bytes + 255) & ~size_t(255);
         for (auto it = free_.begin(); it != free_.end(); ++it) {
             if (it->second >= bytes) {
@@ -104,6 +113,7 @@ struct Arena {
     }
     void free(void* p, size_t bytes)
     {
+        std::lock_guard<std::mutex> lk(mu);
         bytes      = (bytes + 255) & ~size_t(255);
         size_t off = size_t(static_cast<char*>(p) - base);
         used -= bytes;
@@ -306,6 +316,22 @@ void block_tables(const std::vector<int>& seg_nodes, int G, std::vector<int32_t>
     for (size_t k = 0; k < seg_nodes.size(); ++k) {
         block_first.push_back(int32_t(block_seg.size()));
         const int nb = segment_blocks(seg_nodes[k], G);
+        block_seg.insert(block_seg.end(), size_t(nb), int32_t(k));
+    }
+    while (block_seg.empty() || block_seg.size() % 8 != 0) {
+        block_seg.push_back(-1);
+    }
+}
+
+// block -> segment table for segments of different G (mixed launches)
+void block_tables_mixed(const std::vector<int>& seg_nodes, const std::vector<int>& seg_G,
+                        std::vector<int32_t>& block_seg, std::vector<int32_t>& block_first)
+{
+    block_seg.clear();
+    block_first.clear();
+    for (size_t k = 0; k < seg_nodes.size(); ++k) {
+        block_first.push_back(int32_t(block_seg.size()));
+        const int nb = segment_blocks(seg_nodes[k], seg_G[k]);
         block_seg.insert(block_seg.end(), size_t(nb), int32_t(k));
     }
     while (block_seg.empty() || block_seg.size() % 8 != 0) {
@@ -542,6 +568,64 @@ struct Batch {
 
     int tile = 1; // runs per node-major tile (see Tiling in mgo_internal.h)
 
+    // one launch table over all classes of an integrator kind that has classes of several G
+    struct KindLaunch {
+        std::vector<int> cls;
+        bool any_edges = false;
+        DevBuf segs, block_seg, block_first, inst_ids;
+        LaunchDev launch{};
+    };
+    std::deque<KindLaunch> mixed;
+    std::vector<int> class_mixed; // class -> index into mixed, or -1
+
+    void build_mixed_tables(std::vector<InstDev>& idev)
+    {
+        std::map<int, std::vector<int>> by_kind;
+        for (size_t ci = 0; ci < classes.size(); ++ci) {
+            by_kind[classes[ci].kind].push_back(int(ci));
+        }
+        class_mixed.assign(classes.size(), -1);
+        for (auto& kv : by_kind) {
+            if (kv.second.size() < 2) {
+                continue;
+            }
+            mixed.emplace_back();
+            KindLaunch& K = mixed.back();
+            K.cls         = kv.second;
+            std::vector<SegDev> segs;
+            std::vector<int> nodes, gs;
+            std::vector<int32_t> ids;
+            for (size_t k = 0; k < K.cls.size(); ++k) {
+                ClassHost& C = classes[size_t(K.cls[k])];
+                class_mixed[size_t(K.cls[k])] = int(mixed.size()) - 1;
+                segs.push_back(C.seg_host);
+                nodes.push_back(C.num_nodes);
+                gs.push_back(C.G);
+                for (int ii : C.insts) {
+                    ids.push_back(inst_offset + ii);
+                    idev[size_t(ii)].seg = int32_t(k);
+                    K.any_edges          = K.any_edges || inst[size_t(ii)].has_edges;
+                }
+            }
+            std::vector<int32_t> bseg, bfirst;
+            block_tables_mixed(nodes, gs, bseg, bfirst);
+            K.segs.upload(segs, stream);
+            K.block_seg.upload(bseg, stream);
+            K.block_first.upload(bfirst, stream);
+            K.inst_ids.upload(ids, stream);
+            device_bytes += K.segs.bytes + K.block_seg.bytes + K.block_first.bytes + K.inst_ids.bytes;
+            LaunchDev& ld  = K.launch;
+            ld.segs        = K.segs.as<SegDev>();
+            ld.block_seg   = K.block_seg.as<int32_t>();
+            ld.block_first = K.block_first.as<int32_t>();
+            ld.inst_ids    = K.inst_ids.as<int32_t>();
+            ld.num_inst    = int(ids.size());
+            ld.num_blocks  = int(bseg.size());
+            ld.G           = 0;
+            ld.kind        = kv.first;
+        }
+    }
+
     void build(std::vector<RunSpec>& runs, hipStream_t s, int tile_runs = 1)
     {
         stream = s;
@@ -578,6 +662,7 @@ struct Batch {
         for (size_t ci = 0; ci < classes.size(); ++ci) {
             build_class(ci, runs, idev);
         }
+        build_mixed_tables(idev);
         inst_dev.upload(idev, stream);
         device_bytes += inst_dev.bytes;
         hip_check(hipStreamSynchronize(stream), "batch upload");
@@ -899,45 +984,57 @@ struct Batch {
 
     // enqueue one half-day cycle for every instance; with ev != nullptr, events bracket the node and the
     // edge launches (ev[0..1] node, ev[2..3] edge)
-    void enqueue_cycle(hipEvent_t* ev = nullptr)
+    // launch tables of a cycle: every class on its own, except classes covered by a mixed table
+    void launch_tables(std::vector<const LaunchDev*>& node_ld, std::vector<const LaunchDev*>& edge_ld) const
     {
-        InstDev* id = inst_dev.as<InstDev>();
-        hip_check(launch_clock(id, int(inst.size()), stream), "clock kernel");
-        if (ev) {
-            hip_check(hipEventRecord(ev[0], stream), "event");
-        }
-        for (auto& C : classes) {
-            hip_check(launch_node(C.launch, id, stream), "node kernel");
-        }
-        if (ev) {
-            hip_check(hipEventRecord(ev[1], stream), "event");
-            hip_check(hipEventRecord(ev[2], stream), "event");
-        }
-        for (auto& C : classes) {
+        node_ld.clear();
+        edge_ld.clear();
+        for (size_t ci = 0; ci < classes.size(); ++ci) {
+            if (!class_mixed.empty() && class_mixed[ci] >= 0) {
+                continue;
+            }
+            const ClassHost& C = classes[ci];
+            node_ld.push_back(&C.launch);
             bool any_edges = false;
             for (int ii : C.insts) {
                 any_edges = any_edges || inst[size_t(ii)].has_edges;
             }
             if (any_edges) {
-                hip_check(launch_edge(C.launch, id, stream), "edge kernel");
+                edge_ld.push_back(&C.launch);
             }
+        }
+        for (const KindLaunch& K : mixed) {
+            node_ld.push_back(&K.launch);
+            if (K.any_edges) {
+                edge_ld.push_back(&K.launch);
+            }
+        }
+    }
+
+    // enqueue one half-day cycle for every instance; with ev != nullptr, events bracket the node and the
+    // edge launches (ev[0..1] node, ev[2..3] edge)
+    void enqueue_cycle(hipEvent_t* ev = nullptr)
+    {
+        std::vector<const LaunchDev*> node_ld, edge_ld;
+        launch_tables(node_ld, edge_ld);
+        InstDev* id = inst_dev.as<InstDev>();
+        hip_check(launch_clock(id, int(inst.size()), stream), "clock kernel");
+        if (ev) {
+            hip_check(hipEventRecord(ev[0], stream), "event");
+        }
+        for (const LaunchDev* ld : node_ld) {
+            hip_check(launch_node(*ld, id, stream), "node kernel");
+        }
+        if (ev) {
+            hip_check(hipEventRecord(ev[1], stream), "event");
+            hip_check(hipEventRecord(ev[2], stream), "event");
+        }
+        for (const LaunchDev* ld : edge_ld) {
+            hip_check(launch_edge(*ld, id, stream), "edge kernel");
         }
         if (ev) {
             hip_check(hipEventRecord(ev[3], stream), "event");
         }
-    }
-
-    int edge_classes() const
-    {
-        int n = 0;
-        for (auto& C : classes) {
-            bool any_edges = false;
-            for (int ii : C.insts) {
-                any_edges = any_edges || inst[size_t(ii)].has_edges;
-            }
-            n += any_edges ? 1 : 0;
-        }
-        return n;
     }
 
     // number of GraphSimulation loop iterations from the host mirror of every clock to its tmax
@@ -1019,8 +1116,10 @@ struct Batch {
             edge_ms += b;
         }
         if (!evs.empty()) {
-            node_launches = int(n) * int(classes.size());
-            edge_launches = int(n) * edge_classes();
+            std::vector<const LaunchDev*> node_ld, edge_ld;
+            launch_tables(node_ld, edge_ld);
+            node_launches = int(n) * int(node_ld.size());
+            edge_launches = int(n) * int(edge_ld.size());
         }
         for (auto& e : evs) {
             (void)hipEventDestroy(e);
@@ -1651,6 +1750,114 @@ mgo_status mgo_batch_read_daily(mgo_context* ctx, mgo_batch_id id, int32_t first
             fail(MGO_ERR_OUT_OF_RANGE, "runs %d..%d out of range", first_run, first_run + num_runs);
         }
         read_all_daily(*b, daily_out, size_t(first_run), size_t(num_runs));
+    });
+}
+
+// Runs [first, first + num) of a batch as one ensemble: same class, node count, age groups and days.
+// Fills the per-run output offsets (doubles into the class's output buffer) on the device.
+struct EnsembleView {
+    const double* out = nullptr;
+    int nodes = 0, G = 0, days = 0;
+    DevBuf base;
+};
+
+static void ensemble_view(Batch& b, int32_t first, int32_t num, EnsembleView& v)
+{
+    if (first < 0 || num < 1 || size_t(first) + size_t(num) > b.inst.size()) {
+        fail(MGO_ERR_OUT_OF_RANGE, "runs %d..%d outside the batch", first, first + num - 1);
+    }
+    const InstHost& I0 = b.inst[size_t(first)];
+    std::vector<int64_t> base;
+    for (int r = first; r < first + num; ++r) {
+        const InstHost& I = b.inst[size_t(r)];
+        if (I.cls != I0.cls || I.num_nodes != I0.num_nodes || I.ndays != I0.ndays || I.day0 != I0.day0) {
+            fail(MGO_ERR_INVALID_VALUE, "run %d differs in shape from run %d (nodes, age groups, days)", r, first);
+        }
+        base.push_back(I.out_base);
+    }
+    if (I0.ndays < 1) {
+        fail(MGO_ERR_INVALID_VALUE, "runs have no daily output");
+    }
+    v.out   = b.classes[size_t(I0.cls)].out.as<double>();
+    v.nodes = I0.num_nodes;
+    v.G     = I0.G;
+    v.days  = I0.ndays;
+    v.base.upload(base, b.stream);
+}
+
+static void download(Batch& b, const DevBuf& src, size_t doubles, double* dst)
+{
+    hip_check(hipMemcpyAsync(dst, src.p, doubles * sizeof(double), hipMemcpyDeviceToHost, b.stream), "download");
+    hip_check(hipStreamSynchronize(b.stream), "download");
+}
+
+mgo_status mgo_batch_ensemble_mean(mgo_context* ctx, mgo_batch_id id, int32_t first_run, int32_t num_runs,
+                                   double* mean_out)
+{
+    return guarded(ctx, [&] {
+        Batch* b = find_batch(ctx->batches, id);
+        if (!mean_out) {
+            fail(MGO_ERR_INVALID_VALUE, "output is NULL");
+        }
+        EnsembleView v;
+        ensemble_view(*b, first_run, num_runs, v);
+        const int64_t positions = int64_t(v.days) * v.nodes * v.G * NC;
+        DevBuf mean;
+        mean.alloc(size_t(positions) * sizeof(double));
+        hip_check(launch_ensemble_mean(v.out, v.base.as<int64_t>(), num_runs, positions, mean.as<double>(), b->stream),
+                  "ensemble mean");
+        download(*b, mean, size_t(positions), mean_out);
+    });
+}
+
+mgo_status mgo_batch_ensemble_percentiles(mgo_context* ctx, mgo_batch_id id, int32_t first_run, int32_t num_runs,
+                                          const double* p, int32_t num_p, double* out)
+{
+    return guarded(ctx, [&] {
+        Batch* b = find_batch(ctx->batches, id);
+        if (!out || !p || num_p < 1) {
+            fail(MGO_ERR_INVALID_VALUE, "percentiles or output missing");
+        }
+        if (num_runs > 8192) {
+            fail(MGO_ERR_UNSUPPORTED, "percentiles over more than 8192 runs");
+        }
+        std::vector<int32_t> ranks;
+        for (int q = 0; q < num_p; ++q) {
+            if (!(p[q] > 0.0 && p[q] < 1.0)) {
+                fail(MGO_ERR_INVALID_VALUE, "percentile %g outside (0, 1)", p[q]);
+            }
+            ranks.push_back(int32_t(static_cast<size_t>(double(num_runs) * p[q])));
+        }
+        EnsembleView v;
+        ensemble_view(*b, first_run, num_runs, v);
+        const int64_t positions = int64_t(v.days) * v.nodes * v.G * NC;
+        DevBuf dranks, sel;
+        dranks.upload(ranks, b->stream);
+        sel.alloc(size_t(positions) * size_t(num_p) * sizeof(double));
+        hip_check(launch_ensemble_select(v.out, v.base.as<int64_t>(), num_runs, positions, dranks.as<int32_t>(), num_p,
+                                         sel.as<double>(), b->stream),
+                  "ensemble percentiles");
+        download(*b, sel, size_t(positions) * size_t(num_p), out);
+    });
+}
+
+mgo_status mgo_batch_sum_nodes(mgo_context* ctx, mgo_batch_id id, int32_t first_run, int32_t num_runs, double* out)
+{
+    return guarded(ctx, [&] {
+        Batch* b = find_batch(ctx->batches, id);
+        if (!out) {
+            fail(MGO_ERR_INVALID_VALUE, "output is NULL");
+        }
+        EnsembleView v;
+        ensemble_view(*b, first_run, num_runs, v);
+        const int width = v.G * NC;
+        const size_t n  = size_t(num_runs) * size_t(v.days) * size_t(width);
+        DevBuf sums;
+        sums.alloc(n * sizeof(double));
+        hip_check(launch_sum_nodes(v.out, v.base.as<int64_t>(), num_runs, v.days, v.nodes, width, sums.as<double>(),
+                                   b->stream),
+                  "sum nodes");
+        download(*b, sums, n, out);
     });
 }
 

@@ -216,7 +216,7 @@ struct LaunchDev {
     const int32_t* inst_ids;    // instances of this class
     int32_t num_inst;
     int32_t num_blocks;         // launched blocks (multiple of 8)
-    int32_t G;
+    int32_t G;                  // age groups of every segment, or 0: segments differ (mixed kernels)
     int32_t kind;
 };
 

@@ -18,6 +18,7 @@
 //   replay_phase<G>    per flagged instance: sequential return pass with the negative-value fix-up
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <cstdint>
 
@@ -483,7 +484,7 @@ __device__ __forceinline__ size_t seg_node(const InstDev& I, int v)
 constexpr uint32_t kMaxAttemptsPerPhase = 1u << 20;
 
 template <int G, int K>
-__global__ void __launch_bounds__(BLOCK) node_phase(LaunchDev ld, InstDev* __restrict__ inst)
+__device__ __forceinline__ void node_body(const LaunchDev& ld, InstDev* __restrict__ inst)
 {
     Lane<G> L;
     const SegDev* cp = load_lane<G>(ld, L);
@@ -833,8 +834,7 @@ __device__ __forceinline__ bool fixup_negative(const double (&y)[NC], double (&m
 // edges as in[s < v], out[all], in[s > v]; the lane keeps its age group's compartments in registers and
 // walks that sequence (a deterministic segmented sum in the reference's order, no atomics).
 template <int G>
-__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) depart_phase(LaunchDev ld,
-                                                                                       InstDev* __restrict__ inst)
+__device__ __forceinline__ void depart_body(const LaunchDev& ld, InstDev* __restrict__ inst)
 {
     int node, age, base;
     const SegDev* cp = lane_map<G>(ld, node, age, base);
@@ -938,8 +938,7 @@ __device__ __forceinline__ int32_t record_slot(const FxRec* r, int32_t lo, int32
 // reproduces the sequential result exactly. A node whose out-edge records did not change in pass p-1
 // would repeat its walk of pass p-1 unchanged, so pass p only re-walks the nodes marked dirty then.
 template <int G, bool FIRST>
-__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) return_phase(LaunchDev ld,
-                                                                                       InstDev* __restrict__ inst)
+__device__ __forceinline__ void return_body(const LaunchDev& ld, InstDev* __restrict__ inst)
 {
     int node, age, base;
     const SegDev* cp = lane_map<G>(ld, node, age, base);
@@ -1110,7 +1109,7 @@ __global__ void fixup_check(LaunchDev ld, InstDev* inst)
 // (metapopulation_mobility_instant.h:568-603). Fallback for the rare return phases whose fix-up records
 // did not settle within FX_MAX_PASSES passes or overflowed the record pool; lanes = age groups.
 template <int G>
-__global__ void __launch_bounds__(WAVE) replay_phase(LaunchDev ld, InstDev* __restrict__ inst)
+__device__ __forceinline__ void replay_body(const LaunchDev& ld, InstDev* __restrict__ inst)
 {
     const int ii = ld.inst_ids[blockIdx.x];
     InstDev& I   = inst[ii];
@@ -1175,6 +1174,86 @@ __global__ void __launch_bounds__(WAVE) replay_phase(LaunchDev ld, InstDev* __re
     }
 }
 
+// ------------------------------------------------------------------------------- kernels
+// One kernel per (G, integrator) for launches over segments of one G; the *_mixed kernels serve launch
+// tables whose segments have different G (LaunchDev::G == 0): each block reads its segment's G and runs
+// the body specialised for it (blocks never span segments).
+__device__ __forceinline__ int block_groups(const LaunchDev& ld)
+{
+    const int lb = xcd_block();
+    if (lb >= ld.num_blocks) {
+        return 0;
+    }
+    const int seg = ld.block_seg[lb];
+    return seg < 0 ? 0 : ld.segs[seg].G;
+}
+
+#define MGO_DISPATCH_G(G_EXPR, CALL)                                                                           \
+    switch (G_EXPR) {                                                                                          \
+    case 1: { constexpr int GG = 1; CALL; } break;                                                             \
+    case 2: { constexpr int GG = 2; CALL; } break;                                                             \
+    case 3: { constexpr int GG = 3; CALL; } break;                                                             \
+    case 4: { constexpr int GG = 4; CALL; } break;                                                             \
+    case 5: { constexpr int GG = 5; CALL; } break;                                                             \
+    case 6: { constexpr int GG = 6; CALL; } break;                                                             \
+    case 7: { constexpr int GG = 7; CALL; } break;                                                             \
+    case 8: { constexpr int GG = 8; CALL; } break;                                                             \
+    default: break;                                                                                            \
+    }
+
+template <int G, int K>
+__global__ void __launch_bounds__(BLOCK) node_phase(LaunchDev ld, InstDev* __restrict__ inst)
+{
+    node_body<G, K>(ld, inst);
+}
+
+template <int K>
+__global__ void __launch_bounds__(BLOCK) node_phase_mixed(LaunchDev ld, InstDev* __restrict__ inst)
+{
+    MGO_DISPATCH_G(block_groups(ld), (node_body<GG, K>(ld, inst)))
+}
+
+template <int G>
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) depart_phase(LaunchDev ld,
+                                                                                       InstDev* __restrict__ inst)
+{
+    depart_body<G>(ld, inst);
+}
+
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) depart_phase_mixed(LaunchDev ld,
+                                                                                             InstDev* __restrict__ inst)
+{
+    MGO_DISPATCH_G(block_groups(ld), (depart_body<GG>(ld, inst)))
+}
+
+template <int G, bool FIRST>
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) return_phase(LaunchDev ld,
+                                                                                       InstDev* __restrict__ inst)
+{
+    return_body<G, FIRST>(ld, inst);
+}
+
+template <bool FIRST>
+__global__ void __launch_bounds__(BLOCK) return_phase_mixed(LaunchDev ld, InstDev* __restrict__ inst)
+{
+    MGO_DISPATCH_G(block_groups(ld), (return_body<GG, FIRST>(ld, inst)))
+}
+
+template <int G>
+__global__ void __launch_bounds__(WAVE) replay_phase(LaunchDev ld, InstDev* __restrict__ inst)
+{
+    replay_body<G>(ld, inst);
+}
+
+__global__ void __launch_bounds__(WAVE) replay_phase_mixed(LaunchDev ld, InstDev* __restrict__ inst)
+{
+    const InstDev& I = inst[ld.inst_ids[blockIdx.x]];
+    if (!I.c_active || I.c_action != 2 || I.fx_state != 2) {
+        return;
+    }
+    MGO_DISPATCH_G(ld.segs[I.seg].G, (replay_body<GG>(ld, inst)))
+}
+
 // ------------------------------------------------------------------------------- launchers
 
 template <int G>
@@ -1227,9 +1306,53 @@ hipError_t launch_clock(InstDev* inst, int n, hipStream_t s)
     return hipGetLastError();
 }
 
+static hipError_t launch_node_mixed(const LaunchDev& c, InstDev* inst, hipStream_t s)
+{
+    const int nb = c.num_blocks;
+    switch (c.kind) {
+    case MGO_INTEGRATOR_CASH_KARP54:
+        hipLaunchKernelGGL((node_phase_mixed<MGO_INTEGRATOR_CASH_KARP54>), dim3(nb), dim3(BLOCK), 0, s, c, inst);
+        break;
+    case MGO_INTEGRATOR_RKF45:
+        hipLaunchKernelGGL((node_phase_mixed<MGO_INTEGRATOR_RKF45>), dim3(nb), dim3(BLOCK), 0, s, c, inst);
+        break;
+    case MGO_INTEGRATOR_CASH_KARP54_FIXED:
+        hipLaunchKernelGGL((node_phase_mixed<MGO_INTEGRATOR_CASH_KARP54_FIXED>), dim3(nb), dim3(BLOCK), 0, s, c, inst);
+        break;
+    default:
+        hipLaunchKernelGGL((node_phase_mixed<MGO_INTEGRATOR_EULER>), dim3(nb), dim3(BLOCK), 0, s, c, inst);
+        break;
+    }
+    return hipGetLastError();
+}
+
+static hipError_t launch_edge_mixed(const LaunchDev& c, InstDev* inst, hipStream_t s)
+{
+    const int nb = c.num_blocks;
+    hipLaunchKernelGGL(depart_phase_mixed, dim3(nb), dim3(BLOCK), 0, s, c, inst);
+    hipLaunchKernelGGL((return_phase_mixed<true>), dim3(nb), dim3(BLOCK), 0, s, c, inst);
+    hipError_t err = hipGetLastError();
+    const int ni   = (c.num_inst + 255) / 256;
+    for (int p = 1; p < FX_MAX_PASSES && err == hipSuccess; ++p) {
+        hipLaunchKernelGGL(fixup_check, dim3(ni), dim3(256), 0, s, c, inst);
+        hipLaunchKernelGGL((return_phase_mixed<false>), dim3(nb), dim3(BLOCK), 0, s, c, inst);
+        err = hipGetLastError();
+    }
+    if (err != hipSuccess) {
+        return err;
+    }
+    hipLaunchKernelGGL(fixup_check, dim3(ni), dim3(256), 0, s, c, inst);
+    if (c.num_inst > 0) {
+        hipLaunchKernelGGL(replay_phase_mixed, dim3(c.num_inst), dim3(WAVE), 0, s, c, inst);
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_node(const LaunchDev& c, InstDev* inst, hipStream_t s)
 {
     switch (c.G) {
+    case 0:
+        return launch_node_mixed(c, inst, s);
     case 1:
         return launch_node_g<1>(c, inst, s);
     case 2:
@@ -1254,6 +1377,8 @@ hipError_t launch_node(const LaunchDev& c, InstDev* inst, hipStream_t s)
 hipError_t launch_edge(const LaunchDev& c, InstDev* inst, hipStream_t s)
 {
     switch (c.G) {
+    case 0:
+        return launch_edge_mixed(c, inst, s);
     case 1:
         return launch_edge_g<1>(c, inst, s);
     case 2:
@@ -1273,6 +1398,136 @@ hipError_t launch_edge(const LaunchDev& c, InstDev* inst, hipStream_t s)
     default:
         return hipErrorInvalidValue;
     }
+}
+
+} // namespace mgo
+
+namespace mgo
+{
+// ------------------------------------------------------------------------------- ensemble analysis
+// Post-processing of the daily outputs of several runs of the same shape (cpp/memilio/data/
+// analyze_result.h:267-343). A "position" is one (day, node, compartment) value of a run's output; run r's
+// output starts at base[r] (doubles). Every kernel reproduces the reference's arithmetic order.
+
+// ensemble_mean (analyze_result.h:289-315): mean = sum over runs, in run order, of value / num_runs
+__global__ void ensemble_mean_kernel(const double* __restrict__ out, const int64_t* __restrict__ base, int runs,
+                                     int64_t positions, double* __restrict__ mean)
+{
+    const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= positions) {
+        return;
+    }
+    const double n = double(runs);
+    double acc     = 0.0;
+    for (int r = 0; r < runs; ++r) {
+        acc += out[base[r] + p] / n;
+    }
+    mean[p] = acc;
+}
+
+// sum_nodes (analyze_result.h:267-287): per run and day, the node values summed in node order
+__global__ void sum_nodes_kernel(const double* __restrict__ out, const int64_t* __restrict__ base, int runs,
+                                 int days, int nodes, int width, double* __restrict__ sums)
+{
+    const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; // (run, day, element)
+    if (q >= int64_t(runs) * days * width) {
+        return;
+    }
+    const int e   = int(q % width);
+    const int64_t rd = q / width;
+    const int d   = int(rd % days);
+    const int r   = int(rd / days);
+    const double* src = out + base[r] + (int64_t(d) * nodes) * width + e;
+    double acc        = 0.0;
+    for (int v = 0; v < nodes; ++v) {
+        acc += src[int64_t(v) * width];
+    }
+    sums[q] = acc;
+}
+
+// ensemble_percentile (analyze_result.h:316-343): the value at index floor(num_runs * p) of the sorted
+// run values of each position. A block sorts the values of `ppb` consecutive positions in LDS (bitonic
+// sort over a power-of-two length padded with +inf) and writes the requested order statistics.
+__global__ void __launch_bounds__(256) ensemble_select_kernel(const double* __restrict__ out,
+                                                               const int64_t* __restrict__ base, int runs, int npad,
+                                                               int ppb, int64_t positions,
+                                                               const int32_t* __restrict__ ranks, int num_ranks,
+                                                               double* __restrict__ sel)
+{
+    extern __shared__ double lds[];
+    const int64_t p0 = int64_t(blockIdx.x) * ppb;
+    const int total  = ppb * npad;
+    for (int i = int(threadIdx.x); i < total; i += int(blockDim.x)) {
+        const int j      = i % ppb; // position within the block (fastest: adjacent addresses)
+        const int r      = i / ppb;
+        const int64_t p  = p0 + j;
+        double v         = INFINITY;
+        if (r < runs && p < positions) {
+            v = out[base[r] + p];
+        }
+        lds[j * npad + r] = v;
+    }
+    __syncthreads();
+    const int half = total / 2;
+    for (int k = 2; k <= npad; k <<= 1) {
+        for (int s = k >> 1; s > 0; s >>= 1) {
+            for (int i = int(threadIdx.x); i < half; i += int(blockDim.x)) {
+                const int j   = i / (npad / 2);            // segment
+                const int t   = i % (npad / 2);            // pair index within the segment
+                const int lo  = 2 * s * (t / s) + (t % s); // first element of the pair
+                const int hi  = lo + s;
+                const bool up = ((lo & k) == 0);
+                double* seg   = lds + j * npad;
+                const double a = seg[lo], b = seg[hi];
+                if ((a > b) == up) {
+                    seg[lo] = b;
+                    seg[hi] = a;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int i = int(threadIdx.x); i < ppb * num_ranks; i += int(blockDim.x)) {
+        const int j     = i % ppb;
+        const int q     = i / ppb;
+        const int64_t p = p0 + j;
+        if (p < positions) {
+            sel[int64_t(q) * positions + p] = lds[j * npad + ranks[q]];
+        }
+    }
+}
+
+hipError_t launch_ensemble_mean(const double* out, const int64_t* base, int runs, int64_t positions, double* mean,
+                                hipStream_t s)
+{
+    const int64_t nb = (positions + 255) / 256;
+    hipLaunchKernelGGL(ensemble_mean_kernel, dim3(unsigned(nb)), dim3(256), 0, s, out, base, runs, positions, mean);
+    return hipGetLastError();
+}
+
+hipError_t launch_sum_nodes(const double* out, const int64_t* base, int runs, int days, int nodes, int width,
+                            double* sums, hipStream_t s)
+{
+    const int64_t n  = int64_t(runs) * days * width;
+    const int64_t nb = (n + 255) / 256;
+    hipLaunchKernelGGL(sum_nodes_kernel, dim3(unsigned(nb)), dim3(256), 0, s, out, base, runs, days, nodes, width,
+                       sums);
+    return hipGetLastError();
+}
+
+hipError_t launch_ensemble_select(const double* out, const int64_t* base, int runs, int64_t positions,
+                                  const int32_t* ranks, int num_ranks, double* sel, hipStream_t s)
+{
+    int npad = 2;
+    while (npad < runs) {
+        npad <<= 1;
+    }
+    const int ppb = std::max(1, 8192 / npad); // 64 KiB of LDS per block
+    const size_t lds = size_t(ppb) * npad * sizeof(double);
+    const int64_t nb = (positions + ppb - 1) / ppb;
+    hipLaunchKernelGGL(ensemble_select_kernel, dim3(unsigned(nb)), dim3(256), lds, s, out, base, runs, npad, ppb,
+                       positions, ranks, num_ranks, sel);
+    return hipGetLastError();
 }
 
 } // namespace mgo

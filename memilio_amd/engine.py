@@ -117,6 +117,34 @@ class Batch:
             o += sz
         return res
 
+    def _shape(self, first, count):
+        count = len(self.specs) - first if count is None else count
+        s, c, t = self.specs[first], self.cfgs[first], self.tmaxs[first]
+        return count, s.num_days(c.t0, t), s.num_nodes, s.state_size()
+
+    def ensemble_mean(self, first=0, count=None):
+        """ensemble_mean over runs [first, first+count): [day][node][10G], computed on the device."""
+        count, nd, n, w = self._shape(first, count)
+        out = np.zeros((nd, n, w))
+        self.ctx.check(self.ctx.lib.mgo_batch_ensemble_mean(self.ctx.ptr, self.id, first, count, capi.dptr(out)))
+        return out
+
+    def ensemble_percentiles(self, ps, first=0, count=None):
+        """ensemble_percentile for each p in ps: [len(ps)][day][node][10G], computed on the device."""
+        count, nd, n, w = self._shape(first, count)
+        p = np.ascontiguousarray(ps, dtype=np.float64)
+        out = np.zeros((p.size, nd, n, w))
+        self.ctx.check(self.ctx.lib.mgo_batch_ensemble_percentiles(self.ctx.ptr, self.id, first, count, capi.dptr(p),
+                                                                   p.size, capi.dptr(out)))
+        return out
+
+    def sum_nodes(self, first=0, count=None):
+        """sum_nodes of runs [first, first+count): [run][day][10G], computed on the device."""
+        count, nd, n, w = self._shape(first, count)
+        out = np.zeros((count, nd, w))
+        self.ctx.check(self.ctx.lib.mgo_batch_sum_nodes(self.ctx.ptr, self.id, first, count, capi.dptr(out)))
+        return out
+
     def close(self):
         if self.id.value:
             self.ctx.lib.mgo_batch_destroy(self.ctx.ptr, self.id)

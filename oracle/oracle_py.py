@@ -173,3 +173,33 @@ def integrate_test_problem(cfg, t0, tmax, dt, capacity=100000):
 
 def contact_at(spec, t, i, j):
     return lib().orc_contact_at(C.byref(spec.desc()), t, i, j)
+
+
+# ------------------------------------------------------------------------------- ensemble analysis
+# cpp/memilio/data/analyze_result.h:267-343, restated over arrays results[run][day][node][element]
+
+def ensemble_mean(results):
+    """ensemble_mean (analyze_result.h:289-315): per position, sum over runs in run order of value/num_runs."""
+    n = len(results)
+    acc = np.zeros_like(results[0])
+    for r in range(n):
+        acc = acc + results[r] / float(n)
+    return acc
+
+
+def ensemble_percentile(results, p):
+    """ensemble_percentile (analyze_result.h:316-343): sorted run values at index size_t(num_runs * p)."""
+    n = len(results)
+    s = np.sort(np.stack(results), axis=0)
+    return s[int(float(n) * p)]
+
+
+def sum_nodes(results):
+    """sum_nodes (analyze_result.h:267-287): per run and day, nodes summed in node order: [run][day][element]."""
+    out = []
+    for res in results:
+        acc = np.zeros((res.shape[0], res.shape[2]))
+        for v in range(res.shape[1]):
+            acc = acc + res[:, v, :]
+        out.append(acc)
+    return np.stack(out)

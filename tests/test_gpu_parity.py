@@ -270,3 +270,26 @@ def test_tiling_mixed_batch_and_autotune(oracle):
         assert np.array_equal(a, b)
     r, _ = oracle.graph_run(extra[1], cfgs[-1], 7.0)
     assert rel_err(out[-1], r) < 1e-12
+
+
+def test_ensemble_statistics_on_device(oracle):
+    """ensemble_mean, ensemble_percentile and sum_nodes computed on the device from the resident daily
+    outputs equal the oracle's restatement on the same outputs bitwise (37 runs: not a power of two)."""
+    specs = _shared_ensemble(37, 15, 3, seed=61)
+    cfgs = [synthetic.ck_cfg(1e-8)] * len(specs)
+    tmaxs = [10.0] * len(specs)
+    ps = [0.05, 0.25, 0.5, 0.75, 0.95]
+    with Context() as c:
+        b = Batch(c, specs, cfgs, tmaxs)
+        b.run()
+        daily = b.read_daily()
+        mean = b.ensemble_mean()
+        pct = b.ensemble_percentiles(ps)
+        sums = b.sum_nodes()
+        part = b.ensemble_mean(first=5, count=11)
+        b.close()
+    assert np.array_equal(mean, oracle.ensemble_mean(daily))
+    for k, p in enumerate(ps):
+        assert np.array_equal(pct[k], oracle.ensemble_percentile(daily, p)), p
+    assert np.array_equal(sums, oracle.sum_nodes(daily))
+    assert np.array_equal(part, oracle.ensemble_mean(daily[5:16]))

@@ -132,3 +132,29 @@ def test_fixup_scenario_triggers(oracle):
         _, st = oracle.graph_run(spec, cfg, 20.0)
         assert st.negative_fixups > 0
         assert st.returns > 0
+
+
+def test_ensemble_analysis_known_answers(oracle):
+    """The oracle's ensemble_mean / ensemble_percentile / sum_nodes against the known answers of the
+    reference's TestEnsembleMean, TestEnsemblePercentile and TestSumNodes (cpp/tests/test_analyze_result.cpp:
+    220-380), arrays laid out [run][time][node][element]."""
+    # sum_nodes: 2 runs, 3 nodes, 3 times, 1 element
+    r1 = np.array([[0, 3, 6], [1, 4, 7], [2, 5, 8]], float)[:, :, None]
+    r2 = np.array([[0.5, 7.5, 1.5], [2.0, 9.5, 2.5], [5.0, 1.0, 3.0]])[:, :, None]
+    s = oracle.sum_nodes([r1, r2])
+    assert s.shape == (2, 3, 1)
+    assert np.allclose(s[0, :, 0], [9, 12, 15]) and np.allclose(s[1, :, 0], [9.5, 14, 9])
+    # ensemble_mean: 2 runs, 2 nodes, 3 times, 1 element
+    m1 = np.array([[0, 0], [1, 1], [2, 2]], float)[:, :, None]
+    m2 = np.array([[0.5, 1.5], [3.0, 0.5], [0.0, 1.0]])[:, :, None]
+    mean = oracle.ensemble_mean([m1, m2])
+    assert np.allclose(mean[:, 0, 0], [0.25, 2.0, 1.0]) and np.allclose(mean[:, 1, 0], [0.75, 0.75, 1.5])
+    # ensemble_percentile: 4 runs, 2 nodes, 3 times, 2 elements
+    runs = [np.zeros((3, 2, 2)) for _ in range(4)]
+    runs[0][0, 1, 0] = 0.2
+    runs[1][1, 0, 1], runs[1][0, 1, 0] = 1.0, 0.1
+    runs[2][1, 0, 1], runs[2][0, 1, 0] = 2.0, 0.3
+    runs[3][1, 0, 1] = 3.0
+    for p, a, b in ((0.2, 0.0, 0.0), (0.4, 1.0, 0.1), (0.7, 2.0, 0.2), (0.9, 3.0, 0.3)):
+        q = oracle.ensemble_percentile(runs, p)
+        assert q[1, 0, 1] == a and q[0, 1, 0] == b, p
