@@ -69,7 +69,7 @@ def ensemble_bench(args, ctx, specs, cfgs, tmaxs):
         out["graph_value"] = node_days / gs
         log(f"hipGraph-replayed run: {gs * 1e3:.1f} ms wall ({ginfo.device_ms:.1f} ms device) -> "
             f"{out['graph_value']:.3e} node-days/s")
-    ctx.set_flags(capi.CTX_PROFILE_KERNELS)
+    ctx.set_flags(capi.CTX_PROFILE_KERNELS | args.base_flags)
     for w in range(args.warmup):
         batch.reset()
         info = batch.run()
@@ -86,7 +86,7 @@ def ensemble_bench(args, ctx, specs, cfgs, tmaxs):
             f"{info.node_kernel_ms:.1f} ms / {info.node_launches} launches, edge kernels "
             f"{info.edge_kernel_ms:.1f} ms, steps {info.steps}, attempts {info.attempts}, fix-up passes "
             f"{info.fixup_passes}, replays {info.exact_replays}")
-    ctx.set_flags(0)
+    ctx.set_flags(args.base_flags)
     total = sum(times)
     out["value"] = node_days * args.steps / total
     out["ms_per_step"] = total / args.steps * 1e3
@@ -223,6 +223,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="skip the hipGraph-replayed comparison run")
     ap.add_argument("--tile", type=int, default=0, help="runs per node-major tile (0 = autotuned)")
+    ap.add_argument("--no-graph-capture", action="store_true",
+                    help="launch every kernel directly everywhere (no hipGraph capture; implies --no-graph)")
     ap.add_argument("--uneven", type=int, default=4096, help="graphs in the uneven batch (0 = skip)")
     ap.add_argument("--uneven-steps", type=int, default=2)
     ap.add_argument("--serve-rates", type=float, nargs="*", default=[50.0, 200.0, 800.0])
@@ -244,7 +246,11 @@ def main():
     tmaxs = [days] * S
     log(f"inputs generated in {time.perf_counter() - t:.1f}s, {base.num_edges} edges")
 
-    ctx = Context()   # one context (one stream) for every measurement of this process
+    if args.no_graph_capture:
+        args.no_graph = True
+    base_flags = capi.CTX_NO_GRAPH_CAPTURE if args.no_graph_capture else 0
+    args.base_flags = base_flags
+    ctx = Context(flags=base_flags)   # one context (one stream) for every measurement of this process
     ctx.set_tile(args.tile)
     ens = ensemble_bench(args, ctx, specs, cfgs, tmaxs)
     infos = ens["infos"]
@@ -257,6 +263,18 @@ def main():
     edge_ms = sum(i.edge_kernel_ms for i in infos) / max(sum(i.edge_launches for i in infos), 1)
     log(f"node kernel: {launches} launches, avg {node_ms:.3f} ms, {achieved:.2f} TFLOP/s; edge phase avg "
         f"{edge_ms:.3f} ms per cycle")
+
+    # whole-job roofline of the ensemble (per node-day: F_nd FLOPs, B_nd compulsory HBM bytes)
+    n_el = 10 * G
+    deg = base.num_edges / N
+    job_flops = (flops / max(len(infos), 1) + days * (S * base.num_edges * 131 * G + S * N * (2 * G * G + 17 * G + 7)))
+    job_bytes = S * N * days * (40 * n_el + (8 * n_el + 8) * deg / S)
+    t_roof = max(job_flops / (FP64_PEAK_TFLOPS * 1e12), job_bytes / (HBM_PEAK_GBS * 1e9))
+    job_roof = {"flops_per_step": job_flops, "hbm_bytes_per_step": job_bytes, "t_roof_ms": t_roof * 1e3,
+                "frac": t_roof / (ens["ms_per_step"] * 1e-3),
+                "bound": "FP64" if job_flops / (FP64_PEAK_TFLOPS * 1e12) > job_bytes / (HBM_PEAK_GBS * 1e9) else "HBM"}
+    log(f"job roofline: {job_flops:.3e} FLOP, {job_bytes:.3e} B per step, t_roof {t_roof * 1e3:.1f} ms "
+        f"({job_roof['bound']}), fraction {job_roof['frac']:.3f}")
 
     plain_value = None if args.no_plain else plain_bench(ctx, base, cfg, days)
     uneven = uneven_bench(args, ctx) if args.uneven > 0 else None
@@ -306,6 +324,7 @@ def main():
                      "kernel": f"node_phase<G={G},CashKarp54>", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
                      "avg_launch_ms": node_ms, "launches": launches, "flops_per_launch": flops_per_launch},
+        "job_roofline": job_roof,
         "uneven_batch": uneven,
         "serving": serving,
         "cpu_baseline": cpu,

@@ -568,64 +568,6 @@ struct Batch {
 
     int tile = 1; // runs per node-major tile (see Tiling in mgo_internal.h)
 
-    // one launch table over all classes of an integrator kind that has classes of several G
-    struct KindLaunch {
-        std::vector<int> cls;
-        bool any_edges = false;
-        DevBuf segs, block_seg, block_first, inst_ids;
-        LaunchDev launch{};
-    };
-    std::deque<KindLaunch> mixed;
-    std::vector<int> class_mixed; // class -> index into mixed, or -1
-
-    void build_mixed_tables(std::vector<InstDev>& idev)
-    {
-        std::map<int, std::vector<int>> by_kind;
-        for (size_t ci = 0; ci < classes.size(); ++ci) {
-            by_kind[classes[ci].kind].push_back(int(ci));
-        }
-        class_mixed.assign(classes.size(), -1);
-        for (auto& kv : by_kind) {
-            if (kv.second.size() < 2) {
-                continue;
-            }
-            mixed.emplace_back();
-            KindLaunch& K = mixed.back();
-            K.cls         = kv.second;
-            std::vector<SegDev> segs;
-            std::vector<int> nodes, gs;
-            std::vector<int32_t> ids;
-            for (size_t k = 0; k < K.cls.size(); ++k) {
-                ClassHost& C = classes[size_t(K.cls[k])];
-                class_mixed[size_t(K.cls[k])] = int(mixed.size()) - 1;
-                segs.push_back(C.seg_host);
-                nodes.push_back(C.num_nodes);
-                gs.push_back(C.G);
-                for (int ii : C.insts) {
-                    ids.push_back(inst_offset + ii);
-                    idev[size_t(ii)].seg = int32_t(k);
-                    K.any_edges          = K.any_edges || inst[size_t(ii)].has_edges;
-                }
-            }
-            std::vector<int32_t> bseg, bfirst;
-            block_tables_mixed(nodes, gs, bseg, bfirst);
-            K.segs.upload(segs, stream);
-            K.block_seg.upload(bseg, stream);
-            K.block_first.upload(bfirst, stream);
-            K.inst_ids.upload(ids, stream);
-            device_bytes += K.segs.bytes + K.block_seg.bytes + K.block_first.bytes + K.inst_ids.bytes;
-            LaunchDev& ld  = K.launch;
-            ld.segs        = K.segs.as<SegDev>();
-            ld.block_seg   = K.block_seg.as<int32_t>();
-            ld.block_first = K.block_first.as<int32_t>();
-            ld.inst_ids    = K.inst_ids.as<int32_t>();
-            ld.num_inst    = int(ids.size());
-            ld.num_blocks  = int(bseg.size());
-            ld.G           = 0;
-            ld.kind        = kv.first;
-        }
-    }
-
     void build(std::vector<RunSpec>& runs, hipStream_t s, int tile_runs = 1)
     {
         stream = s;
@@ -662,7 +604,6 @@ struct Batch {
         for (size_t ci = 0; ci < classes.size(); ++ci) {
             build_class(ci, runs, idev);
         }
-        build_mixed_tables(idev);
         inst_dev.upload(idev, stream);
         device_bytes += inst_dev.bytes;
         hip_check(hipStreamSynchronize(stream), "batch upload");
@@ -982,18 +923,13 @@ struct Batch {
         ld.kind        = C.kind;
     }
 
-    // enqueue one half-day cycle for every instance; with ev != nullptr, events bracket the node and the
-    // edge launches (ev[0..1] node, ev[2..3] edge)
-    // launch tables of a cycle: every class on its own, except classes covered by a mixed table
+    // launch tables of a cycle: one per class. (Batches keep per-G kernels so that a run's arithmetic is
+    // the same compiled code alone and in any batch; mixed tables serve the request loop.)
     void launch_tables(std::vector<const LaunchDev*>& node_ld, std::vector<const LaunchDev*>& edge_ld) const
     {
         node_ld.clear();
         edge_ld.clear();
-        for (size_t ci = 0; ci < classes.size(); ++ci) {
-            if (!class_mixed.empty() && class_mixed[ci] >= 0) {
-                continue;
-            }
-            const ClassHost& C = classes[ci];
+        for (const ClassHost& C : classes) {
             node_ld.push_back(&C.launch);
             bool any_edges = false;
             for (int ii : C.insts) {
@@ -1001,12 +937,6 @@ struct Batch {
             }
             if (any_edges) {
                 edge_ld.push_back(&C.launch);
-            }
-        }
-        for (const KindLaunch& K : mixed) {
-            node_ld.push_back(&K.launch);
-            if (K.any_edges) {
-                edge_ld.push_back(&K.launch);
             }
         }
     }
