@@ -194,15 +194,20 @@ def serving_bench(args, ctx):
         srv.stop()
         bad = [tk for tk in tickets if infos[tk].status != capi.OK]
         lat = np.array([infos[tk].done_ms - infos[tk].submit_ms for tk in tickets])
+        wait = np.array([infos[tk].admit_ms - infos[tk].submit_ms for tk in tickets])
+        service = np.array([infos[tk].done_ms - infos[tk].admit_ms for tk in tickets])
         first = min(infos[tk].submit_ms for tk in tickets)
         last = max(infos[tk].done_ms for tk in tickets)
         nd = float(sum(meta.values()))
         res = {"rate_rps": rate, "requests": len(tickets), "updates": updates, "failed": len(bad),
                "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+               "p50_admission_wait_ms": float(np.percentile(wait, 50)),
+               "p50_resident_ms": float(np.percentile(service, 50)),
                "node_days": nd, "node_days_per_s": nd / ((last - first) * 1e-3),
                "wall_s": t_end - t_start}
         log(f"serve {rate} req/s: {len(tickets)} solves + {updates} updates, p50 {res['p50_ms']:.1f} ms, "
-            f"p99 {res['p99_ms']:.1f} ms, {res['node_days_per_s']:.3e} node-days/s, failed {len(bad)}")
+            f"p99 {res['p99_ms']:.1f} ms (p50 wait for admission {res['p50_admission_wait_ms']:.1f} ms, resident "
+            f"{res['p50_resident_ms']:.1f} ms), {res['node_days_per_s']:.3e} node-days/s, failed {len(bad)}")
         results.append(res)
     return results
 
@@ -299,6 +304,16 @@ def main():
                       for g, r in zip(ens["sample"], ref))
         log(f"cpu baseline: {cpu['value']:.3e} node-days/s on {args.cpu_threads} threads; max rel err {max_rel:.2e}")
 
+    # HBM bytes per node-kernel launch of this workload, measured by rocprofv3 PMC passes (FETCH_SIZE,
+    # WRITE_SIZE) and scaled by the counters' reading for a copy of known size (profiles/)
+    traffic, traffic_src = None, None
+    tf = os.path.join(ROOT, "profiles", "node_phase_ensemble1024x400_g6_traffic.json")
+    if os.path.exists(tf) and (S, N, G, days) == (1024, 400, 6, 100.0):
+        with open(tf) as f:
+            t = json.load(f)
+        traffic = t["fetch_bytes_per_launch"] + t["write_bytes_per_launch"]
+        traffic_src = f"profiles/{os.path.basename(tf)} ({t['command']})"
+
     value = ens["value"]
     res = {
         "metric": "node-days/s batched+served vs plain GPU at <=1e-6 rel err; p50/p99 latency",
@@ -322,8 +337,9 @@ def main():
         "max_rel_err_vs_oracle": max_rel,
         "roofline": {"bound": "mfma", "roof": "FP64 vector ALU (78.6 TFLOP/s = FP64 peak of MI355X)",
                      "kernel": f"node_phase<G={G},CashKarp54>", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                     "avg_launch_ms": node_ms, "launches": launches, "flops_per_launch": flops_per_launch},
+                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
+                     "traffic_source": traffic_src, "avg_launch_ms": node_ms, "launches": launches,
+                     "flops_per_launch": flops_per_launch},
         "job_roofline": job_roof,
         "uneven_batch": uneven,
         "serving": serving,

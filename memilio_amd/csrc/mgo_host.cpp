@@ -285,6 +285,8 @@ struct ClassHost {
     DevBuf y, snap, ypre, rc, lcoef, cbase, cdelta, nodep, dtime, dmat, tnode, dtnode, steps, attempts, out,
         ipool, epool, cvec, estructs, inst_ids;
     DevBuf fx_rec, fx_pool, fx_dirty;  // fix-up records of the return passes
+    DevBuf node_order;                 // node-kernel lane slot -> node (empty: identity)
+    std::vector<int32_t> order;        // host copy, built by place_nodes
     DevBuf y0;                         // initial state, for reset
     std::vector<double> tnode0, dtnode0; // initial clocks, for reset
     size_t out_doubles = 0;
@@ -627,7 +629,9 @@ struct Batch {
             }
             it->second.push_back(ii);
         }
+        bool shared = false;
         for (auto* grp : order) {
+            shared = shared || grp->size() > 1;
             for (size_t first = 0; first < grp->size(); first += size_t(tile)) {
                 const int k = int(std::min(grp->size() - first, size_t(tile)));
                 const int n = inst[size_t((*grp)[first])].num_nodes;
@@ -637,6 +641,19 @@ struct Batch {
                     I.node_stride = k;
                 }
                 C.num_nodes += n * k;
+            }
+        }
+        // node-kernel order: each graph node of a group of runs sharing a graph over all its runs
+        C.order.clear();
+        if (shared) {
+            for (auto* grp : order) {
+                const int n = inst[size_t((*grp)[0])].num_nodes;
+                for (int v = 0; v < n; ++v) {
+                    for (int ii : *grp) {
+                        const InstHost& I = inst[size_t(ii)];
+                        C.order.push_back(int32_t(I.node_base + v * I.node_stride));
+                    }
+                }
             }
         }
     }
@@ -902,6 +919,11 @@ struct Batch {
         d.steps     = C.steps.as<uint32_t>();
         d.attempts  = C.attempts.as<uint32_t>();
         d.estructs  = C.estructs.as<EdgeStructDev>();
+        if (!C.order.empty()) {
+            C.node_order.upload(C.order, stream);
+            device_bytes += C.node_order.bytes;
+        }
+        d.node_order = C.order.empty() ? nullptr : C.node_order.as<int32_t>();
         d.fx_rec    = C.fx_rec.as<FxRec>();
         d.fx_pool   = C.fx_pool.as<double>();
         d.fx_dirty  = C.fx_dirty.as<int32_t>();
@@ -1550,7 +1572,7 @@ static int autotune_tile(mgo_context* ctx, const std::vector<RunSpec>& runs)
     }
     int best_tile    = 1;
     double best_ms   = INFINITY;
-    const int cands[] = {1, 4, 16, 64};
+    const int cands[] = {1, 2, 4, 8, 16, 64};
     for (int c : cands) {
         if (c > bucket) {
             break;

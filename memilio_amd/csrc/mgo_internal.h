@@ -23,7 +23,10 @@
 // tile sits at segment node tile_base + v * T + j (node_base = tile_base + j, node_stride = T). A
 // wavefront then holds the same graph node of 64/G consecutive runs: similar parameters (similar step
 // counts under adaptive control) and identical edge lists (uniform edge walks, adjacent gathers).
-// T = 1 is the run-major layout. Results do not depend on T.
+// T = 1 is the run-major layout. Results do not depend on T. The node kernel (compute-bound, where step-
+// count divergence matters) visits nodes through SegDev::node_order, which puts one graph node of all runs
+// sharing the graph into consecutive lanes whatever T is; the edge kernels (gather-bound, where the
+// working set of the gathered rows matters) follow the memory layout, so T is tuned for them.
 #pragma once
 
 #include <cstdint>
@@ -202,6 +205,8 @@ struct SegDev {
     uint32_t* steps;
     uint32_t* attempts;
     const EdgeStructDev* estructs; // indexed by InstDev::estruct
+    const int32_t* node_order;     // node-kernel lane slot -> segment node, or NULL (identity); groups the
+                                   // same graph node of the runs sharing a graph whatever the tile size
     FxRec* fx_rec;                 // [in-edge][2] tagged fix-up record versions
     double* fx_pool;               // fixed traveler vectors [slot][G][10]
     int32_t* fx_dirty;             // [2][node] tag of the pass that changed a record of the node's out-edges

@@ -144,6 +144,35 @@ __device__ __forceinline__ double season_at(const Lane<G>& L, double t)
     return 1.0 + L.np.seasonality * sin(MGO_PI * ((L.np.start_day + t) / 182.5 + 0.5));
 }
 
+// Seasonal factors of the five later stages of a step, spread over the node's G lanes: lane a evaluates
+// stages j = a, a + G, ... (j < 5) at t + c_j h, and every lane reads a stage's factor from the lane that
+// owns it (one sin per lane and step attempt for G >= 5 instead of five).
+template <int G>
+struct StageSeasons {
+    static constexpr int N = (5 + G - 1) / G;
+    double v[N];
+};
+
+template <int G>
+__device__ __forceinline__ StageSeasons<G> stage_seasons(const Lane<G>& L, double t, double h, double c2, double c3,
+                                                         double c4, double c5, double c6)
+{
+    StageSeasons<G> s;
+#pragma unroll
+    for (int q = 0; q < StageSeasons<G>::N; ++q) {
+        const int j    = L.age + q * G;
+        const double c = (j == 0) ? c2 : (j == 1) ? c3 : (j == 2) ? c4 : (j == 3) ? c5 : c6;
+        s.v[q]         = (j < 5) ? season_at<G>(L, t + c * h) : 1.0;
+    }
+    return s;
+}
+
+template <int G, int J>
+__device__ __forceinline__ double stage_season(const Lane<G>& L, const StageSeasons<G>& s)
+{
+    return __shfl(s.v[J / G], L.base + J % G);
+}
+
 // Force-of-infection coefficient of the lane's age group: season * tp * sum_j C_gj(t) w_j(pop),
 // w_j = divN_j * (rel_j pop[INS_j] + risk_j pop[ISy_j]) (model.h:99-145, factored over j). Also returns
 // the ICU-adjusted ISev->ICr / ISev->D coefficients (model.h:147-152, 196-204).
@@ -244,34 +273,35 @@ __device__ __forceinline__ void cash_karp_stages(const SegDev& c, Lane<G>& L, co
                                                  const double (&k1)[NC], double t, double h, double (&yo)[NC],
                                                  double (&er)[NC], bool want_err)
 {
+    const StageSeasons<G> ss = stage_seasons<G>(L, t, h, ck::C2, ck::C3, ck::C4, ck::C5, ck::C6);
     double k2[NC], k3[NC], k4[NC], k5[NC], yt[NC];
 #pragma unroll
     for (int i = 0; i < NC; ++i) {
         yt[i] = y[i] + (ck::A21 * h) * k1[i];
     }
-    rhs<G>(c, L, yt, t + ck::C2 * h, season_at<G>(L, t + ck::C2 * h), k2);
+    rhs<G>(c, L, yt, t + ck::C2 * h, stage_season<G, 0>(L, ss), k2);
 #pragma unroll
     for (int i = 0; i < NC; ++i) {
         yt[i] = y[i] + (ck::A31 * h) * k1[i] + (ck::A32 * h) * k2[i];
     }
-    rhs<G>(c, L, yt, t + ck::C3 * h, season_at<G>(L, t + ck::C3 * h), k3);
+    rhs<G>(c, L, yt, t + ck::C3 * h, stage_season<G, 1>(L, ss), k3);
 #pragma unroll
     for (int i = 0; i < NC; ++i) {
         yt[i] = y[i] + (ck::A41 * h) * k1[i] + (ck::A42 * h) * k2[i] + (ck::A43 * h) * k3[i];
     }
-    rhs<G>(c, L, yt, t + ck::C4 * h, season_at<G>(L, t + ck::C4 * h), k4);
+    rhs<G>(c, L, yt, t + ck::C4 * h, stage_season<G, 2>(L, ss), k4);
 #pragma unroll
     for (int i = 0; i < NC; ++i) {
         yt[i] = y[i] + (ck::A51 * h) * k1[i] + (ck::A52 * h) * k2[i] + (ck::A53 * h) * k3[i] + (ck::A54 * h) * k4[i];
     }
-    rhs<G>(c, L, yt, t + ck::C5 * h, season_at<G>(L, t + ck::C5 * h), k5);
+    rhs<G>(c, L, yt, t + ck::C5 * h, stage_season<G, 3>(L, ss), k5);
 #pragma unroll
     for (int i = 0; i < NC; ++i) {
         yt[i] = y[i] + (ck::A61 * h) * k1[i] + (ck::A62 * h) * k2[i] + (ck::A63 * h) * k3[i] + (ck::A64 * h) * k4[i] +
                 (ck::A65 * h) * k5[i];
     }
     double k6[NC];
-    rhs<G>(c, L, yt, t + ck::C6 * h, season_at<G>(L, t + ck::C6 * h), k6);
+    rhs<G>(c, L, yt, t + ck::C6 * h, stage_season<G, 4>(L, ss), k6);
 #pragma unroll
     for (int i = 0; i < NC; ++i) {
         yo[i] = y[i] + (ck::B1 * h) * k1[i] + (ck::B3 * h) * k3[i] + (ck::B4 * h) * k4[i] + (ck::B6 * h) * k6[i];
@@ -305,33 +335,34 @@ __device__ __forceinline__ void rkf45_stages(const SegDev& c, Lane<G>& L, const 
                                              const double (&k1)[NC], double t, double h, double (&yo)[NC],
                                              double (&er)[NC])
 {
+    const StageSeasons<G> ss = stage_seasons<G>(L, t, h, rkf::C2, rkf::C3, rkf::C4, rkf::C5, rkf::C6);
     double k2[NC], k3[NC], k4[NC], k5[NC], k6[NC], yt[NC];
 #pragma unroll
     for (int i = 0; i < NC; ++i) {
         yt[i] = y[i] + (h * rkf::A21) * k1[i];
     }
-    rhs<G>(c, L, yt, t + rkf::C2 * h, season_at<G>(L, t + rkf::C2 * h), k2);
+    rhs<G>(c, L, yt, t + rkf::C2 * h, stage_season<G, 0>(L, ss), k2);
 #pragma unroll
     for (int i = 0; i < NC; ++i) {
         yt[i] = y[i] + (h * rkf::A31) * k1[i] + (h * rkf::A32) * k2[i];
     }
-    rhs<G>(c, L, yt, t + rkf::C3 * h, season_at<G>(L, t + rkf::C3 * h), k3);
+    rhs<G>(c, L, yt, t + rkf::C3 * h, stage_season<G, 1>(L, ss), k3);
 #pragma unroll
     for (int i = 0; i < NC; ++i) {
         yt[i] = y[i] + (h * rkf::A41) * k1[i] + (h * rkf::A42) * k2[i] + (h * rkf::A43) * k3[i];
     }
-    rhs<G>(c, L, yt, t + rkf::C4 * h, season_at<G>(L, t + rkf::C4 * h), k4);
+    rhs<G>(c, L, yt, t + rkf::C4 * h, stage_season<G, 2>(L, ss), k4);
 #pragma unroll
     for (int i = 0; i < NC; ++i) {
         yt[i] = y[i] + (h * rkf::A51) * k1[i] + (h * rkf::A52) * k2[i] + (h * rkf::A53) * k3[i] + (h * rkf::A54) * k4[i];
     }
-    rhs<G>(c, L, yt, t + rkf::C5 * h, season_at<G>(L, t + rkf::C5 * h), k5);
+    rhs<G>(c, L, yt, t + rkf::C5 * h, stage_season<G, 3>(L, ss), k5);
 #pragma unroll
     for (int i = 0; i < NC; ++i) {
         yt[i] = y[i] + (h * rkf::A61) * k1[i] + (h * rkf::A62) * k2[i] + (h * rkf::A63) * k3[i] +
                 (h * rkf::A64) * k4[i] + (h * rkf::A65) * k5[i];
     }
-    rhs<G>(c, L, yt, t + rkf::C6 * h, season_at<G>(L, t + rkf::C6 * h), k6);
+    rhs<G>(c, L, yt, t + rkf::C6 * h, stage_season<G, 4>(L, ss), k6);
 #pragma unroll
     for (int i = 0; i < NC; ++i) {
         // y + dt * (K * b_low) and dt * |K * (b_high - b_low)| (adapt_rk.h:268-271)
@@ -363,6 +394,9 @@ __device__ __forceinline__ const SegDev* load_lane(const LaunchDev& ld, Lane<G>&
     L.base          = grp * G; // lane index (within the wavefront) of the group's first lane
     if (grp >= NPW || L.node >= c->num_nodes) {
         return nullptr;
+    }
+    if (c->node_order) {
+        L.node = c->node_order[L.node]; // same graph node of consecutive runs in one wavefront
     }
     L.np            = c->nodep[L.node];
     const double* p = c->lcoef + (size_t(L.node) * G + L.age) * NUM_LC;
