@@ -158,7 +158,7 @@ def serving_bench(args, ctx):
         is_update = rng.random(n_arr) < 0.1
         picks = rng.integers(0, len(pool), n_arr)
         scales = rng.uniform(0.3, 1.0, n_arr)
-        srv = Server(ctx)
+        srv = Server(ctx, flags=capi.SERVE_BATCH_KERNELS if args.serve_batch_kernels else 0)
         live, tickets, updates = [], [], 0
         meta = {}
         t_start = time.perf_counter()
@@ -195,7 +195,8 @@ def serving_bench(args, ctx):
         bad = [tk for tk in tickets if infos[tk].status != capi.OK]
         lat = np.array([infos[tk].done_ms - infos[tk].submit_ms for tk in tickets])
         wait = np.array([infos[tk].admit_ms - infos[tk].submit_ms for tk in tickets])
-        service = np.array([infos[tk].done_ms - infos[tk].admit_ms for tk in tickets])
+        service = np.array([infos[tk].retire_ms - infos[tk].admit_ms for tk in tickets])
+        copy = np.array([infos[tk].done_ms - infos[tk].retire_ms for tk in tickets])
         first = min(infos[tk].submit_ms for tk in tickets)
         last = max(infos[tk].done_ms for tk in tickets)
         nd = float(sum(meta.values()))
@@ -203,11 +204,13 @@ def serving_bench(args, ctx):
                "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
                "p50_admission_wait_ms": float(np.percentile(wait, 50)),
                "p50_resident_ms": float(np.percentile(service, 50)),
+               "p50_result_copy_ms": float(np.percentile(copy, 50)),
                "node_days": nd, "node_days_per_s": nd / ((last - first) * 1e-3),
                "wall_s": t_end - t_start}
         log(f"serve {rate} req/s: {len(tickets)} solves + {updates} updates, p50 {res['p50_ms']:.1f} ms, "
             f"p99 {res['p99_ms']:.1f} ms (p50 wait for admission {res['p50_admission_wait_ms']:.1f} ms, resident "
-            f"{res['p50_resident_ms']:.1f} ms), {res['node_days_per_s']:.3e} node-days/s, failed {len(bad)}")
+            f"{res['p50_resident_ms']:.1f} ms, result copy {res['p50_result_copy_ms']:.1f} ms), "
+            f"{res['node_days_per_s']:.3e} node-days/s, failed {len(bad)}")
         results.append(res)
     return results
 
@@ -235,6 +238,8 @@ def main():
     ap.add_argument("--serve-rates", type=float, nargs="*", default=[50.0, 200.0, 800.0])
     ap.add_argument("--serve-seconds", type=float, default=4.0)
     ap.add_argument("--serve-pool", type=int, default=384)
+    ap.add_argument("--serve-batch-kernels", action="store_true",
+                    help="serve with the per-(integrator, G) batch kernels (MGO_SERVE_BATCH_KERNELS)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: every run of this build uses "

@@ -277,9 +277,20 @@ typedef struct mgo_request {
 } mgo_request;
 
 typedef struct mgo_serve_cfg {
-    int32_t max_resident;        /* upper bound on concurrently resident requests (0 = unlimited) */
-    size_t hbm_budget_bytes;     /* 0 = the context budget */
+    int32_t max_resident;        /* upper bound on concurrently resident requests (0 = 4096) */
+    size_t hbm_budget_bytes;     /* 0 = the context budget, at most 32 GiB */
+    int32_t flags;               /* MGO_SERVE_* */
+    size_t result_pool_bytes;    /* pinned host memory holding finished requests' daily values until
+                                    mgo_serve_take (0 = 2 GiB); a result that does not fit is staged in
+                                    pageable memory instead (slower copy) */
 } mgo_serve_cfg;
+
+/* Serving launches one set of kernels per integrator kind over the resident requests of every age-group count
+ * (blocks dispatch on their segment's G). With MGO_SERVE_BATCH_KERNELS it launches one set per (integrator,
+ * G) instead: the very kernels a batch or graph run uses, so a served request's results are bitwise equal to
+ * those runs; more launches per simulated day (higher latency at low load). Either way a request's result
+ * does not depend on the other requests. */
+#define MGO_SERVE_BATCH_KERNELS 1
 
 /* Scenario update on a resident request: contact-matrix scaling s applied from time t_start as a damping
  * of coefficient 1-s on every node (ContactMatrix::add_damping semantics, damping.h:381-396,616-658). If
@@ -296,7 +307,8 @@ typedef struct mgo_result_info {
     mgo_status status;          /* MGO_OK or the error that ended the request */
     double submit_ms;           /* context clock at submission */
     double admit_ms;            /* context clock at admission to the device */
-    double done_ms;             /* context clock when the result was complete */
+    double retire_ms;           /* context clock when its last day was computed and its slot freed */
+    double done_ms;             /* context clock when its daily values were in host memory */
     int32_t num_nodes;
     int32_t num_groups;
     int32_t num_days;           /* daily values available */

@@ -61,6 +61,7 @@ INTEGRATOR_EULER = 3
 CTX_NO_GRAPH_CAPTURE = 1
 CTX_PROFILE_KERNELS = 2
 CTX_SEQUENTIAL_FIXUP = 4
+SERVE_BATCH_KERNELS = 1  # mgo_serve_cfg.flags: per-(integrator, G) launches, bitwise equal to batch runs
 
 DBL_MAX = np.finfo(np.float64).max
 DBL_MIN = np.finfo(np.float64).tiny
@@ -116,6 +117,7 @@ class DeviceCfg(C.Structure):
         ("device", C.c_int32),
         ("hbm_budget_bytes", C.c_size_t),
         ("flags", C.c_int32),
+        ("result_pool_bytes", C.c_size_t),
     ]
 
 
@@ -158,6 +160,8 @@ class ServeCfg(C.Structure):
     _fields_ = [
         ("max_resident", C.c_int32),
         ("hbm_budget_bytes", C.c_size_t),
+        ("flags", C.c_int32),
+        ("result_pool_bytes", C.c_size_t),
     ]
 
 
@@ -176,6 +180,7 @@ class ResultInfo(C.Structure):
         ("status", C.c_int32),
         ("submit_ms", C.c_double),
         ("admit_ms", C.c_double),
+        ("retire_ms", C.c_double),
         ("done_ms", C.c_double),
         ("num_nodes", C.c_int32),
         ("num_groups", C.c_int32),

@@ -31,6 +31,8 @@
 namespace mgo
 {
 hipError_t launch_clock(InstDev* inst, int n, hipStream_t s);
+hipError_t launch_scatter_slots(InstDev* table, const InstDev* src, const int32_t* ids, int n, hipStream_t s);
+hipError_t launch_set_segments(InstDev* table, const int32_t* ids, int n, hipStream_t s);
 hipError_t launch_node(const LaunchDev& c, InstDev* inst, hipStream_t s);
 hipError_t launch_edge(const LaunchDev& c, InstDev* inst, hipStream_t s);
 hipError_t launch_ensemble_mean(const double* out, const int64_t* base, int runs, int64_t positions, double* mean,
@@ -71,18 +73,25 @@ void hip_check(hipError_t e, const char* what)
     }
 }
 
-// First-fit sub-allocator over one device allocation (serving: per-request buffers without hipMalloc).
-// first-fit device memory arena (serving); alloc / free are thread-safe
+// First-fit sub-allocator over one device allocation (serving: per-request buffers without hipMalloc) or
+// one pinned host allocation (serving: result staging); alloc / free are thread-safe.
 struct Arena {
     char* base  = nullptr;
     size_t size = 0;
     size_t used = 0;
+    bool host   = false;
     std::map<size_t, size_t> free_; // offset -> length
     std::mutex mu;
-    void init(size_t bytes)
+    void init(size_t bytes, bool pinned_host = false)
     {
         size = bytes;
-        hip_check(hipMalloc(reinterpret_cast<void**>(&base), size), "serving arena");
+        host = pinned_host;
+        if (host) {
+            hip_check(hipHostMalloc(reinterpret_cast<void**>(&base), size, hipHostMallocDefault), "pinned pool");
+        }
+        else {
+            hip_check(hipMalloc(reinterpret_cast<void**>(&base), size), "serving arena");
+        }
         free_.clear();
         free_[0] = size;
         used     = 0;
@@ -90,11 +99,17 @@ struct Arena {
     void release()
     {
         if (base) {
-            (void)hipFree(base);
+            if (host) {
+                (void)hipHostFree(base);
+            }
+            else {
+                (void)hipFree(base);
+            }
         }
         base = nullptr;
     }
-    void* alloc(size_t bytes)
+    // nullptr when no free range is large enough
+    void* try_alloc(size_t bytes)
     {
         std::lock_guard<std::mutex> lk(mu);
         bytes = (bytes + 255) & ~size_t(255);
@@ -108,6 +123,13 @@ struct Arena {
                 used += bytes;
                 return base + off;
             }
+        }
+        return nullptr;
+    }
+    void* alloc(size_t bytes)
+    {
+        if (void* p = try_alloc(bytes)) {
+            return p;
         }
         fail(MGO_ERR_OUT_OF_MEMORY, "serving arena exhausted (%zu of %zu bytes used)", used, size);
     }
@@ -1155,7 +1177,8 @@ struct Batch {
         return std::max(0, last - I.day0 + 1);
     }
 
-    void read_daily(size_t i, int ndays, double* out)
+    // sync = false leaves the copy in flight on `stream` (the caller synchronises before using `out`)
+    void read_daily(size_t i, int ndays, double* out, bool sync = true)
     {
         const InstHost& I = inst[i];
         const ClassHost& C = classes[size_t(I.cls)];
@@ -1166,7 +1189,9 @@ struct Batch {
         hip_check(hipMemcpyAsync(out, C.out.as<double>() + I.out_base, nd * sizeof(double), hipMemcpyDeviceToHost,
                                  stream),
                   "read daily");
-        hip_check(hipStreamSynchronize(stream), "read daily");
+        if (sync) {
+            hip_check(hipStreamSynchronize(stream), "read daily");
+        }
     }
 
     // per-node values of instance i from a segment array with `width` values per node

@@ -1340,6 +1340,51 @@ hipError_t launch_clock(InstDev* inst, int n, hipStream_t s)
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------------------------- instance slots
+// Serving keeps every resident request's instance record in one table of slots. Admission and retirement
+// write whole records (table[ids[k]] = src[k]), class rebuilds renumber the segments (table[ids[k]].seg = k);
+// one launch each per loop iteration instead of one copy per request.
+static_assert(sizeof(InstDev) % 8 == 0, "instance records are copied as 8-byte words");
+
+__global__ void scatter_slots_kernel(InstDev* __restrict__ table, const InstDev* __restrict__ src,
+                                     const int32_t* __restrict__ ids, int n)
+{
+    constexpr int W = int(sizeof(InstDev) / 8);
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= int64_t(n) * W) {
+        return;
+    }
+    const int k = int(i / W), w = int(i % W);
+    reinterpret_cast<uint64_t*>(table + ids[k])[w] = reinterpret_cast<const uint64_t*>(src + k)[w];
+}
+
+__global__ void set_segments_kernel(InstDev* __restrict__ table, const int32_t* __restrict__ ids, int n)
+{
+    const int k = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (k < n) {
+        table[ids[k]].seg = k;
+    }
+}
+
+hipError_t launch_scatter_slots(InstDev* table, const InstDev* src, const int32_t* ids, int n, hipStream_t s)
+{
+    if (n <= 0) {
+        return hipSuccess;
+    }
+    const int64_t words = int64_t(n) * int64_t(sizeof(InstDev) / 8);
+    hipLaunchKernelGGL(scatter_slots_kernel, dim3(unsigned((words + 255) / 256)), dim3(256), 0, s, table, src, ids, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_set_segments(InstDev* table, const int32_t* ids, int n, hipStream_t s)
+{
+    if (n <= 0) {
+        return hipSuccess;
+    }
+    hipLaunchKernelGGL(set_segments_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, table, ids, n);
+    return hipGetLastError();
+}
+
 static hipError_t launch_node_mixed(const LaunchDev& c, InstDev* inst, hipStream_t s)
 {
     const int nb = c.num_blocks;

@@ -198,9 +198,9 @@ class GraphSim:
 class Server:
     """Continuous-batching request loop of a context (mgo_serve_*)."""
 
-    def __init__(self, ctx, max_resident=0, hbm_budget_bytes=0):
+    def __init__(self, ctx, max_resident=0, hbm_budget_bytes=0, flags=0, result_pool_bytes=0):
         self.ctx = ctx
-        cfg = capi.ServeCfg(max_resident, hbm_budget_bytes)
+        cfg = capi.ServeCfg(max_resident, hbm_budget_bytes, flags, result_pool_bytes)
         ctx.check(ctx.lib.mgo_serve_start(ctx.ptr, C.byref(cfg)))
         self.requests = {}
         self.finished = {}   # polled, not yet taken: ticket -> ResultInfo

@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 from memilio_amd import capi, synthetic
-from memilio_amd.engine import Context, MgoError, Server
+from memilio_amd.engine import Batch, Context, MgoError, Server
 from tests import setups
 
 pytestmark = pytest.mark.gpu
@@ -61,6 +61,47 @@ def test_serve_alone_vs_mix(oracle):
         assert rel_err(mixed[i], ref) < tol, i
 
 
+def test_serve_equals_batch_bitwise():
+    """With MGO_SERVE_BATCH_KERNELS serving launches the per-(integrator, G) kernels a batch uses: a served
+    request's daily values equal the same request run in a device batch, bit for bit."""
+    reqs = _requests()
+    with Context() as ctx:
+        srv = Server(ctx, flags=capi.SERVE_BATCH_KERNELS)
+        tickets = [srv.submit(*r[:3]) for r in reqs]
+        infos = srv.wait(tickets)
+        served = [srv.take(tk) for tk in tickets]
+        srv.stop()
+        assert all(infos[tk].status == capi.OK for tk in tickets)
+        b = Batch(ctx, [r[0] for r in reqs], [r[1] for r in reqs], [r[2] for r in reqs])
+        b.run()
+        batched = b.read_daily()
+        b.close()
+    for i, (s, r) in enumerate(zip(served, batched)):
+        assert np.array_equal(s.reshape(r.shape), r), i
+
+
+def test_serve_batch_kernels_mix_and_small_pool(oracle):
+    """MGO_SERVE_BATCH_KERNELS: alone equals in a mix bitwise and every request meets the parity bars; a
+    result pool too small for some results stages those in pageable memory with the same values."""
+    reqs = _requests()
+    with Context() as ctx:
+        srv = Server(ctx, flags=capi.SERVE_BATCH_KERNELS, result_pool_bytes=256 << 10)
+        t = srv.submit(*reqs[4][:3])
+        srv.wait([t])
+        alone = srv.take(t)
+        tickets = {i: srv.submit(*reqs[i][:3]) for i in range(len(reqs))}
+        infos = srv.wait(tickets.values())
+        mixed = {i: srv.take(tk) for i, tk in tickets.items()}
+        srv.stop()
+        with pytest.raises(MgoError) as e:
+            Server(ctx, flags=8)
+        assert e.value.status == capi.ERR_INVALID_VALUE
+    assert all(infos[tk].status == capi.OK for tk in tickets.values())
+    assert np.array_equal(alone, mixed[4])
+    for i, (spec, cfg, tmax, tol) in enumerate(reqs):
+        assert rel_err(mixed[i], _oracle_daily(oracle, spec, cfg, tmax)) < tol, i
+
+
 def test_serve_queueing_and_budget(oracle):
     """max_resident = 2: requests queue and are admitted as others retire; results unchanged."""
     reqs = _requests()
@@ -71,9 +112,10 @@ def test_serve_queueing_and_budget(oracle):
         outs = [srv.take(tk) for tk in tickets]
         srv.stop()
     admits = sorted(infos[tk].admit_ms for tk in tickets)
-    dones = sorted(infos[tk].done_ms for tk in tickets)
+    retires = sorted(infos[tk].retire_ms for tk in tickets)
     # the third admission waits for the first retirement
-    assert admits[2] >= dones[0]
+    assert admits[2] >= retires[0]
+    assert all(infos[tk].done_ms >= infos[tk].retire_ms >= infos[tk].admit_ms for tk in tickets)
     for (spec, cfg, tmax, tol), out in zip(reqs, outs):
         assert rel_err(out, _oracle_daily(oracle, spec, cfg, tmax)) < tol
 
@@ -92,7 +134,8 @@ def test_serve_damping_update(oracle):
         for s, ts, lev, typ in ups:
             srv.update(tk, s, ts, lev, typ)
         infos = srv.wait([tb, tk], timeout_s=120.0)
-        assert infos[tk].admit_ms >= infos[tb].done_ms
+        assert infos[tk].admit_ms >= infos[tb].retire_ms   # the slot frees at the blocker's retirement
+        assert infos[tb].done_ms >= infos[tb].retire_ms
         srv.take(tb)
         got = srv.take(tk)
         srv.stop()
