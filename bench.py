@@ -21,6 +21,7 @@ import argparse
 import json
 import os
 import sys
+import threading
 import time
 
 import numpy as np
@@ -29,7 +30,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 from memilio_amd import capi, synthetic  # noqa: E402
-from memilio_amd.engine import Batch, Context, GraphSim, Server  # noqa: E402
+from memilio_amd.engine import Batch, Context, GraphSim, MgoError, Server  # noqa: E402
 
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
 HBM_PEAK_GBS = 8000.0
@@ -161,6 +162,33 @@ def serving_bench(args, ctx):
         srv = Server(ctx, flags=capi.SERVE_BATCH_KERNELS if args.serve_batch_kernels else 0)
         live, tickets, updates = [], [], 0
         meta = {}
+        # a client thread takes every result as it finishes (its pinned staging is recycled at once)
+        infos, submitted_all, n_solves = {}, threading.Event(), [0]
+
+        def client():
+            pending = []
+            while True:
+                got = srv.poll()
+                for r in got:
+                    infos[r.ticket] = r
+                    pending.append(r.ticket)
+                keep = []
+                for tk in pending:
+                    if tk not in srv.requests:   # submit() has not recorded it yet
+                        keep.append(tk)
+                        continue
+                    try:
+                        srv.take(tk)
+                    except MgoError:
+                        pass                     # a failed request: its status is in infos
+                pending = keep
+                if submitted_all.is_set() and len(infos) >= n_solves[0] and not pending:
+                    return
+                if not got:
+                    time.sleep(0.0005)
+
+        taker = threading.Thread(target=client, daemon=True)
+        taker.start()
         t_start = time.perf_counter()
         t_next = t_start
         for k in range(n_arr):
@@ -171,26 +199,27 @@ def serving_bench(args, ctx):
                     break
                 time.sleep(min(t_next - now, 0.002))
             if is_update[k]:
-                for r in srv.poll():
-                    srv.finished[r.ticket] = r
-                live = [tk for tk in live if tk not in srv.finished]
+                live = [tk for tk in live if tk not in infos]
                 if live:
                     tk = live[int(rng.integers(0, len(live)))]
                     try:
                         srv.update(tk, float(scales[k]))
                         updates += 1
                     except Exception:
-                        pass   # finished between poll and update
+                        pass   # finished between the check and the update
                 continue
             spec, horizon, tol = pool[picks[k]]
             tk = srv.submit(spec, synthetic.ck_cfg(tol), horizon)
             tickets.append(tk)
             live.append(tk)
             meta[tk] = spec.num_nodes * horizon
-        infos = srv.wait(tickets, timeout_s=600.0)
+        n_solves[0] = len(tickets)
+        submitted_all.set()
+        taker.join(timeout=600.0)
+        if taker.is_alive():
+            raise TimeoutError(f"serving at {rate} req/s: results not complete in 600 s")
         t_end = time.perf_counter()
-        for tk in tickets:
-            srv.take(tk)
+        st = srv.stats()
         srv.stop()
         bad = [tk for tk in tickets if infos[tk].status != capi.OK]
         lat = np.array([infos[tk].done_ms - infos[tk].submit_ms for tk in tickets])
@@ -206,11 +235,19 @@ def serving_bench(args, ctx):
                "p50_resident_ms": float(np.percentile(service, 50)),
                "p50_result_copy_ms": float(np.percentile(copy, 50)),
                "node_days": nd, "node_days_per_s": nd / ((last - first) * 1e-3),
+               "loop": {"iterations": st["iterations"], "cycles": st["cycles"],
+                        "max_resident": st["max_resident_seen"], "device_s": st["device_ms"] * 1e-3,
+                        "host_s": st["host_ms"] * 1e-3,
+                        "device_node_days_per_s": 0.5 * st["node_cycles"] / max(st["device_ms"] * 1e-3, 1e-9)},
                "wall_s": t_end - t_start}
         log(f"serve {rate} req/s: {len(tickets)} solves + {updates} updates, p50 {res['p50_ms']:.1f} ms, "
             f"p99 {res['p99_ms']:.1f} ms (p50 wait for admission {res['p50_admission_wait_ms']:.1f} ms, resident "
             f"{res['p50_resident_ms']:.1f} ms, result copy {res['p50_result_copy_ms']:.1f} ms), "
             f"{res['node_days_per_s']:.3e} node-days/s, failed {len(bad)}")
+        lp = res["loop"]
+        log(f"  loop: {lp['iterations']} iterations, {lp['cycles']} cycles, up to {lp['max_resident']} resident, "
+            f"device {lp['device_s']:.2f} s ({lp['device_node_days_per_s']:.3e} node-days/s while running), "
+            f"host {lp['host_s']:.2f} s")
         results.append(res)
     return results
 

@@ -323,6 +323,20 @@ mgo_status mgo_serve_poll(mgo_context* ctx, mgo_result_info* out, int32_t capaci
 mgo_status mgo_serve_take(mgo_context* ctx, mgo_ticket ticket, double* daily_out, size_t capacity_doubles);
 /* Waits until every submitted request is finished, then stops the worker. */
 mgo_status mgo_serve_stop(mgo_context* ctx);
+
+/* Counters of the serving loop since mgo_serve_start (monitoring; no counterpart in the reference). */
+typedef struct mgo_serve_stats {
+    int64_t iterations;        /* worker loop iterations (admission / retirement points) */
+    int64_t cycles;            /* half-day cycles launched */
+    int64_t admitted;
+    int64_t retired;
+    double node_cycles;        /* sum over cycles of the resident nodes advanced */
+    double device_ms;          /* worker time from an iteration's first launch to the end of its cycles */
+    double host_ms;            /* worker time in admission, updates, launch-table rebuilds and retirement */
+    int32_t max_resident_seen; /* most requests resident at once */
+    int32_t reserved;
+} mgo_serve_stats;
+mgo_status mgo_serve_get_stats(mgo_context* ctx, mgo_serve_stats* out);
 /* Milliseconds on the context clock (steady clock, zero at context creation). */
 double mgo_clock_ms(const mgo_context* ctx);
 

@@ -188,6 +188,20 @@ class ResultInfo(C.Structure):
     ]
 
 
+class ServeStats(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int64),
+        ("cycles", C.c_int64),
+        ("admitted", C.c_int64),
+        ("retired", C.c_int64),
+        ("node_cycles", C.c_double),
+        ("device_ms", C.c_double),
+        ("host_ms", C.c_double),
+        ("max_resident_seen", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
 # every symbol include/mgo.h declares, with its ctypes signature
 SIGNATURES = {
     "mgo_context_create": (C.c_int32, [C.POINTER(DeviceCfg), C.POINTER(C.c_void_p)]),
@@ -218,6 +232,7 @@ SIGNATURES = {
     "mgo_serve_submit": (C.c_int32, [C.c_void_p, C.POINTER(Request), C.POINTER(C.c_int64)]),
     "mgo_serve_update": (C.c_int32, [C.c_void_p, C.c_int64, C.POINTER(DampingUpdate)]),
     "mgo_serve_poll": (C.c_int32, [C.c_void_p, C.POINTER(ResultInfo), C.c_int32, _ip]),
+    "mgo_serve_get_stats": (C.c_int32, [C.c_void_p, C.POINTER(ServeStats)]),
     "mgo_serve_take": (C.c_int32, [C.c_void_p, C.c_int64, _dp, C.c_size_t]),
     "mgo_serve_stop": (C.c_int32, [C.c_void_p]),
     "mgo_clock_ms": (C.c_double, [C.c_void_p]),

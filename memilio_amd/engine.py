@@ -246,6 +246,12 @@ class Server:
                 time.sleep(0.0005)
         return {t: self.finished[t] for t in want}
 
+    def stats(self):
+        """Counters of the serving loop (mgo_serve_get_stats) as a dict."""
+        st = capi.ServeStats()
+        self.ctx.check(self.ctx.lib.mgo_serve_get_stats(self.ctx.ptr, C.byref(st)))
+        return {name: getattr(st, name) for name, _ in capi.ServeStats._fields_ if name != "reserved"}
+
     def stop(self):
         if self.running:
             self.ctx.check(self.ctx.lib.mgo_serve_stop(self.ctx.ptr))

@@ -265,7 +265,7 @@ def test_tiling_mixed_batch_and_autotune(oracle):
         ref, _ = c1.ensemble_run(specs, cfgs, tmaxs)
     with Context() as c2:
         out, info = c2.ensemble_run(specs, cfgs, tmaxs)
-    assert info.tile in (1, 4, 16)
+    assert info.tile in (1, 2, 4, 8, 16, 64)   # the autotuner's candidates
     for a, b in zip(ref, out):
         assert np.array_equal(a, b)
     r, _ = oracle.graph_run(extra[1], cfgs[-1], 7.0)
