@@ -98,6 +98,35 @@ enum {
  * duplicates (same start,end) keep the last one, as Graph::add_edge's insert_sorted_replace does
  * (graph.h:231-241).
  */
+/* Dynamic NPIs of the nodes' contact matrices (osecir DynamicNPIsInfectedSymptoms,
+ * cpp/models/ode_secir/parameters.h:342-355; mio::DynamicNPIs, cpp/memilio/epidemiology/dynamic_npis.h:36-242).
+ * Checked as osecir::Simulation::advance does (cpp/models/ode_secir/model.h:289-367): at the start of every
+ * node advance and at the end of an active NPI, against InfectedSymptoms / total population * base value;
+ * an exceeded threshold adds its dampings to the node's contact matrix (implement_dynamic_npis,
+ * dynamic_npis.h:318-392, with make_contact_damping_matrix of value * group weights). */
+enum {
+    MGO_NPI_DURATION = 0,    /* DynamicNPIs::get_duration (default 14) */
+    MGO_NPI_DELAY,           /* get_implementation_delay (default 0) */
+    MGO_NPI_BASE_VALUE,      /* get_base_value (default 1) */
+    MGO_NPI_DIRECTIVE_BEGIN, /* get_directive_begin (default lowest()) */
+    MGO_NPI_DIRECTIVE_END,   /* get_directive_end (default max()) */
+    MGO_NPI_NUM_PARAMS
+};
+
+typedef struct mgo_dynamic_npis {
+    const int32_t* node_offsets;    /* [num_nodes+1]: thresholds of node v are [node_offsets[v], node_offsets[v+1]) */
+    const double* node_params;      /* [num_nodes][MGO_NPI_NUM_PARAMS] */
+    const double* thresholds;       /* [T] threshold values (set_threshold order; equal values: the later wins) */
+    const int32_t* damping_offsets; /* [T+1]: dampings of threshold k are [damping_offsets[k], damping_offsets[k+1]) */
+    const double* damping_values;   /* [D] DampingSampling value */
+    const int32_t* damping_levels;  /* [D] DampingLevel */
+    const int32_t* damping_types;   /* [D] DampingType */
+    const double* damping_weights;  /* [D][G] group weights */
+} mgo_dynamic_npis;
+
+/* Device limit: entries of a node's damping list (its initial dampings plus those dynamic NPIs add) */
+#define MGO_NPI_LIST_EXTRA 32
+
 typedef struct mgo_graph_desc {
     int32_t num_nodes;
     int32_t num_groups;               /* G, 1..MGO_MAX_GROUPS */
@@ -116,6 +145,7 @@ typedef struct mgo_graph_desc {
     const int32_t* edge_end;          /* [num_edges] */
     const double* edge_coeffs;        /* [num_edges][edge_coeff_len] */
     int32_t edge_coeff_len;           /* 10*G, or 1 (one coefficient for all compartments); 0 means 10*G */
+    const mgo_dynamic_npis* dynamic_npis; /* NULL = no dynamic NPIs */
 } mgo_graph_desc;
 
 /* ---------------------------------------------------------------- integrators

@@ -70,6 +70,24 @@ _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
 
 
+NPI_DURATION, NPI_DELAY, NPI_BASE_VALUE, NPI_DIRECTIVE_BEGIN, NPI_DIRECTIVE_END = range(5)
+NPI_NUM_PARAMS = 5
+NPI_LIST_EXTRA = 32
+
+
+class DynamicNpis(C.Structure):
+    _fields_ = [
+        ("node_offsets", _ip),
+        ("node_params", _dp),
+        ("thresholds", _dp),
+        ("damping_offsets", _ip),
+        ("damping_values", _dp),
+        ("damping_levels", _ip),
+        ("damping_types", _ip),
+        ("damping_weights", _dp),
+    ]
+
+
 class GraphDesc(C.Structure):
     _fields_ = [
         ("num_nodes", C.c_int32),
@@ -89,6 +107,7 @@ class GraphDesc(C.Structure):
         ("edge_end", _ip),
         ("edge_coeffs", _dp),
         ("edge_coeff_len", C.c_int32),
+        ("dynamic_npis", C.POINTER(DynamicNpis)),
     ]
 
 

@@ -27,6 +27,7 @@
 
 #include "../../include/mgo.h"
 #include "mgo_internal.h"
+#include "mgo_npi.h"
 
 namespace mgo
 {
@@ -308,6 +309,12 @@ struct ClassHost {
         ipool, epool, cvec, estructs, inst_ids;
     DevBuf fx_rec, fx_pool, fx_dirty;  // fix-up records of the return passes
     DevBuf node_order;                 // node-kernel lane slot -> node (empty: identity)
+    // dynamic NPIs (nodes with NF_NPI): per-node records, thresholds, damping lists and caches; host copies
+    // of what the run changes, for reset
+    DevBuf npi_nodes, npi_thr, npi_thr_damp, npi_damp, npi_list, npi_cache_t, npi_cache_m, npi_work, npi_y0;
+    std::vector<NpiNode> npi_nodes0;
+    std::vector<double> npi_list0, npi_cache_t0, npi_cache_m0;
+    std::vector<NodeP> nodep0;
     std::vector<int32_t> order;        // host copy, built by place_nodes
     DevBuf y0;                         // initial state, for reset
     std::vector<double> tnode0, dtnode0; // initial clocks, for reset
@@ -407,6 +414,38 @@ void validate(const mgo_graph_desc* g, const mgo_sim_cfg& cfg)
     }
     if (g->damping_offsets && (!g->damping_times || !g->damping_coeffs)) {
         fail(MGO_ERR_INVALID_VALUE, "damping arrays missing");
+    }
+    if (const mgo_dynamic_npis* dn = g->dynamic_npis) {
+        if (!dn->node_offsets || !dn->node_params) {
+            fail(MGO_ERR_INVALID_VALUE, "dynamic NPIs: node_offsets and node_params are required");
+        }
+        if (dn->node_offsets[0] != 0) {
+            fail(MGO_ERR_INVALID_VALUE, "dynamic NPIs: node_offsets[0] must be 0");
+        }
+        for (int v = 0; v < g->num_nodes; ++v) {
+            if (dn->node_offsets[v + 1] < dn->node_offsets[v]) {
+                fail(MGO_ERR_INVALID_VALUE, "dynamic NPIs: node_offsets must be non-decreasing");
+            }
+            const double* p = dn->node_params + size_t(v) * MGO_NPI_NUM_PARAMS;
+            if (!(p[MGO_NPI_DURATION] >= 0) || !(p[MGO_NPI_DELAY] >= 0) || !std::isfinite(p[MGO_NPI_BASE_VALUE]) ||
+                !(p[MGO_NPI_DIRECTIVE_BEGIN] < p[MGO_NPI_DIRECTIVE_END])) {
+                fail(MGO_ERR_INVALID_VALUE, "dynamic NPIs of node %d: duration and delay must be >= 0, the base value "
+                     "finite and the directive begin before its end", v);
+            }
+        }
+        const int T = dn->node_offsets[g->num_nodes];
+        if (T > 0 && (!dn->thresholds || !dn->damping_offsets)) {
+            fail(MGO_ERR_INVALID_VALUE, "dynamic NPIs: thresholds and damping_offsets are required");
+        }
+        for (int t = 0; t < T; ++t) {
+            if (dn->damping_offsets[t + 1] < dn->damping_offsets[t] || dn->damping_offsets[t] < 0) {
+                fail(MGO_ERR_INVALID_VALUE, "dynamic NPIs: damping_offsets must be non-decreasing");
+            }
+        }
+        if (T > 0 && dn->damping_offsets[T] > dn->damping_offsets[0] &&
+            (!dn->damping_values || !dn->damping_levels || !dn->damping_types || !dn->damping_weights)) {
+            fail(MGO_ERR_INVALID_VALUE, "dynamic NPIs: damping arrays missing");
+        }
     }
     const int k = cfg.integrator.kind;
     if (k < MGO_INTEGRATOR_CASH_KARP54 || k > MGO_INTEGRATOR_EULER) {
@@ -550,6 +589,86 @@ void build_edges(const mgo_graph_desc* g, EdgeStructHost& es, std::vector<double
     }
 }
 
+// ------------------------------------------------------------------------------------------ dynamic NPIs
+// Host build of the per-node NPI data of one class (see NpiNode and mgo_npi.h).
+struct NpiBuild {
+    struct Node {
+        size_t node;
+        NpiNode rec{};
+        std::vector<double> list, cache_t, cache_m;
+    };
+    std::vector<Node> nodes;
+    std::vector<double> thr, damp;  // threshold values (per node descending); dampings [3 + G]
+    std::vector<int32_t> thr_damp;  // first damping of each threshold (+ a final end marker)
+};
+
+// DynamicNPIs of node v of graph g (desc arrays) with its contact matrix's initial dampings dl
+static void add_npi_node(NpiBuild& b, size_t node, const mgo_graph_desc* g, int v, int G, const std::vector<DampIn>& dl)
+{
+    const mgo_dynamic_npis* dn = g->dynamic_npis;
+    NpiBuild::Node nd;
+    nd.node           = node;
+    NpiNode& r        = nd.rec;
+    const double* par = dn->node_params + size_t(v) * MGO_NPI_NUM_PARAMS;
+    r.duration        = par[MGO_NPI_DURATION];
+    r.delay           = par[MGO_NPI_DELAY];
+    r.base            = par[MGO_NPI_BASE_VALUE];
+    r.begin           = par[MGO_NPI_DIRECTIVE_BEGIN];
+    r.end             = par[MGO_NPI_DIRECTIVE_END];
+    r.value           = -DBL_MAX;
+    r.t_end           = 0.0;
+    double total      = 0.0; // Populations::get_total of the model (summed in storage order)
+    for (int k = 0; k < G * NC; ++k) {
+        total += g->populations[size_t(v) * G * NC + size_t(k)];
+    }
+    r.total = total;
+    // DynamicNPIs::set_threshold: sorted by value, descending; an equal value replaces the earlier one
+    std::vector<std::pair<double, int>> thr;
+    for (int t = dn->node_offsets[v]; t < dn->node_offsets[v + 1]; ++t) {
+        const double val = dn->thresholds[t];
+        auto it          = thr.begin();
+        while (it != thr.end() && it->first > val) {
+            ++it;
+        }
+        if (it != thr.end() && !(val > it->first)) {
+            it->second = t;
+        }
+        else {
+            thr.insert(it, std::make_pair(val, t));
+        }
+    }
+    r.thr_first = int32_t(b.thr.size());
+    r.thr_count = int32_t(thr.size());
+    for (auto& tv : thr) {
+        b.thr.push_back(tv.first);
+        b.thr_damp.push_back(int32_t(b.damp.size() / size_t(3 + G)));
+        for (int d = dn->damping_offsets[tv.second]; d < dn->damping_offsets[tv.second + 1]; ++d) {
+            b.damp.push_back(dn->damping_values[d]);
+            b.damp.push_back(double(dn->damping_levels[d]));
+            b.damp.push_back(double(dn->damping_types[d]));
+            b.damp.insert(b.damp.end(), dn->damping_weights + size_t(d) * G, dn->damping_weights + size_t(d + 1) * G);
+        }
+    }
+    // damping list: the initial dampings in add order, room for MGO_NPI_LIST_EXTRA more
+    r.list_cap = int32_t(dl.size()) + MGO_NPI_LIST_EXTRA;
+    nd.list.assign(size_t(r.list_cap) * size_t(npi_entry_size(G)), 0.0);
+    int count = 0;
+    for (const DampIn& d : dl) {
+        npi_list_add(nd.list.data(), count, r.list_cap, G, d.time, d.level, d.type, d.m.data());
+    }
+    r.list_count = count;
+    nd.cache_t.assign(size_t(r.list_cap) + 2, 0.0);
+    nd.cache_m.assign((size_t(r.list_cap) + 2) * size_t(G) * G, 0.0);
+    std::vector<double> work(size_t(npi_work_size(G, r.list_cap)));
+    const int m = npi_build_cache(nd.list.data(), count, G, r.list_cap, nd.cache_t.data(), nd.cache_m.data(),
+                                  work.data());
+    if (m < 0) {
+        fail(MGO_ERR_UNSUPPORTED, "node %d: more than %d damping levels", v, NPI_LEVELS);
+    }
+    r.cache_count = m;
+    b.nodes.push_back(std::move(nd));
+}
+
 struct Batch {
     hipStream_t stream = nullptr;
     std::deque<ClassHost> classes;
@@ -680,6 +799,66 @@ struct Batch {
         }
     }
 
+    // device buffers of a class's dynamic NPIs; points the nodes' damping caches into them
+    void upload_npis(ClassHost& C, NpiBuild& b, std::vector<NodeP>& nodep)
+    {
+        C.npi_nodes0.clear();
+        if (b.nodes.empty()) {
+            return;
+        }
+        const int G  = C.G;
+        const size_t n = size_t(G) * G;
+        b.thr_damp.push_back(int32_t(b.damp.size() / size_t(3 + G)));
+        size_t nl = 0, nct = 0, nw = 0;
+        for (auto& nd : b.nodes) {
+            nl += nd.list.size();
+            nct += nd.cache_t.size();
+            nw += size_t(npi_work_size(G, nd.rec.list_cap));
+        }
+        C.npi_list0.clear();
+        C.npi_cache_t0.clear();
+        C.npi_cache_m0.clear();
+        C.npi_list0.reserve(nl);
+        C.npi_cache_t0.reserve(nct);
+        C.npi_cache_m0.reserve(nct * n);
+        for (auto& nd : b.nodes) {
+            C.npi_list0.insert(C.npi_list0.end(), nd.list.begin(), nd.list.end());
+            C.npi_cache_t0.insert(C.npi_cache_t0.end(), nd.cache_t.begin(), nd.cache_t.end());
+            C.npi_cache_m0.insert(C.npi_cache_m0.end(), nd.cache_m.begin(), nd.cache_m.end());
+        }
+        C.npi_list.upload(C.npi_list0, stream);
+        C.npi_cache_t.upload(C.npi_cache_t0, stream);
+        C.npi_cache_m.upload(C.npi_cache_m0, stream);
+        C.npi_work.alloc(nw * sizeof(double));
+        C.npi_y0.alloc(b.nodes.size() * size_t(G) * NC * sizeof(double));
+        C.npi_thr.upload(b.thr, stream);
+        C.npi_thr_damp.upload(b.thr_damp, stream);
+        C.npi_damp.upload(b.damp, stream);
+        C.npi_nodes0.assign(size_t(C.num_nodes), NpiNode{});
+        size_t ol = 0, oc = 0, ow = 0;
+        for (size_t k = 0; k < b.nodes.size(); ++k) {
+            NpiBuild::Node& nd = b.nodes[k];
+            NpiNode r          = nd.rec;
+            r.list             = C.npi_list.as<double>() + ol;
+            r.cache_t          = C.npi_cache_t.as<double>() + oc;
+            r.cache_m          = C.npi_cache_m.as<double>() + oc * n;
+            r.work             = C.npi_work.as<double>() + ow;
+            r.y0               = C.npi_y0.as<double>() + k * size_t(G) * NC;
+            ol += nd.list.size();
+            oc += nd.cache_t.size();
+            ow += size_t(npi_work_size(G, r.list_cap));
+            C.npi_nodes0[nd.node] = r;
+            NodeP& np             = nodep[nd.node];
+            np.damp_time          = r.cache_t;
+            np.damp_mat           = r.cache_m;
+            np.damp_cnt           = r.cache_count;
+        }
+        C.npi_nodes.upload(C.npi_nodes0, stream);
+        C.nodep0 = nodep;
+        device_bytes += C.npi_list.bytes + C.npi_cache_t.bytes + C.npi_cache_m.bytes + C.npi_work.bytes +
+                        C.npi_y0.bytes + C.npi_thr.bytes + C.npi_thr_damp.bytes + C.npi_damp.bytes + C.npi_nodes.bytes;
+    }
+
     void build_class(size_t ci, std::vector<RunSpec>& runs, std::vector<InstDev>& idev)
     {
         ClassHost& C = classes[ci];
@@ -697,6 +876,7 @@ struct Batch {
         };
         std::vector<StructOff> soffs;
         std::vector<size_t> damp_off(N, 0);
+        NpiBuild npi;
         std::map<std::tuple<const void*, const void*, const void*, int, int>, int> struct_cache;
         int64_t out_total = 0;
         int64_t fx_total  = 0;
@@ -763,7 +943,12 @@ struct Batch {
                 np.damp_cnt    = 0;
                 np.damp_time   = nullptr;
                 np.damp_mat    = nullptr;
-                if (!dl.empty()) {
+                const mgo_dynamic_npis* dn = g->dynamic_npis;
+                if (dn && dn->node_offsets[v + 1] > dn->node_offsets[v]) {
+                    add_npi_node(npi, node, g, v, G, dl);
+                    np.flags |= NF_NPI | NF_DAMPED;
+                }
+                else if (!dl.empty()) {
                     std::vector<double> tt, mm;
                     damping_cache(dl, G, tt, mm);
                     np.flags |= NF_DAMPED;
@@ -856,11 +1041,12 @@ struct Batch {
         up(C.dtime, dtime);
         up(C.dmat, dmat);
         for (size_t node = 0; node < N; ++node) {
-            if (nodep[node].damp_cnt > 0) {
+            if (nodep[node].damp_cnt > 0 && !(nodep[node].flags & NF_NPI)) {
                 nodep[node].damp_time = C.dtime.as<double>() + damp_off[node];
                 nodep[node].damp_mat  = C.dmat.as<double>() + damp_off[node] * G * G;
             }
         }
+        upload_npis(C, npi, nodep);
         up(C.nodep, nodep);
         up(C.tnode, tnode);
         up(C.dtnode, dtnode);
@@ -949,6 +1135,10 @@ struct Batch {
         d.fx_rec    = C.fx_rec.as<FxRec>();
         d.fx_pool   = C.fx_pool.as<double>();
         d.fx_dirty  = C.fx_dirty.as<int32_t>();
+        d.npi          = C.npi_nodes0.empty() ? nullptr : C.npi_nodes.as<NpiNode>();
+        d.npi_thr      = C.npi_thr.as<double>();
+        d.npi_thr_damp = C.npi_thr_damp.as<int32_t>();
+        d.npi_damp     = C.npi_damp.as<double>();
         C.seg_host = d;
         std::vector<SegDev> segs{d};
         std::vector<int32_t> bseg, bfirst;
@@ -965,6 +1155,7 @@ struct Batch {
         ld.num_blocks  = int(bseg.size());
         ld.G           = G;
         ld.kind        = C.kind;
+        ld.npi         = C.npi_nodes0.empty() ? 0 : 1;
     }
 
     // launch tables of a cycle: one per class. (Batches keep per-G kernels so that a run's arithmetic is
@@ -1121,6 +1312,18 @@ struct Batch {
             hip_check(hipMemsetAsync(C.steps.p, 0, C.steps.bytes, stream), "reset steps");
             hip_check(hipMemsetAsync(C.attempts.p, 0, C.attempts.bytes, stream), "reset attempts");
             C.clear_fixup_records(stream);
+            if (!C.npi_nodes0.empty()) {
+                // damping lists, caches and NPI state as built (dynamic NPIs change them during a run)
+                auto h2d = [&](DevBuf& b, const auto& v, const char* what) {
+                    hip_check(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice, stream),
+                              what);
+                };
+                h2d(C.npi_nodes, C.npi_nodes0, "reset NPI records");
+                h2d(C.npi_list, C.npi_list0, "reset damping lists");
+                h2d(C.npi_cache_t, C.npi_cache_t0, "reset damping caches");
+                h2d(C.npi_cache_m, C.npi_cache_m0, "reset damping caches");
+                h2d(C.nodep, C.nodep0, "reset node records");
+            }
         }
         hip_check(hipMemcpyAsync(inst_dev.p, inst0.data(), inst0.size() * sizeof(InstDev), hipMemcpyHostToDevice,
                                  stream),

@@ -94,6 +94,25 @@ enum : int32_t {
     NF_ICU_ACTIVE  = 2, // ICUCapacity < DBL_MAX: the critical-care smoother depends on the state
     NF_DAMPED      = 4, // contact matrix has dampings
     NF_DETECTION   = 8, // commuter detection interval non-empty
+    NF_NPI         = 16, // dynamic NPIs: the damping list lives in an NpiNode and changes during the run
+};
+
+// Dynamic NPIs of one node (osecir::Simulation::advance, models/ode_secir/model.h:289-367). The damping
+// list, its cache (NodeP::damp_time / damp_mat point into it) and the NPI state change during the run;
+// the kernel that changes them is the node kernel (one lane per node, see mgo_npi.h).
+struct NpiNode {
+    double duration, delay, base, begin, end; // DynamicNPIs parameters
+    double total;    // the model's total population (get_infections_relative's denominator)
+    double value;    // threshold of the active NPI (m_dynamic_npi.first), lowest() before the first
+    double t_end;    // end of the active NPI (m_dynamic_npi.second)
+    double* list;    // [list_cap][3 + G*G]
+    double* cache_t; // [list_cap + 2]
+    double* cache_m; // [list_cap + 2][G][G]
+    double* work;    // npi_work_size(G, list_cap) doubles
+    double* y0;      // [G][10] state at the start of the current node phase (return coefficients)
+    int32_t thr_first, thr_count; // thresholds in SegDev::npi_thr (descending values)
+    int32_t list_count, list_cap;
+    int32_t cache_count, pad;
 };
 
 struct NodeP {
@@ -211,6 +230,10 @@ struct SegDev {
     double* fx_pool;               // fixed traveler vectors [slot][G][10]
     int32_t* fx_dirty;             // [2][node] tag of the pass that changed a record of the node's out-edges
                                    // (plane = pass parity)
+    NpiNode* npi;                  // [node] dynamic NPIs (nodes with NF_NPI), or NULL
+    const double* npi_thr;         // threshold values
+    const int32_t* npi_thr_damp;   // [threshold + 1] first damping of each threshold in npi_damp
+    const double* npi_damp;        // [damping][3 + G] value, level, type, group weights
 };
 
 // One kernel launch over the segments of a class.
@@ -223,6 +246,8 @@ struct LaunchDev {
     int32_t num_blocks;         // launched blocks (multiple of 8)
     int32_t G;                  // age groups of every segment, or 0: segments differ (mixed kernels)
     int32_t kind;
+    int32_t npi;                // some node has dynamic NPIs (per-G kernels only)
+    int32_t pad;
 };
 
 } // namespace mgo

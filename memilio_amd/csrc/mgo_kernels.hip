@@ -24,6 +24,7 @@
 
 #include "../../include/mgo.h"
 #include "mgo_internal.h"
+#include "mgo_npi.h"
 
 namespace mgo
 {
@@ -509,6 +510,102 @@ __device__ __forceinline__ size_t seg_node(const InstDev& I, int v)
     return size_t(I.node_base) + size_t(v) * size_t(I.node_stride);
 }
 
+// Return coefficients of the lane's node as a destination (calculate_mobility_returns,
+// metapopulation_mobility_instant.h:460-471): rates and force of infection of the state pop at the departure
+// time, frozen for the Euler step of the returning travelers.
+template <int G>
+__device__ __forceinline__ void return_coefficients(const SegDev& c, Lane<G>& L, const InstDev& I,
+                                                    const double (&pop)[NC])
+{
+    double c_icr, c_d;
+    const double kappa = force_of_infection<G>(c, L, pop, I.c_tdep, season_at<G>(L, I.c_tdep), c_icr, c_d);
+    double* rc         = c.rc + (size_t(L.node) * G + L.age) * RC_STRIDE;
+    rc[RC_KAPPA]       = kappa;
+    rc[RC_E]           = L.c[LC_E];
+    rc[RC_INS_ISY]     = L.c[LC_INS_ISY];
+    rc[RC_INS_R]       = L.c[LC_INS_R];
+    rc[RC_ISY_ISEV]    = L.c[LC_ISY_ISEV];
+    rc[RC_ISY_R]       = L.c[LC_ISY_R];
+    rc[RC_ISEV_ICR]    = c_icr;
+    rc[RC_ISEV_R]      = L.c[LC_ISEV_R];
+    rc[RC_ISEV_D]      = c_d;
+    rc[RC_ICR_D]       = L.c[LC_ICR_D];
+    rc[RC_ICR_R]       = L.c[LC_ICR_R];
+}
+
+// ------------------------------------------------------------------------------- dynamic NPIs
+// The check at the start of one piece of osecir::Simulation::advance (models/ode_secir/model.h:301-349) for
+// the lane's node at time t with state y, and the length of the piece (model.h:351-357: at most one day,
+// ending at the active NPI's end). Every lane of the node reaches the same decision; the first lane of the
+// group implements the NPI on the node's damping list and cache (mgo_npi.h). Returns true in `changed`
+// when the contact matrix changed.
+template <int G>
+__device__ double npi_piece(const SegDev& c, Lane<G>& L, InstDev& I, const double (&y)[NC], double t, double tmax,
+                            bool& changed)
+{
+    NpiNode* nn         = c.npi + L.node;
+    const double sum    = group_sum<G>(y[ISY], L.base); // InfectedSymptoms summed over the groups in order
+    const double begin  = nn->begin, end = nn->end;
+    double t_end        = nn->t_end;
+    changed             = false;
+    // floating_point_greater_equal(a, b, 1e-10) (math/floating_point.h:96-100)
+    auto ge = [](double a, double b) {
+        return !(a - b < -(1e-10 + DBL_MIN * fmax(fabs(a), fabs(b))));
+    };
+    if (ge(t, begin) && t < end) {
+        const double inf_rel = sum / nn->total * nn->base;
+        int k                = 0;
+        while (k < nn->thr_count && !(inf_rel > c.npi_thr[nn->thr_first + k])) {
+            ++k;
+        }
+        const bool expired = ge(t, t_end);
+        if (k < nn->thr_count) {
+            const double thr   = c.npi_thr[nn->thr_first + k];
+            const double value = nn->value;
+            if (thr > value || expired) {
+                const bool renewal = expired && !(thr > value);
+                const double delay = renewal ? 0.0 : (t > 0 ? nn->delay : 0.0);
+                if (t + delay < end) {
+                    const double t_start = t + delay;
+                    const double te      = (t_start + nn->duration < end) ? t_start + nn->duration : end;
+                    const double tsd     = (!renewal && t_start > 0.0) ? t_start + 1.0 : t_start;
+                    t_end                = te;
+                    changed              = true;
+                    if (L.age == 0) {
+                        nn->value = thr;
+                        nn->t_end = te;
+                        const int gi = nn->thr_first + k;
+                        const int d0 = c.npi_thr_damp[gi], d1 = c.npi_thr_damp[gi + 1];
+                        int count    = nn->list_count;
+                        bool ok      = npi_implement(nn->list, count, nn->list_cap, G, c.npi_damp + d0 * (3 + G),
+                                                     d1 - d0, tsd, te + 1.0, nn->work);
+                        const int m  = ok ? npi_build_cache(nn->list, count, G, nn->list_cap, nn->cache_t,
+                                                            nn->cache_m, nn->work)
+                                          : -1;
+                        if (m < 0) {
+                            I.status = MGO_ERR_OUT_OF_RANGE; // damping list or level capacity exceeded
+                        }
+                        else {
+                            nn->list_count  = count;
+                            nn->cache_count = m;
+                            const_cast<NodeP*>(c.nodep)[L.node].damp_cnt = m;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    if (changed) {
+        __threadfence(); // the group's other lanes read the new cache written by its first lane
+        L.np.damp_cnt = __shfl(*static_cast<volatile int32_t*>(&nn->cache_count), L.base);
+    }
+    double dt_eff = (tmax - t < 1.0) ? tmax - t : 1.0;
+    if (t < t_end) {
+        dt_eff = (t_end - t < dt_eff) ? t_end - t : dt_eff;
+    }
+    return dt_eff;
+}
+
 // ------------------------------------------------------------------------------- node phase
 // SystemIntegrator::advance (integrator.h:184-249) of every node to the cycle's end time with the
 // integrator core K: odeint Cash-Karp controller (stepper_wrapper.h:112-163), RKIntegratorCore
@@ -517,7 +614,7 @@ __device__ __forceinline__ size_t seg_node(const InstDev& I, int v)
 // the k1 evaluation of its neighbours in the wavefront.
 constexpr uint32_t kMaxAttemptsPerPhase = 1u << 20;
 
-template <int G, int K>
+template <int G, int K, bool NPI = false>
 __device__ __forceinline__ void node_body(const LaunchDev& ld, InstDev* __restrict__ inst)
 {
     Lane<G> L;
@@ -538,8 +635,13 @@ __device__ __forceinline__ void node_body(const LaunchDev& ld, InstDev* __restri
     }
     double t            = c.tnode[L.node];
     double dt           = c.dtnode[L.node];
-    const double t0     = t;
-    const double tmax   = I.c_t1;
+    const double t_phase = I.c_t1;
+    // Without dynamic NPIs the phase is one SystemIntegrator::advance to c_t1; with them it is the pieces of
+    // osecir::Simulation::advance (at most a day each, ending at NPI ends), each starting with the NPI check
+    const bool npi      = NPI && (L.np.flags & NF_NPI) != 0; // NPI: instantiation for classes with NPIs
+    double t_loop       = t; // osecir::Simulation::advance's loop time (model.h:301)
+    double tmax         = t_phase;
+    double t0           = t;
     const double dt_max = I.dt_max;
     double dt_min       = I.dt_min;
     double dt_restore   = 0.0;
@@ -550,167 +652,185 @@ __device__ __forceinline__ void node_body(const LaunchDev& ld, InstDev* __restri
     int next_day     = max(0, int(floor(I.c_t0 + 1e-10)) + 1 - I.day0);
     const double tde = I.c_t1 - 1e-10;
 
-    if (I.c_action == 2) {
-        // return coefficients of this node as a destination: its state after the departures, at the
-        // departure time (the `pop` of calculate_mobility_returns, metapopulation_mobility_instant.h:460-471)
-        double c_icr, c_d;
-        const double kappa = force_of_infection<G>(c, L, y, I.c_tdep, season_at<G>(L, I.c_tdep), c_icr, c_d);
-        double* rc         = c.rc + (size_t(L.node) * G + L.age) * RC_STRIDE;
-        rc[RC_KAPPA]       = kappa;
-        rc[RC_E]           = L.c[LC_E];
-        rc[RC_INS_ISY]     = L.c[LC_INS_ISY];
-        rc[RC_INS_R]       = L.c[LC_INS_R];
-        rc[RC_ISY_ISEV]    = L.c[LC_ISY_ISEV];
-        rc[RC_ISY_R]       = L.c[LC_ISY_R];
-        rc[RC_ISEV_ICR]    = c_icr;
-        rc[RC_ISEV_R]      = L.c[LC_ISEV_R];
-        rc[RC_ISEV_D]      = c_d;
-        rc[RC_ICR_D]       = L.c[LC_ICR_D];
-        rc[RC_ICR_R]       = L.c[LC_ICR_R];
-    }
-    bool active  = fabs((tmax - t) / (tmax - t0)) > eps;
+    bool active  = false;
     bool need_k1 = true;
-    bool fail = true, valid = true;
+    bool fail = true, valid = true, first = true, stalled = false;
     uint32_t nsteps = 0, natt = 0;
     double k1[NC];
-    while (active) {
-        if (natt >= kMaxAttemptsPerPhase) {
-            // a step size driven below any progress (dt_min ~ DBL_MIN with a persistent error): stop this
-            // node and report instead of spinning; the reference would not terminate here
-            if (L.age == 0) {
-                I.status = MGO_ERR_OUT_OF_RANGE;
-            }
-            break;
+    for (;;) { // pieces of the node advance (one without dynamic NPIs)
+        bool changed = false;
+        if (npi) {
+            tmax = t_loop + npi_piece<G>(c, L, I, y, t_loop, t_phase, changed);
         }
-        if (need_k1) {
-            if (dt > tmax - t) {
-                dt_restore = dt;
-                dt         = tmax - t;
-                dt_min     = fmin(tmax - t, dt_min);
-            }
-            rhs<G>(c, L, y, t, season_at<G>(L, t), k1);
-            need_k1 = false;
-            fail    = true;
-            valid   = true;
-            if (K == MGO_INTEGRATOR_RKF45) {
-                dt = fmin(dt, dt_max);
-            }
-        }
-        double yo[NC], er[NC];
-        bool done       = false;
-        const double ta = t;
-        ++natt;
-        if (K == MGO_INTEGRATOR_CASH_KARP54) {
-            if (dt < dt_min) {
-                valid = false;
-                dt    = dt_min;
-            }
-            if (!(dt - dt_max <= DBL_EPSILON)) {
-                dt   = dt_max; // check_step_size_limit: rejected without a step
-                fail = true;
-            }
-            else {
-                cash_karp_stages<G>(c, L, y, k1, t, dt, yo, er, true);
-                // max_i |err_i| / (abs + rel (|y_i| + dt |k1_i|)) (odeint default_error_checker): the index of
-                // the largest quotient by cross-multiplied comparison, then one division
-                double en = 0.0, ed = 1.0;
+        if (I.c_action == 2 && (first || changed)) {
+            // return coefficients of this node as a destination: its state after the departures, at the
+            // departure time (the `pop` of calculate_mobility_returns, metapopulation_mobility_instant.h:460-471),
+            // with the contact matrix as it is after the node's NPI checks of this phase
+            double pop[NC];
 #pragma unroll
-                for (int i = 0; i < NC; ++i) {
-                    const double n = fabs(er[i]);
-                    const double d = I.abs_tol + I.rel_tol * (fabs(y[i]) + dt * fabs(k1[i]));
-                    if (n * ed > en * d) {
-                        en = n;
-                        ed = d;
-                    }
+            for (int i = 0; i < NC; ++i) {
+                pop[i] = first ? y[i] : c.npi[L.node].y0[L.age * NC + i];
+                if (first && npi) {
+                    c.npi[L.node].y0[L.age * NC + i] = y[i];
                 }
-                double e = en / ed;
-                e = group_max<G>(e, L.base);
-                if (e > 1.0) {
-                    dt *= fmax(0.9 * pow(e, -1.0 / 3.0), 0.2);
-                    dt   = fmin(dt, dt_max);
+            }
+            return_coefficients<G>(c, L, I, pop);
+        }
+        first  = false;
+        t0     = t;
+        active = fabs((tmax - t) / (tmax - t0)) > eps;
+        while (active) {
+            if (natt >= kMaxAttemptsPerPhase) {
+                // a step size driven below any progress (dt_min ~ DBL_MIN with a persistent error): stop this
+                // node and report instead of spinning; the reference would not terminate here
+                if (L.age == 0) {
+                    I.status = MGO_ERR_OUT_OF_RANGE;
+                }
+                stalled = true;
+                break;
+            }
+            if (need_k1) {
+                if (dt > tmax - t) {
+                    dt_restore = dt;
+                    dt         = tmax - t;
+                    dt_min     = fmin(tmax - t, dt_min);
+                }
+                rhs<G>(c, L, y, t, season_at<G>(L, t), k1);
+                need_k1 = false;
+                fail    = true;
+                valid   = true;
+                if (K == MGO_INTEGRATOR_RKF45) {
+                    dt = fmin(dt, dt_max);
+                }
+            }
+            double yo[NC], er[NC];
+            bool done       = false;
+            const double ta = t;
+            ++natt;
+            if (K == MGO_INTEGRATOR_CASH_KARP54) {
+                if (dt < dt_min) {
+                    valid = false;
+                    dt    = dt_min;
+                }
+                if (!(dt - dt_max <= DBL_EPSILON)) {
+                    dt   = dt_max; // check_step_size_limit: rejected without a step
                     fail = true;
                 }
                 else {
-                    t += dt;
-                    if (e < 0.5) {
-                        e = fmax(1.0 / 3125.0, e);
-                        dt *= 0.9 * pow(e, -0.2);
-                        dt = fmin(dt, dt_max);
-                    }
-                    fail = false;
-                }
-            }
-            done = !(fail && valid);
-            if (done) {
-                dt = fmax(dt, dt_min);
-                if (fail) {
-                    t += dt_min;
-                }
-            }
-        }
-        else if (K == MGO_INTEGRATOR_RKF45) {
-            if (dt < dt_min) {
-                valid = false;
-                dt    = dt_min;
-            }
-            rkf45_stages<G>(c, L, y, k1, t, dt, yo, er);
-            bool conv = true;
-            double q  = INFINITY;
-#pragma unroll
-            for (int i = 0; i < NC; ++i) {
-                const double ep = I.abs_tol + fabs(yo[i]) * I.rel_tol;
-                conv            = conv && (er[i] <= ep);
-                q               = fmin(q, ep / er[i]);
-            }
-            conv = group_all<G>(conv, L.base);
-            q    = group_min<G>(q, L.base);
-            if (conv || !valid) {
-                t += dt;
-            }
-            double dt_new = dt * pow(q, 1.0 / 5.0);
-            dt_new *= 0.9;
-            dt   = fmin(dt_new, dt_max);
-            done = conv || !valid;
-            if (done) {
-                dt = fmax(dt, dt_min);
-            }
-        }
-        else if (K == MGO_INTEGRATOR_CASH_KARP54_FIXED) {
-            cash_karp_stages<G>(c, L, y, k1, t, dt, yo, er, false);
-            t += dt;
-            done = true;
-        }
-        else { // Euler
-#pragma unroll
-            for (int i = 0; i < NC; ++i) {
-                yo[i] = y[i] + dt * k1[i];
-            }
-            t += dt;
-            done = true;
-        }
-        if (done) {
-            ++nsteps;
-            // interpolate_simulation_result (analyze_result.h:235-252): days in (ta, t] strictly inside
-            // the phase
-            while (next_day < ndays && double(next_day + I.day0) <= t && double(next_day + I.day0) < tde) {
-                const double dd = double(next_day + I.day0);
-                if (dd > ta) {
-                    const double w = (dd - ta) / (t - ta);
-                    double* op     = out_ptr(I, L.np.local, L.age, G, next_day);
+                    cash_karp_stages<G>(c, L, y, k1, t, dt, yo, er, true);
+                    // max_i |err_i| / (abs + rel (|y_i| + dt |k1_i|)) (odeint default_error_checker): the index of
+                    // the largest quotient by cross-multiplied comparison, then one division
+                    double en = 0.0, ed = 1.0;
 #pragma unroll
                     for (int i = 0; i < NC; ++i) {
-                        op[i] = y[i] + w * (yo[i] - y[i]);
+                        const double n = fabs(er[i]);
+                        const double d = I.abs_tol + I.rel_tol * (fabs(y[i]) + dt * fabs(k1[i]));
+                        if (n * ed > en * d) {
+                            en = n;
+                            ed = d;
+                        }
+                    }
+                    double e = en / ed;
+                    e = group_max<G>(e, L.base);
+                    if (e > 1.0) {
+                        dt *= fmax(0.9 * pow(e, -1.0 / 3.0), 0.2);
+                        dt   = fmin(dt, dt_max);
+                        fail = true;
+                    }
+                    else {
+                        t += dt;
+                        if (e < 0.5) {
+                            e = fmax(1.0 / 3125.0, e);
+                            dt *= 0.9 * pow(e, -0.2);
+                            dt = fmin(dt, dt_max);
+                        }
+                        fail = false;
                     }
                 }
-                ++next_day;
+                done = !(fail && valid);
+                if (done) {
+                    dt = fmax(dt, dt_min);
+                    if (fail) {
+                        t += dt_min;
+                    }
+                }
             }
+            else if (K == MGO_INTEGRATOR_RKF45) {
+                if (dt < dt_min) {
+                    valid = false;
+                    dt    = dt_min;
+                }
+                rkf45_stages<G>(c, L, y, k1, t, dt, yo, er);
+                bool conv = true;
+                double q  = INFINITY;
 #pragma unroll
-            for (int i = 0; i < NC; ++i) {
-                y[i] = yo[i];
+                for (int i = 0; i < NC; ++i) {
+                    const double ep = I.abs_tol + fabs(yo[i]) * I.rel_tol;
+                    conv            = conv && (er[i] <= ep);
+                    q               = fmin(q, ep / er[i]);
+                }
+                conv = group_all<G>(conv, L.base);
+                q    = group_min<G>(q, L.base);
+                if (conv || !valid) {
+                    t += dt;
+                }
+                double dt_new = dt * pow(q, 1.0 / 5.0);
+                dt_new *= 0.9;
+                dt   = fmin(dt_new, dt_max);
+                done = conv || !valid;
+                if (done) {
+                    dt = fmax(dt, dt_min);
+                }
             }
-            need_k1 = true;
-            active  = fabs((tmax - t) / (tmax - t0)) > eps;
+            else if (K == MGO_INTEGRATOR_CASH_KARP54_FIXED) {
+                cash_karp_stages<G>(c, L, y, k1, t, dt, yo, er, false);
+                t += dt;
+                done = true;
+            }
+            else { // Euler
+#pragma unroll
+                for (int i = 0; i < NC; ++i) {
+                    yo[i] = y[i] + dt * k1[i];
+                }
+                t += dt;
+                done = true;
+            }
+            if (done) {
+                ++nsteps;
+                // interpolate_simulation_result (analyze_result.h:235-252): days in (ta, t] strictly inside
+                // the phase
+                while (next_day < ndays && double(next_day + I.day0) <= t && double(next_day + I.day0) < tde) {
+                    const double dd = double(next_day + I.day0);
+                    if (dd > ta) {
+                        const double w = (dd - ta) / (t - ta);
+                        double* op     = out_ptr(I, L.np.local, L.age, G, next_day);
+#pragma unroll
+                        for (int i = 0; i < NC; ++i) {
+                            op[i] = y[i] + w * (yo[i] - y[i]);
+                        }
+                    }
+                    ++next_day;
+                }
+#pragma unroll
+                for (int i = 0; i < NC; ++i) {
+                    y[i] = yo[i];
+                }
+                need_k1 = true;
+                active  = fabs((tmax - t) / (tmax - t0)) > eps;
+            }
         }
+        if (!npi || stalled) {
+            break;
+        }
+        // end of a piece: SystemIntegrator::advance's exit (integrator.h:244-248), then the next piece
+        dt         = fmax(dt, dt_restore);
+        dt_restore = 0.0;
+        dt_min     = I.dt_min;
+        t_loop     = tmax;
+        if (!(t_loop < t_phase)) {
+            break;
+        }
+        need_k1 = true;
     }
     dt = fmax(dt, dt_restore);
 #pragma unroll
@@ -1235,10 +1355,10 @@ __device__ __forceinline__ int block_groups(const LaunchDev& ld)
     default: break;                                                                                            \
     }
 
-template <int G, int K>
+template <int G, int K, bool NPI>
 __global__ void __launch_bounds__(BLOCK) node_phase(LaunchDev ld, InstDev* __restrict__ inst)
 {
-    node_body<G, K>(ld, inst);
+    node_body<G, K, NPI>(ld, inst);
 }
 
 template <int K>
@@ -1294,20 +1414,28 @@ template <int G>
 static hipError_t launch_node_g(const LaunchDev& c, InstDev* inst, hipStream_t s)
 {
     const int nb = c.num_blocks;
+#define MGO_NODE_LAUNCH(KIND)                                                                                  \
+    if (c.npi) {                                                                                               \
+        hipLaunchKernelGGL((node_phase<G, KIND, true>), dim3(nb), dim3(BLOCK), 0, s, c, inst);                 \
+    }                                                                                                          \
+    else {                                                                                                     \
+        hipLaunchKernelGGL((node_phase<G, KIND, false>), dim3(nb), dim3(BLOCK), 0, s, c, inst);                \
+    }
     switch (c.kind) {
     case MGO_INTEGRATOR_CASH_KARP54:
-        hipLaunchKernelGGL((node_phase<G, MGO_INTEGRATOR_CASH_KARP54>), dim3(nb), dim3(BLOCK), 0, s, c, inst);
+        MGO_NODE_LAUNCH(MGO_INTEGRATOR_CASH_KARP54)
         break;
     case MGO_INTEGRATOR_RKF45:
-        hipLaunchKernelGGL((node_phase<G, MGO_INTEGRATOR_RKF45>), dim3(nb), dim3(BLOCK), 0, s, c, inst);
+        MGO_NODE_LAUNCH(MGO_INTEGRATOR_RKF45)
         break;
     case MGO_INTEGRATOR_CASH_KARP54_FIXED:
-        hipLaunchKernelGGL((node_phase<G, MGO_INTEGRATOR_CASH_KARP54_FIXED>), dim3(nb), dim3(BLOCK), 0, s, c, inst);
+        MGO_NODE_LAUNCH(MGO_INTEGRATOR_CASH_KARP54_FIXED)
         break;
     default:
-        hipLaunchKernelGGL((node_phase<G, MGO_INTEGRATOR_EULER>), dim3(nb), dim3(BLOCK), 0, s, c, inst);
+        MGO_NODE_LAUNCH(MGO_INTEGRATOR_EULER)
         break;
     }
+#undef MGO_NODE_LAUNCH
     return hipGetLastError();
 }
 

@@ -6,6 +6,8 @@ per-node osecir parameters (cpp/models/ode_secir/parameters.h:46-394), one conta
 with its dampings (cpp/memilio/epidemiology/contact_matrix.h, damping.h), initial populations and
 the mobility edges with their coefficient vectors.
 """
+import ctypes as C
+
 import numpy as np
 
 from . import capi
@@ -59,6 +61,7 @@ class GraphSpec:
         self.edge_start = np.zeros(0, np.int32)
         self.edge_end = np.zeros(0, np.int32)
         self.edge_coeffs = np.zeros(0)   # [E] (one value per edge) or [E][10G]
+        self.npis = None                 # dynamic NPI arrays (set_dynamic_npis)
         self._desc = None
 
     # ---------------------------------------------------------------- building
@@ -79,6 +82,45 @@ class GraphSpec:
         self.damping_levels = np.asarray(levels, np.int32)
         self.damping_types = np.asarray(types, np.int32)
         self.damping_coeffs = (np.stack(coeffs) if coeffs else np.zeros((0, G, G))).astype(np.float64)
+        self._desc = None
+
+    def set_dynamic_npis(self, per_node):
+        """Dynamic NPIs of the nodes' contact matrices (osecir DynamicNPIsInfectedSymptoms).
+
+        per_node: list (len num_nodes) of None or dicts with keys ``thresholds`` (list of (value, dampings),
+        dampings = list of (value, level, type, group_weights[G])), and optionally ``duration`` (14),
+        ``delay`` (0), ``base_value`` (1), ``directive_begin`` (lowest), ``directive_end`` (max)."""
+        G = self.G
+        node_offsets, params, thresholds, damp_offsets = [0], [], [], [0]
+        values, levels, types, weights = [], [], [], []
+        for cfg in per_node:
+            cfg = cfg or {}
+            p = np.zeros(capi.NPI_NUM_PARAMS)
+            p[capi.NPI_DURATION] = cfg.get("duration", 14.0)
+            p[capi.NPI_DELAY] = cfg.get("delay", 0.0)
+            p[capi.NPI_BASE_VALUE] = cfg.get("base_value", 1.0)
+            p[capi.NPI_DIRECTIVE_BEGIN] = cfg.get("directive_begin", -capi.DBL_MAX)
+            p[capi.NPI_DIRECTIVE_END] = cfg.get("directive_end", capi.DBL_MAX)
+            params.append(p)
+            for value, dampings in cfg.get("thresholds", []):
+                thresholds.append(float(value))
+                for (v, lev, typ, w) in dampings:
+                    values.append(float(v))
+                    levels.append(int(lev))
+                    types.append(int(typ))
+                    weights.append(np.broadcast_to(np.asarray(w, dtype=np.float64), (G,)))
+                damp_offsets.append(len(values))
+            node_offsets.append(len(thresholds))
+        self.npis = {
+            "node_offsets": np.asarray(node_offsets, np.int32),
+            "node_params": np.ascontiguousarray(np.stack(params), dtype=np.float64),
+            "thresholds": np.asarray(thresholds, np.float64),
+            "damping_offsets": np.asarray(damp_offsets, np.int32),
+            "damping_values": np.asarray(values, np.float64),
+            "damping_levels": np.asarray(levels, np.int32),
+            "damping_types": np.asarray(types, np.int32),
+            "damping_weights": np.ascontiguousarray(np.stack(weights) if weights else np.zeros((0, G))),
+        }
         self._desc = None
 
     def set_edges(self, start, end, coeffs):
@@ -132,6 +174,13 @@ class GraphSpec:
             d.edge_end = capi.iptr(self.edge_end) if self.num_edges else None
             d.edge_coeffs = capi.dptr(self.edge_coeffs) if self.num_edges else None
             d.edge_coeff_len = 1 if self.edge_coeffs.ndim == 1 else NC * self.G
+            if self.npis is not None:
+                n = self.npis
+                self._npi_struct = capi.DynamicNpis(
+                    capi.iptr(n["node_offsets"]), capi.dptr(n["node_params"]), capi.dptr(n["thresholds"]),
+                    capi.iptr(n["damping_offsets"]), capi.dptr(n["damping_values"]), capi.iptr(n["damping_levels"]),
+                    capi.iptr(n["damping_types"]), capi.dptr(n["damping_weights"]))
+                d.dynamic_npis = C.pointer(self._npi_struct)
             self._desc = d
         return self._desc
 

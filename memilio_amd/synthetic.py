@@ -152,3 +152,18 @@ def fixed_cfg(dt=0.1, t0=0.0, dt_graph=0.5):
     """Fixed-step Cash-Karp (ExplicitStepperWrapper<runge_kutta_cash_karp54>) with step dt."""
     return capi.sim_cfg(t0=t0, dt_graph=dt_graph, dt_node=dt,
                         integrator=capi.integrator_cfg(capi.INTEGRATOR_CASH_KARP54_FIXED, 0.0, 0.0, 0.0, 0.0))
+
+
+def with_npis(spec, thresholds=((200.0, 0.3), (800.0, 0.6)), duration=14.0, delay=3.0, base_value=100000.0,
+              directive_begin=None, directive_end=None, levels=(0, 1)):
+    """Dynamic NPIs on every node of spec: per threshold (cases per base_value, damping value), one damping of
+    that value with uniform group weights, at level levels[k % len(levels)]. Returns spec."""
+    cfg = {"thresholds": [(thr, [(val, levels[k % len(levels)], 0, np.ones(spec.G))])
+                          for k, (thr, val) in enumerate(thresholds)],
+           "duration": duration, "delay": delay, "base_value": base_value}
+    if directive_begin is not None:
+        cfg["directive_begin"] = directive_begin
+    if directive_end is not None:
+        cfg["directive_end"] = directive_end
+    spec.set_dynamic_npis([cfg] * spec.num_nodes)
+    return spec

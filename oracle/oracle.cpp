@@ -114,6 +114,28 @@ public:
         return m_dampings.empty();
     }
 
+    // damping.h:336-341 remove(i)
+    void remove(size_t i)
+    {
+        m_dampings.erase(m_dampings.begin() + long(i));
+        m_cache_mats.clear();
+        m_cache_times.clear();
+        update_cache();
+    }
+
+    const std::vector<Damping>& list() const
+    {
+        return m_dampings;
+    }
+    int rows() const
+    {
+        return m_rows;
+    }
+    int cols() const
+    {
+        return m_cols;
+    }
+
 private:
     // damping.h:570-600 update_cache
     void update_cache()
@@ -224,6 +246,151 @@ struct ContactGroup {
     }
 };
 
+// ------------------------------------------------------------------------------------------ dynamic NPIs
+// DampingSampling (epidemiology/damping_sampling.h:40-230) as far as implement_dynamic_npis reads it.
+struct NpiDamping {
+    double value;
+    int level, type;
+    std::vector<int> mats; // matrix indices of the group
+    Vec weights;           // group weights
+};
+
+// make_contact_damping_matrix (damping_sampling.h:282-288): d_ij = 1 - sqrt((1 - g_i)(1 - g_j)), g = value * w
+static Vec make_contact_damping_matrix(const Vec& g)
+{
+    const size_t n = g.size();
+    Vec m(n * n);
+    for (size_t i = 0; i < n; ++i) {
+        for (size_t j = 0; j < n; ++j) {
+            m[i * n + j] = 1.0 - std::sqrt((1.0 - g[i]) * (1.0 - g[j]));
+        }
+    }
+    return m;
+}
+
+// get_active_damping (dynamic_npis.h:291-303): coefficients of the last damping of (level, type) with
+// time <= t, zeros if none
+static Vec get_active_damping(const Dampings& d, int level, int type, double t)
+{
+    const auto& l = d.list();
+    for (auto it = l.rbegin(); it != l.rend(); ++it) {
+        if (it->level == level && it->type == type && it->time <= t) {
+            return it->coeffs;
+        }
+    }
+    return Vec(size_t(d.rows()) * d.cols(), 0.0);
+}
+
+// get_damping_indices (dynamic_npis.h:260-272): dampings of (level, type) with begin < time < end
+static std::vector<size_t> get_damping_indices(const Dampings& d, int level, int type, double begin, double end)
+{
+    std::vector<size_t> idx;
+    const auto& l = d.list();
+    for (size_t i = 0; i < l.size(); ++i) {
+        if (l[i].level == level && l[i].type == type && l[i].time > begin && l[i].time < end) {
+            idx.push_back(i);
+        }
+    }
+    return idx;
+}
+
+// implement_dynamic_npis (dynamic_npis.h:318-392) on the dampings of a group of matrices. make_matrix:
+// 0 = the scaled weight vector itself (reshaped), 1 = make_contact_damping_matrix.
+static void implement_dynamic_npis(std::vector<Dampings*>& group, const std::vector<NpiDamping>& npis, double begin,
+                                   double end, int make_matrix)
+{
+    for (auto& npi : npis) {
+        for (int mat_idx : npi.mats) {
+            Dampings& expr = *group[size_t(mat_idx)];
+            Vec active     = get_active_damping(expr, npi.level, npi.type, begin);
+            Vec active_end = get_active_damping(expr, npi.level, npi.type, end);
+            Vec g(npi.weights.size());
+            for (size_t k = 0; k < g.size(); ++k) {
+                g[k] = npi.value * npi.weights[k];
+            }
+            Vec value = make_matrix == 1 ? make_contact_damping_matrix(g) : g;
+            bool implemented = false;
+            bool any_less    = false;
+            for (size_t k = 0; k < value.size(); ++k) {
+                any_less = any_less || (active[k] < value[k]);
+            }
+            if (any_less) {
+                Damping d{Vec(value.size()), npi.level, npi.type, begin};
+                for (size_t k = 0; k < value.size(); ++k) {
+                    d.coeffs[k] = std::max(value[k], active[k]);
+                }
+                expr.add(d);
+                implemented = true;
+            }
+            for (size_t i : get_damping_indices(expr, npi.level, npi.type, begin, end)) {
+                const Damping& cur = expr.list()[i];
+                Damping d{Vec(value.size()), npi.level, npi.type, cur.time};
+                for (size_t k = 0; k < value.size(); ++k) {
+                    d.coeffs[k] = std::max(cur.coeffs[k], value[k]);
+                }
+                expr.add(d);
+                implemented = true;
+            }
+            if (implemented) {
+                expr.add(Damping{active_end, npi.level, npi.type, end});
+            }
+        }
+    }
+    // remove a damping whose previous damping of the same (level, type) has the same coefficients
+    for (Dampings* expr : group) {
+        for (int i = 0; i < int(expr->list().size()) - 1; ++i) {
+            const auto& l    = expr->list();
+            const size_t cur = l.size() - 1 - size_t(i);
+            for (size_t p = cur; p-- > 0;) {
+                if (l[p].level == l[cur].level && l[p].type == l[cur].type) {
+                    if (l[p].coeffs == l[cur].coeffs) {
+                        expr->remove(cur);
+                    }
+                    break;
+                }
+            }
+        }
+    }
+}
+
+// DynamicNPIs (epidemiology/dynamic_npis.h:36-242)
+struct DynamicNPIs {
+    std::vector<std::pair<double, std::vector<NpiDamping>>> thresholds; // sorted by value, descending
+    double duration = 14.0, delay = 0.0, base = 1.0;
+    double begin = std::numeric_limits<double>::lowest(), end = std::numeric_limits<double>::max();
+    // set_threshold: insert_sorted_replace with t1.first > t2.first (utils/stl_util.h:82-96)
+    void set_threshold(double v, const std::vector<NpiDamping>& d)
+    {
+        auto it = thresholds.begin();
+        while (it != thresholds.end() && it->first > v) {
+            ++it;
+        }
+        if (it != thresholds.end() && !(v > it->first)) {
+            it->second = d;
+        }
+        else {
+            thresholds.insert(it, std::make_pair(v, d));
+        }
+    }
+    // get_max_exceeded_threshold: first threshold (descending) below value; -1 if none
+    int max_exceeded(double value) const
+    {
+        for (size_t k = 0; k < thresholds.size(); ++k) {
+            if (value > thresholds[k].first) {
+                return int(k);
+            }
+        }
+        return -1;
+    }
+};
+
+// floating_point_greater_equal(v1, v2, abs_tol) (math/floating_point.h:96-100), rel_tol = DBL_MIN
+inline bool floating_point_greater_equal(double v1, double v2, double abs_tol)
+{
+    const double diff = v1 - v2;
+    return !(diff < -(abs_tol + std::numeric_limits<double>::min() * std::max(std::abs(v1), std::abs(v2))));
+}
+
 // ------------------------------------------------------------------------------------------ models
 struct Model {
     virtual ~Model() = default;
@@ -258,6 +425,7 @@ struct OsecirModel : Model {
     double nondetection = 0, start_detection = 0, end_detection = 0;
     Vec T_E, T_INS, T_ISy, T_ISev, T_ICr, tp, rel, R, risk, max_risk, sev, cps, dps, dpc;
     ContactGroup contacts;
+    DynamicNPIs npis; // DynamicNPIsInfectedSymptoms (models/ode_secir/parameters.h:342-355)
     mutable Vec flows;
 
     explicit OsecirModel(int g)
@@ -881,14 +1049,76 @@ struct Simulation {
     {
         result.add_time_point(t0, model->initial.data());
     }
+    bool frozen = false;                 // the reference tests' MockSimulation: state constant over time
+    double npi_value = std::numeric_limits<double>::lowest(), npi_end = 0.0; // m_dynamic_npi
     // mio::Simulation::advance: RHS = model.eval_right_hand_side(y, y, t, dydt)
-    void advance(double tmax)
+    void base_advance(double tmax)
     {
+        if (frozen) {
+            result.add_time_point(tmax, result.last_value());
+            return;
+        }
         const Model* m = model.get();
         DerivFunction f = [m](const double* y, double t, double* dydt) {
             m->get_derivatives(y, y, t, dydt);
         };
         system_integrator_advance(*core, f, tmax, dt, result, &steps);
+    }
+    // osecir::Simulation::advance (models/ode_secir/model.h:289-367): dynamic NPIs checked at the start of
+    // every piece of at most one day, pieces clamped at the end of the active NPI
+    void advance(double tmax)
+    {
+        auto* om = dynamic_cast<OsecirModel*>(model.get());
+        if (!om || om->npis.thresholds.empty()) {
+            base_advance(tmax);
+            return;
+        }
+        DynamicNPIs& dn = om->npis;
+        double t        = result.times.back();
+        while (t < tmax) {
+            const double delay = t > 0 ? dn.delay : 0.0;
+            if (floating_point_greater_equal(t, dn.begin, 1e-10) && t < dn.end) {
+                const double inf_rel = infections_relative(*om, result.last_value()) * dn.base;
+                const int k          = dn.max_exceeded(inf_rel);
+                const bool expired   = floating_point_greater_equal(t, npi_end, 1e-10);
+                if (k >= 0 && (dn.thresholds[size_t(k)].first > npi_value || expired)) {
+                    const bool renewal = expired && !(dn.thresholds[size_t(k)].first > npi_value);
+                    const double eff   = renewal ? 0.0 : delay;
+                    if (t + eff < dn.end) {
+                        const double t_start = t + eff;
+                        const double t_end   = std::min(dn.end, t_start + dn.duration);
+                        npi_value            = dn.thresholds[size_t(k)].first;
+                        npi_end              = t_end;
+                        const double tsd     = (!renewal && t_start > 0.0) ? t_start + 1.0 : t_start;
+                        std::vector<Dampings*> group;
+                        for (auto& cm : om->contacts.mats) {
+                            group.push_back(&cm.dampings);
+                        }
+                        implement_dynamic_npis(group, dn.thresholds[size_t(k)].second, tsd, t_end + 1.0, 1);
+                    }
+                }
+            }
+            double dt_eff = std::min(1.0, tmax - t);
+            if (t < npi_end) {
+                dt_eff = std::min(dt_eff, npi_end - t);
+            }
+            base_advance(t + dt_eff);
+            t = t + dt_eff;
+        }
+    }
+    // get_infections_relative (models/ode_secir/model.h:413-423): sum of InfectedSymptoms over the age
+    // groups / the model's total population
+    static double infections_relative(const OsecirModel& m, const double* y)
+    {
+        double sum_inf = 0.0;
+        for (int g = 0; g < m.G; ++g) {
+            sum_inf += y[g * NC + cISy];
+        }
+        double total = 0.0;
+        for (double v : m.initial) {
+            total += v;
+        }
+        return sum_inf / total;
     }
 };
 
@@ -1153,6 +1383,27 @@ static bool build_osecir(const mgo_graph_desc* g, int k, OsecirModel& m)
     }
     m.contacts.mats.clear();
     m.contacts.mats.push_back(cm);
+    if (const mgo_dynamic_npis* dn = g->dynamic_npis) {
+        const double* par = dn->node_params + size_t(k) * MGO_NPI_NUM_PARAMS;
+        m.npis.duration   = par[MGO_NPI_DURATION];
+        m.npis.delay      = par[MGO_NPI_DELAY];
+        m.npis.base       = par[MGO_NPI_BASE_VALUE];
+        m.npis.begin      = par[MGO_NPI_DIRECTIVE_BEGIN];
+        m.npis.end        = par[MGO_NPI_DIRECTIVE_END];
+        for (int t = dn->node_offsets[k]; t < dn->node_offsets[k + 1]; ++t) {
+            std::vector<NpiDamping> ds;
+            for (int d = dn->damping_offsets[t]; d < dn->damping_offsets[t + 1]; ++d) {
+                NpiDamping nd;
+                nd.value = dn->damping_values[d];
+                nd.level = dn->damping_levels[d];
+                nd.type  = dn->damping_types[d];
+                nd.mats  = {0};
+                nd.weights.assign(dn->damping_weights + size_t(d) * G, dn->damping_weights + size_t(d + 1) * G);
+                ds.push_back(nd);
+            }
+            m.npis.set_threshold(dn->thresholds[t], ds);
+        }
+    }
     const double* p = g->populations + size_t(k) * G * NC;
     m.initial.assign(p, p + size_t(G) * NC);
     return true;
@@ -1441,6 +1692,84 @@ int32_t orc_integrate_test_problem(int32_t problem, const mgo_integrator_cfg* cf
         values[i] = ts.value(i)[0];
     }
     return ts.size();
+}
+
+// implement_dynamic_npis on a group of `num_mats` damping lists of rows x cols matrices (the known-answer
+// scenarios of cpp/tests/test_dynamic_npis.cpp:145-317). Lists in/out as CSR over the matrices: entry e has
+// times[e], levels[e], types[e], coeffs[e][rows*cols]. npis: value, level, type, matrix indices
+// (npi_mat_offsets CSR into npi_mats) and weights[num_weights]. make_matrix: 0 = the scaled weights,
+// 1 = make_contact_damping_matrix. Returns the number of output entries (or a negative status).
+int32_t orc_implement_npis(int32_t rows, int32_t cols, int32_t num_mats, const int32_t* list_offsets,
+                           const double* times, const int32_t* levels, const int32_t* types, const double* coeffs,
+                           int32_t num_npis, const double* npi_values, const int32_t* npi_levels,
+                           const int32_t* npi_types, const int32_t* npi_mat_offsets, const int32_t* npi_mats,
+                           int32_t num_weights, const double* npi_weights, double begin, double end,
+                           int32_t make_matrix, int32_t capacity, int32_t* out_offsets, double* out_times,
+                           int32_t* out_levels, int32_t* out_types, double* out_coeffs)
+{
+    const size_t rc = size_t(rows) * cols;
+    std::vector<Dampings> mats;
+    for (int m = 0; m < num_mats; ++m) {
+        mats.emplace_back(rows, cols);
+        for (int e = list_offsets[m]; e < list_offsets[m + 1]; ++e) {
+            mats.back().add(Damping{Vec(coeffs + e * rc, coeffs + (e + 1) * rc), levels[e], types[e], times[e]});
+        }
+    }
+    std::vector<NpiDamping> npis;
+    for (int k = 0; k < num_npis; ++k) {
+        NpiDamping d;
+        d.value = npi_values[k];
+        d.level = npi_levels[k];
+        d.type  = npi_types[k];
+        d.mats.assign(npi_mats + npi_mat_offsets[k], npi_mats + npi_mat_offsets[k + 1]);
+        d.weights.assign(npi_weights + size_t(k) * num_weights, npi_weights + size_t(k + 1) * num_weights);
+        npis.push_back(d);
+    }
+    std::vector<Dampings*> group;
+    for (auto& m : mats) {
+        group.push_back(&m);
+    }
+    implement_dynamic_npis(group, npis, begin, end, make_matrix);
+    int32_t n = 0;
+    out_offsets[0] = 0;
+    for (int m = 0; m < num_mats; ++m) {
+        for (const Damping& d : mats[size_t(m)].list()) {
+            if (n < capacity) {
+                out_times[n]  = d.time;
+                out_levels[n] = d.level;
+                out_types[n]  = d.type;
+                std::copy(d.coeffs.begin(), d.coeffs.end(), out_coeffs + size_t(n) * rc);
+            }
+            ++n;
+        }
+        out_offsets[m + 1] = n;
+    }
+    return n;
+}
+
+// osecir::Simulation::advance(tmax) with dynamic NPIs over the reference tests' MockSimulation (state constant
+// over time, cpp/tests/test_dynamic_npis.cpp:379-416) for node 0 of `model`, started at t0. Writes the
+// effective contact matrix entry (0, 0) at each query time and the number of dampings of the matrix.
+int32_t orc_npi_frozen_run(const mgo_graph_desc* model, double t0, double tmax, int32_t num_query,
+                           const double* query_times, double* out_c00, int32_t* out_num_dampings)
+{
+    if (!valid_desc(model)) {
+        return MGO_ERR_INVALID_VALUE;
+    }
+    OsecirModel m(model->num_groups);
+    build_osecir(model, 0, m);
+    mgo_integrator_cfg ic{MGO_INTEGRATOR_EULER, 0, 0, 0, 0};
+    Simulation sim(m, make_core(ic), t0, 0.1);
+    sim.frozen = true;
+    sim.advance(tmax);
+    auto* om = dynamic_cast<OsecirModel*>(sim.model.get());
+    for (int q = 0; q < num_query; ++q) {
+        out_c00[q] = om->contacts.at(query_times[q], 0, 0);
+    }
+    if (out_num_dampings) {
+        *out_num_dampings = int32_t(om->contacts.mats[0].dampings.list().size());
+    }
+    return MGO_OK;
 }
 
 double orc_contact_at(const mgo_graph_desc* model, double t, int32_t i, int32_t j)

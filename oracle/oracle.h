@@ -73,6 +73,21 @@ int32_t orc_edge_scenario(const mgo_graph_desc* model, double t, const double* c
 int32_t orc_integrate_test_problem(int32_t problem, const mgo_integrator_cfg* core, double t0, double tmax,
                                    double dt, int32_t capacity, double* times, double* values);
 
+/* implement_dynamic_npis (cpp/memilio/epidemiology/dynamic_npis.h:318-392) on a group of damping lists
+ * (known-answer scenarios of cpp/tests/test_dynamic_npis.cpp); see oracle.cpp for the argument layout. */
+int32_t orc_implement_npis(int32_t rows, int32_t cols, int32_t num_mats, const int32_t* list_offsets,
+                           const double* times, const int32_t* levels, const int32_t* types, const double* coeffs,
+                           int32_t num_npis, const double* npi_values, const int32_t* npi_levels,
+                           const int32_t* npi_types, const int32_t* npi_mat_offsets, const int32_t* npi_mats,
+                           int32_t num_weights, const double* npi_weights, double begin, double end,
+                           int32_t make_matrix, int32_t capacity, int32_t* out_offsets, double* out_times,
+                           int32_t* out_levels, int32_t* out_types, double* out_coeffs);
+
+/* osecir::Simulation::advance with dynamic NPIs over a constant-state simulation (the reference tests'
+ * MockSimulation): contact(0,0) of node 0 at the query times afterwards, and its number of dampings. */
+int32_t orc_npi_frozen_run(const mgo_graph_desc* model, double t0, double tmax, int32_t num_query,
+                           const double* query_times, double* out_c00, int32_t* out_num_dampings);
+
 /* Effective contact matrix entry of node 0 at time t (ContactMatrixGroup::get_matrix_at(t)(i, j)). */
 double orc_contact_at(const mgo_graph_desc* model, double t, int32_t i, int32_t j);
 

@@ -58,8 +58,66 @@ def lib():
                                                  C.c_double, C.c_int32, _dp, _dp]
         L.orc_contact_at.restype = C.c_double
         L.orc_contact_at.argtypes = [C.POINTER(capi.GraphDesc), C.c_double, C.c_int32, C.c_int32]
+        L.orc_implement_npis.restype = C.c_int32
+        L.orc_implement_npis.argtypes = [C.c_int32, C.c_int32, C.c_int32, _ip, _dp, _ip, _ip, _dp, C.c_int32, _dp, _ip,
+                                         _ip, _ip, _ip, C.c_int32, _dp, C.c_double, C.c_double, C.c_int32, C.c_int32,
+                                         _ip, _dp, _ip, _ip, _dp]
+        L.orc_npi_frozen_run.restype = C.c_int32
+        L.orc_npi_frozen_run.argtypes = [C.POINTER(capi.GraphDesc), C.c_double, C.c_double, C.c_int32, _dp, _dp, _ip]
         _lib = L
     return _lib
+
+
+def implement_npis(rows, cols, lists, npis, begin, end, make_matrix=0, capacity=256):
+    """implement_dynamic_npis on a group of damping lists.
+
+    lists: per matrix, list of (time, level, type, coeffs[rows*cols]); npis: list of (value, level, type,
+    matrix indices, weights). Returns the lists afterwards in the same form."""
+    rc = rows * cols
+    offs, times, levels, types, coeffs = [0], [], [], [], []
+    for lst in lists:
+        for (t, lev, typ, c) in lst:
+            times.append(t)
+            levels.append(lev)
+            types.append(typ)
+            coeffs.append(np.broadcast_to(np.asarray(c, np.float64), (rc,)))
+        offs.append(len(times))
+    nw = len(npis[0][4]) if npis else 0
+    moffs, mats = [0], []
+    for n in npis:
+        mats += list(n[3])
+        moffs.append(len(mats))
+    i32 = lambda v: np.ascontiguousarray(v, dtype=np.int32)       # noqa: E731
+    f64 = lambda v: np.ascontiguousarray(v, dtype=np.float64)     # noqa: E731
+    a = dict(offs=i32(offs), times=f64(times), levels=i32(levels), types=i32(types),
+             coeffs=f64(np.stack(coeffs) if coeffs else np.zeros((0, rc))),
+             nv=f64([n[0] for n in npis]), nl=i32([n[1] for n in npis]), nt=i32([n[2] for n in npis]),
+             moffs=i32(moffs), mats=i32(mats), w=f64(np.stack([np.asarray(n[4], np.float64) for n in npis])))
+    out_offs = np.zeros(len(lists) + 1, np.int32)
+    ot, ol, oty = np.zeros(capacity), np.zeros(capacity, np.int32), np.zeros(capacity, np.int32)
+    oc = np.zeros((capacity, rc))
+    n = lib().orc_implement_npis(rows, cols, len(lists), capi.iptr(a["offs"]), capi.dptr(a["times"]),
+                                 capi.iptr(a["levels"]), capi.iptr(a["types"]), capi.dptr(a["coeffs"]), len(npis),
+                                 capi.dptr(a["nv"]), capi.iptr(a["nl"]), capi.iptr(a["nt"]), capi.iptr(a["moffs"]),
+                                 capi.iptr(a["mats"]), nw, capi.dptr(a["w"]), begin, end, make_matrix, capacity,
+                                 capi.iptr(out_offs), capi.dptr(ot), capi.iptr(ol), capi.iptr(oty), capi.dptr(oc))
+    if n < 0 or n > capacity:
+        raise RuntimeError(f"orc_implement_npis: {n}")
+    return [[(float(ot[e]), int(ol[e]), int(oty[e]), oc[e].copy()) for e in range(out_offs[m], out_offs[m + 1])]
+            for m in range(len(lists))]
+
+
+def npi_frozen_run(spec, t0, tmax, query_times):
+    """Simulation::advance(tmax) with dynamic NPIs on a constant state from t0: contact(0, 0) of node 0 at the
+    query times afterwards and the number of dampings of its contact matrix."""
+    q = np.ascontiguousarray(query_times, dtype=np.float64)
+    out = np.zeros(len(q))
+    nd = np.zeros(1, np.int32)
+    st = lib().orc_npi_frozen_run(C.byref(spec.desc()), t0, tmax, len(q), capi.dptr(q), capi.dptr(out),
+                                  capi.iptr(nd))
+    if st != 0:
+        raise RuntimeError(f"orc_npi_frozen_run: {st}")
+    return out, int(nd[0])
 
 
 def simulate(spec, cfg, tmax, capacity=200000):

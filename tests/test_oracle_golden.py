@@ -158,3 +158,112 @@ def test_ensemble_analysis_known_answers(oracle):
     for p, a, b in ((0.2, 0.0, 0.0), (0.4, 1.0, 0.1), (0.7, 2.0, 0.2), (0.9, 3.0, 0.3)):
         q = oracle.ensemble_percentile(runs, p)
         assert q[1, 0, 1] == a and q[0, 1, 0] == b, p
+
+
+# ------------------------------------------------------------------------------------------ dynamic NPIs
+# Known answers of cpp/tests/test_dynamic_npis.cpp (the scenarios restated as data).
+
+def _damping_list(lst):
+    return [(t, lev, typ, np.round(np.asarray(c, float), 12).tolist()) for (t, lev, typ, c) in lst]
+
+
+def test_npi_implement_empty(oracle):
+    """test_dynamic_npis.cpp:145-170: two empty vector-shaped lists, NPI 0.8 on both over [0.45, 0.6]."""
+    out = oracle.implement_npis(2, 1, [[], []], [(0.8, 0, 0, [0, 1], [1.0, 1.0])], 0.45, 0.6, make_matrix=0)
+    for lst in out:
+        assert _damping_list(lst) == [(0.45, 0, 0, [0.8, 0.8]), (0.6, 0, 0, [0.0, 0.0])]
+
+
+def test_npi_implement(oracle):
+    """test_dynamic_npis.cpp:172-262: existing dampings before, during and after the NPI, other types and
+    levels, a second NPI overlapping the first."""
+    lists = [[(0.3, 0, 0, 0.15), (0.4, 0, 0, 0.2), (0.5, 0, 1, 0.25), (0.7, 0, 0, 0.35)],
+             [(0.5, 1, 1, 0.25), (0.9, 0, 0, 0.45)]]
+    lists = [[(t, lev, typ, [c] * 3) for (t, lev, typ, c) in lst] for lst in lists]
+    out = oracle.implement_npis(3, 1, lists, [(0.4, 0, 0, [0, 1], [1.0, 1.0, 1.0])], 0.45, 0.6, make_matrix=0)
+    v = lambda c: [c] * 3   # noqa: E731
+    assert _damping_list(out[0]) == [(0.3, 0, 0, v(0.15)), (0.4, 0, 0, v(0.2)), (0.45, 0, 0, v(0.4)),
+                                     (0.5, 0, 1, v(0.25)), (0.6, 0, 0, v(0.2)), (0.7, 0, 0, v(0.35))]
+    assert _damping_list(out[1]) == [(0.45, 0, 0, v(0.4)), (0.5, 1, 1, v(0.25)), (0.6, 0, 0, v(0.0)),
+                                     (0.9, 0, 0, v(0.45))]
+    out2 = oracle.implement_npis(3, 1, out, [(0.3, 0, 0, [0], [1.0, 1.0, 1.0])], 0.3, 0.9, make_matrix=0)
+    assert _damping_list(out2[0]) == [(0.3, 0, 0, v(0.3)), (0.45, 0, 0, v(0.4)), (0.5, 0, 1, v(0.25)),
+                                      (0.6, 0, 0, v(0.3)), (0.7, 0, 0, v(0.35))]
+    assert _damping_list(out2[1]) == _damping_list(out[1])
+
+
+def _npi_model(isy, threshold, base, value=1.0, duration=5.0, delay=0.0, begin=None, end=None, contact=None):
+    """osecir Model(1) of the secir_* tests: InfectedSymptoms = isy, Susceptible = 100 - isy."""
+    from memilio_amd.graphspec import GraphSpec, ISY, S
+    spec = GraphSpec(1, 1)
+    spec.populations[0, 0, ISY] = isy
+    spec.populations[0, 0, S] = 100.0 - isy
+    spec.contact_baseline[0] = 1.0 if contact is None else contact
+    cfg = {"thresholds": [(threshold, [(value, 0, 0, [1.0])])], "duration": duration, "delay": delay,
+           "base_value": base}
+    if begin is not None:
+        cfg["directive_begin"] = begin
+    if end is not None:
+        cfg["directive_end"] = end
+    spec.set_dynamic_npis([cfg])
+    return spec
+
+
+def test_npi_secir_threshold(oracle):
+    """test_dynamic_npis.cpp:421-482: below the threshold nothing happens; above it two dampings appear."""
+    _, n = oracle.npi_frozen_run(_npi_model(1.0, 0.05 * 23000, 23000), 0.0, 3.0, [0.0])
+    assert n == 0
+    _, n = oracle.npi_frozen_run(_npi_model(10.0, 0.05 * 50000, 50000), 0.0, 3.0, [0.0])
+    assert n == 2
+
+
+def test_npi_secir_delayed_implementation(oracle):
+    """test_dynamic_npis.cpp:484-539."""
+    c, _ = oracle.npi_frozen_run(_npi_model(10.0, 2500, 50000, 0.5, delay=3.0), 0.0, 3.0, [0.0, 1.0, 3.0])
+    assert c.tolist() == [0.5, 0.5, 0.5]
+    c, _ = oracle.npi_frozen_run(_npi_model(10.0, 2500, 50000, 0.5, delay=2.0), 1.0, 5.0, [3.0, 4.0])
+    assert c.tolist() == [1.0, 0.5]
+    c, _ = oracle.npi_frozen_run(_npi_model(10.0, 2500, 50000, 0.5, delay=10.0), 1.0, 4.0, [11.0, 12.0])
+    assert c.tolist() == [1.0, 0.5]
+
+
+def test_npi_secir_directives(oracle):
+    """test_dynamic_npis.cpp:541-628."""
+    m = lambda **kw: _npi_model(10.0, 2500, 50000, 0.5, **kw)   # noqa: E731
+    c, _ = oracle.npi_frozen_run(m(begin=5.0), 0.0, 3.0, [0.0, 1.0, 3.0])
+    assert c.tolist() == [1.0, 1.0, 1.0]
+    c, _ = oracle.npi_frozen_run(m(delay=2.0, begin=0.0), 0.0, 3.0, [0.0, 4.0, 6.0])
+    assert c.tolist() == [0.5, 0.5, 1.0]
+    c, _ = oracle.npi_frozen_run(m(delay=2.0, begin=0.0, end=3.0), 0.0, 5.0, [0.0, 2.0, 4.0])
+    assert c.tolist() == [0.5, 0.5, 1.0]
+    c, _ = oracle.npi_frozen_run(m(delay=2.0, begin=0.0, end=1e6), 1.0, 5.0, [3.0, 4.0, 8.0, 9.0])
+    assert c.tolist() == [1.0, 0.5, 0.5, 1.0]
+    c, _ = oracle.npi_frozen_run(m(delay=2.0, begin=0.0, end=4.0), 1.0, 6.0, [1.0, 4.0, 5.0])
+    assert c.tolist() == [1.0, 0.5, 1.0]
+
+
+def test_npi_secir_backward_consistency_and_keep_alive(oracle):
+    """test_dynamic_npis.cpp:630-722: the dynamic NPI equals predefined dampings at t_start+1 / t_end+1;
+    an NPI that expires while the threshold is still exceeded is renewed without a dip."""
+    ts = [1.0, 2.0, 3.0, 3.5, 4.0, 5.0, 7.0, 8.0, 8.5, 9.0]
+    c, _ = oracle.npi_frozen_run(_npi_model(10.0, 2500, 50000, 0.5, delay=2.0), 1.0, 8.0, ts)
+    pre = _npi_model(10.0, 2500, 50000, 0.5, delay=2.0)
+    pre.set_dampings([[(4.0, 0, 0, 0.5), (9.0, 0, 0, 0.0)]])
+    pre.npis = None
+    pre._desc = None
+    expect = [oracle.contact_at(pre, t, 0, 0) for t in ts]
+    assert c.tolist() == expect
+    c, _ = oracle.npi_frozen_run(_npi_model(10.0, 2500, 50000, 0.5, delay=2.0), 1.0, 12.0,
+                                 [4.0, 8.0, 9.0, 10.0, 13.0, 14.0])
+    assert c.tolist() == [0.5, 0.5, 0.5, 0.5, 0.5, 1.0]
+
+
+def test_npi_scenario_triggers(oracle):
+    """The graph scenario of the device NPI tests implements NPIs (guards against a vacuous comparison)."""
+    for kw in (dict(duration=14.0, delay=3.0), dict(duration=2.3, delay=0.7, directive_begin=2.0,
+                                                      directive_end=40.2)):
+        g = synthetic.with_npis(synthetic.graph(num_nodes=30, G=3, seed=61, mean_degree=6.0), **kw)
+        g0 = synthetic.graph(num_nodes=30, G=3, seed=61, mean_degree=6.0)
+        a, _ = oracle.graph_run(g, synthetic.fixed_cfg(0.1), 60.0)
+        b, _ = oracle.graph_run(g0, synthetic.fixed_cfg(0.1), 60.0)
+        assert np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-6)) > 1e-2
