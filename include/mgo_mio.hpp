@@ -676,6 +676,131 @@ public:
 };
 
 // ---------------------------------------------------------------------------------------------- osecir
+// DampingSampling (cpp/memilio/epidemiology/damping_sampling.h:40-230): one damping a dynamic NPI adds,
+// value * group weights on the listed matrices of the contact matrix group.
+template <class FP = double>
+class DampingSampling
+{
+public:
+    template <class V>
+    DampingSampling(double value, DampingLevel level, DampingType type, SimulationTime time,
+                    std::vector<size_t> matrix_indices, const V& group_weights)
+        : m_value(value)
+        , m_level(level)
+        , m_type(type)
+        , m_time(time)
+        , m_mats(std::move(matrix_indices))
+        , m_weights(detail::to_vector(group_weights))
+    {
+    }
+    double get_value() const
+    {
+        return m_value;
+    }
+    DampingLevel get_level() const
+    {
+        return m_level;
+    }
+    DampingType get_type() const
+    {
+        return m_type;
+    }
+    SimulationTime get_time() const
+    {
+        return m_time;
+    }
+    const std::vector<size_t>& get_matrix_indices() const
+    {
+        return m_mats;
+    }
+    const Vector& get_group_weights() const
+    {
+        return m_weights;
+    }
+
+private:
+    double m_value;
+    DampingLevel m_level;
+    DampingType m_type;
+    SimulationTime m_time;
+    std::vector<size_t> m_mats;
+    Vector m_weights;
+};
+
+// DynamicNPIs (cpp/memilio/epidemiology/dynamic_npis.h:36-242): thresholds (cases per base value, sorted
+// descending, an equal threshold replaced) and the NPIs they trigger; duration, implementation delay and
+// the directive interval in which NPIs can start.
+template <class FP = double>
+class DynamicNPIs
+{
+public:
+    void set_threshold(double threshold, const std::vector<DampingSampling<FP>>& dampings)
+    {
+        auto it = m_thresholds.begin();
+        while (it != m_thresholds.end() && it->first > threshold) {
+            ++it;
+        }
+        if (it != m_thresholds.end() && !(threshold > it->first)) {
+            it->second = dampings;
+        }
+        else {
+            m_thresholds.insert(it, std::make_pair(threshold, dampings));
+        }
+    }
+    const std::vector<std::pair<double, std::vector<DampingSampling<FP>>>>& get_thresholds() const
+    {
+        return m_thresholds;
+    }
+    SimulationTime get_duration() const
+    {
+        return m_duration;
+    }
+    void set_duration(SimulationTime v)
+    {
+        m_duration = v;
+    }
+    SimulationTime get_implementation_delay() const
+    {
+        return m_delay;
+    }
+    void set_implementation_delay(SimulationTime v)
+    {
+        m_delay = v;
+    }
+    double get_base_value() const
+    {
+        return m_base;
+    }
+    void set_base_value(double v)
+    {
+        m_base = v;
+    }
+    SimulationTime get_directive_begin() const
+    {
+        return m_begin;
+    }
+    void set_directive_begin(SimulationTime v)
+    {
+        m_begin = v;
+    }
+    SimulationTime get_directive_end() const
+    {
+        return m_end;
+    }
+    void set_directive_end(SimulationTime v)
+    {
+        m_end = v;
+    }
+
+private:
+    std::vector<std::pair<double, std::vector<DampingSampling<FP>>>> m_thresholds;
+    SimulationTime m_duration{14.0};
+    SimulationTime m_delay{0.0};
+    SimulationTime m_begin{std::numeric_limits<double>::lowest()};
+    SimulationTime m_end{std::numeric_limits<double>::max()};
+    double m_base = 1.0;
+};
+
 namespace osecir
 {
 
@@ -774,6 +899,15 @@ struct ContactPatterns {
     }
 };
 
+// DynamicNPIsInfectedSymptoms (parameters.h:342-355)
+template <class FP = double>
+struct DynamicNPIsInfectedSymptoms {
+    static std::string name()
+    {
+        return "DynamicNPIsInfectedSymptoms";
+    }
+};
+
 template <class FP = double>
 class Parameters
 {
@@ -807,6 +941,9 @@ public:
         else if constexpr (requires { Tag::age_index; }) {
             return (m_ages[Tag::age_index]);
         }
+        else if constexpr (std::is_same_v<Tag, DynamicNPIsInfectedSymptoms<FP>>) {
+            return (m_npis);
+        }
         else {
             static_assert(std::is_same_v<Tag, ContactPatterns<FP>>, "parameter not on the device path");
             return (m_contacts);
@@ -820,6 +957,9 @@ public:
         }
         else if constexpr (requires { Tag::age_index; }) {
             return (std::as_const(m_ages[Tag::age_index]));
+        }
+        else if constexpr (std::is_same_v<Tag, DynamicNPIsInfectedSymptoms<FP>>) {
+            return (std::as_const(m_npis));
         }
         else {
             static_assert(std::is_same_v<Tag, ContactPatterns<FP>>, "parameter not on the device path");
@@ -869,6 +1009,7 @@ private:
     std::array<double, MGO_NUM_SCALARS> m_scalars;
     std::vector<AgeArray> m_ages;
     ContactMatrixGroup m_contacts;
+    mgo_mio::DynamicNPIs<FP> m_npis;
 };
 
 // Compartment values indexed by {AgeGroup, InfectionState}; flat index g*10 + state.
@@ -1225,6 +1366,10 @@ struct GraphArrays {
     int n = 0, G = 0;
     std::vector<double> scalars, ages, cbase, cmin, dtimes, dcoeffs, pops, ecoeffs;
     std::vector<int32_t> doffs, dlevels, dtypes, estart, eend;
+    // dynamic NPIs (mgo_dynamic_npis)
+    std::vector<double> npi_params, npi_thr, npi_vals, npi_weights;
+    std::vector<int32_t> npi_offs, npi_doffs, npi_levels, npi_types;
+    std::unique_ptr<mgo_dynamic_npis> npis; // on the heap: desc keeps pointing at it when the arrays move
     mgo_integrator_cfg integrator{};
     double dt_node = 0.1;
     mgo_graph_desc desc{};
@@ -1250,12 +1395,14 @@ struct GraphArrays {
                 ages.push_back(p.age_param(k)[AgeGroup(size_t(a))]);
             }
         }
-        // fold the contact matrix group into one matrix: at most one matrix may carry dampings
+        // fold the contact matrix group into one matrix: at most one matrix may carry dampings (static or
+        // added by dynamic NPIs)
         const ContactMatrixGroup& cg = p.template get<osecir::ContactPatterns<FP>>();
-        int damped                   = -1;
+        int damped                   = add_npis(p.template get<osecir::DynamicNPIsInfectedSymptoms<FP>>(),
+                                                int(cg.get_num_matrices()));
         for (size_t k = 0; k < cg.get_num_matrices(); ++k) {
             if (!cg[k].get_dampings().empty()) {
-                if (damped >= 0) {
+                if (damped >= 0 && damped != int(k)) {
                     throw Error(MGO_ERR_UNSUPPORTED, "more than one damped contact matrix in a node");
                 }
                 damped = int(k);
@@ -1306,6 +1453,43 @@ struct GraphArrays {
         ++n;
     }
 
+    // the node's DynamicNPIsInfectedSymptoms; returns the contact matrix they damp (-1: none)
+    template <class FP>
+    int add_npis(const DynamicNPIs<FP>& dn, int num_mats)
+    {
+        if (npi_offs.empty()) {
+            npi_offs.push_back(0);
+            npi_doffs.push_back(0);
+        }
+        npi_params.push_back(dn.get_duration().get());
+        npi_params.push_back(dn.get_implementation_delay().get());
+        npi_params.push_back(dn.get_base_value());
+        npi_params.push_back(dn.get_directive_begin().get());
+        npi_params.push_back(dn.get_directive_end().get());
+        int target = -1;
+        for (const auto& t : dn.get_thresholds()) {
+            npi_thr.push_back(t.first);
+            for (const auto& d : t.second) {
+                const auto& mi = d.get_matrix_indices();
+                if (mi.size() != 1 || int(mi[0]) >= num_mats || (target >= 0 && int(mi[0]) != target)) {
+                    throw Error(MGO_ERR_UNSUPPORTED, "dynamic NPIs of a node must damp one and the same contact matrix");
+                }
+                if (d.get_group_weights().size() != size_t(G)) {
+                    throw Error(MGO_ERR_INVALID_VALUE, "NPI group weights need one value per age group");
+                }
+                target = int(mi[0]);
+                npi_vals.push_back(d.get_value());
+                npi_levels.push_back(d.get_level().get());
+                npi_types.push_back(d.get_type().get());
+                npi_weights.insert(npi_weights.end(), d.get_group_weights().data(),
+                                   d.get_group_weights().data() + G);
+            }
+            npi_doffs.push_back(int32_t(npi_vals.size()));
+        }
+        npi_offs.push_back(int32_t(npi_thr.size()));
+        return target;
+    }
+
     template <class FP>
     void add_core(const OdeIntegratorCore<FP>& core, double dt)
     {
@@ -1350,6 +1534,18 @@ struct GraphArrays {
         desc.edge_end         = eend.empty() ? nullptr : eend.data();
         desc.edge_coeffs      = ecoeffs.empty() ? nullptr : ecoeffs.data();
         desc.edge_coeff_len   = G * MGO_NUM_COMPARTMENTS;
+        if (!npi_thr.empty()) {
+            npis                  = std::make_unique<mgo_dynamic_npis>();
+            npis->node_offsets    = npi_offs.data();
+            npis->node_params     = npi_params.data();
+            npis->thresholds      = npi_thr.data();
+            npis->damping_offsets = npi_doffs.data();
+            npis->damping_values  = npi_vals.empty() ? nullptr : npi_vals.data();
+            npis->damping_levels  = npi_levels.empty() ? nullptr : npi_levels.data();
+            npis->damping_types   = npi_types.empty() ? nullptr : npi_types.data();
+            npis->damping_weights = npi_weights.empty() ? nullptr : npi_weights.data();
+            desc.dynamic_npis     = npis.get();
+        }
         return desc;
     }
 };

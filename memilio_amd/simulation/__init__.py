@@ -159,6 +159,52 @@ class UncertainContactMatrix:
         self.cont_freq_mat = group
 
 
+class DampingSampling:
+    """mio::DampingSampling (epidemiology/damping_sampling.h:40-230; binding damping_sampling.cpp:33-79): a
+    damping of value * group_weights that a dynamic NPI adds to the listed contact matrices."""
+
+    def __init__(self, value, level, type, time, matrix_indices, group_weights):
+        self.value = value if isinstance(value, UncertainValue) else _FixedValue(float(value))
+        self.level = int(level)
+        self.type = int(type)
+        self.time = float(time)
+        self.matrix_indices = [int(k) for k in matrix_indices]
+        self.group_weights = np.asarray(group_weights, dtype=np.float64).ravel().copy()
+
+
+class _FixedValue(UncertainValue):
+    def __init__(self, v):
+        box = [v]
+        super().__init__(lambda: box[0], lambda x: box.__setitem__(0, float(x)))
+
+
+class DynamicNPIs:
+    """mio::DynamicNPIs (epidemiology/dynamic_npis.h:36-242; binding dynamic_npis.h:35-87): thresholds (cases
+    per base_value, kept in descending order, an equal threshold replaced) and the NPIs they trigger."""
+
+    def __init__(self):
+        self._thresholds = []
+        self.duration = 14.0
+        self.implementation_delay = 0.0
+        self.base_value = 1.0
+        self.directive_begin = -np.finfo(np.float64).max
+        self.directive_end = np.finfo(np.float64).max
+
+    def set_threshold(self, threshold, dampings):
+        threshold = float(threshold)
+        k = 0
+        while k < len(self._thresholds) and self._thresholds[k][0] > threshold:
+            k += 1
+        if k < len(self._thresholds) and not threshold > self._thresholds[k][0]:
+            self._thresholds[k] = (threshold, list(dampings))
+        else:
+            self._thresholds.insert(k, (threshold, list(dampings)))
+
+    @property
+    def threshold(self):
+        return list(self._thresholds)
+
+
 class MobilityParameters:
     """mio::MobilityParameters: one coefficient per compartment and age group (10 * G)."""
 

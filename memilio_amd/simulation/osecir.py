@@ -21,7 +21,7 @@ import numpy as np
 from .. import capi
 from ..engine import Context
 from ..graphspec import GraphSpec, NC
-from . import (AgeGroup, ContactMatrix, ContactMatrixGroup, MobilityParameters, TimeSeries,
+from . import (AgeGroup, ContactMatrix, ContactMatrixGroup, DynamicNPIs, MobilityParameters, TimeSeries,
                UncertainContactMatrix, UncertainValue, interpolate_simulation_result)
 
 __all__ = ["InfectionState", "Model", "ModelGraph", "MobilityGraph", "MobilitySimulation", "GraphParameterStudy",
@@ -140,6 +140,7 @@ class Parameters:
         object.__setattr__(self, "_age", default_age_params(self._G))
         object.__setattr__(self, "ContactPatterns",
                            UncertainContactMatrix(ContactMatrixGroup(1, self._G)))
+        object.__setattr__(self, "DynamicNPIsInfectedSymptoms", DynamicNPIs())
 
     def __getattr__(self, name):
         if name in _SCALARS:
@@ -156,6 +157,10 @@ class Parameters:
             self._age[_AGES[name]][:] = np.asarray(v, dtype=np.float64)
         elif name == "ContactPatterns":
             object.__setattr__(self, name, v if isinstance(v, UncertainContactMatrix) else UncertainContactMatrix(v))
+        elif name == "DynamicNPIsInfectedSymptoms":
+            if not isinstance(v, DynamicNPIs):
+                raise TypeError("DynamicNPIsInfectedSymptoms takes a DynamicNPIs")
+            object.__setattr__(self, name, v)
         else:
             raise AttributeError(f"osecir parameter {name!r} is not on the device path")
 
@@ -387,6 +392,12 @@ def _spec_from(models, edges):
         spec.populations[v] = m.populations._v
         grp = p.ContactPatterns.cont_freq_mat
         damped = [k for k in range(grp.num_matrices) if grp[k].dampings]
+        dn = p.DynamicNPIsInfectedSymptoms
+        targets = {k for _, ds in dn.threshold for d in ds for k in d.matrix_indices}
+        if any(len(d.matrix_indices) != 1 for _, ds in dn.threshold for d in ds) or len(targets) > 1 or \
+                any(k >= grp.num_matrices for k in targets):
+            raise ValueError("dynamic NPIs of a node must damp one and the same contact matrix on the device")
+        damped = sorted(set(damped) | targets)
         if len(damped) > 1:
             raise ValueError("more than one damped contact matrix in a node is not supported on the device")
         B = sum(grp[k].baseline for k in range(grp.num_matrices))
@@ -408,6 +419,15 @@ def _spec_from(models, edges):
         spec.contact_minimum = minima
     if any_damp:
         spec.set_dampings(per_node_damp)
+    npi_cfgs = []
+    for m in models:
+        dn = m.parameters.DynamicNPIsInfectedSymptoms
+        npi_cfgs.append({"thresholds": [(thr, [(float(d.value.value), d.level, d.type, d.group_weights) for d in ds])
+                                        for thr, ds in dn.threshold],
+                         "duration": dn.duration, "delay": dn.implementation_delay, "base_value": dn.base_value,
+                         "directive_begin": dn.directive_begin, "directive_end": dn.directive_end})
+    if any(c["thresholds"] for c in npi_cfgs):
+        spec.set_dynamic_npis(npi_cfgs)
     if edges:
         s = np.array([e[0] for e in edges], np.int32)
         d = np.array([e[1] for e in edges], np.int32)

@@ -336,7 +336,30 @@ int main()
         std::printf("single model RKF45: max rel err %.3e\n", e);
         CHECK(e < 1e-6);
     }
-    // 5. errors surface as mio::Error with the C status
+    // 5. dynamic NPIs (DynamicNPIsInfectedSymptoms) on every node, as the reference's parameter-study examples
+    //    set them: thresholds in cases per 100,000, two NPIs on contact matrix 0 with group weights
+    {
+        auto pg = random_params_graph(16, 3, 21);
+        for (auto& node : pg.nodes()) {
+            mio::DynamicNPIs<double> npis;
+            npis.set_duration(mio::SimulationTime(10.0));
+            npis.set_implementation_delay(mio::SimulationTime(2.0));
+            npis.set_base_value(100000.0);
+            npis.set_threshold(50.0, {mio::DampingSampling<double>(0.4, mio::DampingLevel(0), mio::DampingType(0),
+                                                                   mio::SimulationTime(0), {0},
+                                                                   mio::Vector::Constant(3, 1.0))});
+            npis.set_threshold(300.0, {mio::DampingSampling<double>(0.7, mio::DampingLevel(1), mio::DampingType(0),
+                                                                    mio::SimulationTime(0), {0},
+                                                                    mio::Vector::Constant(3, 0.8))});
+            node.property.parameters.get<mio::osecir::DynamicNPIsInfectedSymptoms<double>>() = npis;
+        }
+        auto sim = mio::make_mobility_sim<double>(0.0, 0.5, sample_graph(pg, 0.0, 0.1, 3, true));
+        sim.advance(30.0);
+        const double e = compare_graph(sim.get_graph(), 0.0, 0.5, 30.0);
+        std::printf("dynamic NPIs, fixed dt: max rel err %.3e\n", e);
+        CHECK(e < 1e-12);
+    }
+    // 6. errors surface as mio::Error with the C status
     {
         Graph g;
         g.add_node(0, mio::osecir::Model<double>(2), 0.0);

@@ -72,3 +72,21 @@ def test_simulate_single_model(oracle):
     cfg = capi.sim_cfg(t0=0.0, dt_node=0.3, integrator=core.cfg())
     ref, _ = oracle.simulate_daily(spec, cfg, 40.0)
     assert rel_err(np.stack(ts.values), ref) < 1e-6
+
+
+def test_mobility_simulation_dynamic_npis(oracle):
+    """The reference's Python mobility example with DynamicNPIsInfectedSymptoms on every node."""
+    graph = ps.mobility_example_graph()
+    for v in range(graph.num_nodes):
+        dn = graph.get_node(v).property.model.parameters.DynamicNPIsInfectedSymptoms
+        dn.implementation_delay = 2.0
+        dn.duration = 7.0
+        dn.base_value = 100000
+        dn.set_threshold(100.0, [mio.DampingSampling(0.5, 0, 0, 0.0, [0], np.ones(1))])
+        dn.set_threshold(500.0, [mio.DampingSampling(0.3, 1, 0, 0.0, [0], np.ones(1))])
+    spec = _spec(graph)
+    sim = osecir.MobilitySimulation(graph, 0.0, dt=0.5)
+    sim.advance(40)
+    cfg = capi.sim_cfg(t0=0.0, dt_graph=0.5, dt_node=0.1)
+    ref, _ = oracle.graph_run(spec, cfg, 40.0)
+    assert rel_err(_results(sim.graph), ref) < 1e-6

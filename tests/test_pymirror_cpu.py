@@ -72,3 +72,35 @@ def test_descriptor_matches_direct_setup(oracle):
     ref, _ = oracle.graph_run(spec, cfg, 10.0)
     assert ref.shape == (11, 2, 10)
     assert abs(ref[-1].sum() - 12000.0) < 1e-6          # population conserved
+
+
+def test_dynamic_npis_descriptor(oracle):
+    """params.DynamicNPIsInfectedSymptoms as the reference's Python examples set it (binding
+    dynamic_npis.h:35-87): descending thresholds, an equal threshold replaced, the descriptor arrays, and the
+    NPIs change the course of the run."""
+    graph = ps.mobility_example_graph()
+    for v in range(graph.num_nodes):
+        p = graph.get_node(v).property.model.parameters
+        dn = p.DynamicNPIsInfectedSymptoms
+        dn.implementation_delay = 2.0
+        dn.duration = 7.0
+        dn.base_value = 100000
+        dn.set_threshold(100.0, [mio.DampingSampling(0.5, 0, 0, 0.0, [0], np.ones(1))])
+        dn.set_threshold(500.0, [mio.DampingSampling(0.3, 1, 0, 0.0, [0], np.ones(1))])
+        dn.set_threshold(100.0, [mio.DampingSampling(0.4, 0, 0, 0.0, [0], np.ones(1))])
+        assert [t for t, _ in dn.threshold] == [500.0, 100.0]
+        assert dn.threshold[1][1][0].value.value == 0.4
+    nodes = [graph.get_node(v).property for v in range(graph.num_nodes)]
+    edges = [(e.start_node_idx, e.end_node_idx, e.property.coefficients) for e in graph._edges]
+    spec = osecir._spec_from([n.model for n in nodes], edges)
+    n = spec.npis
+    assert n["node_offsets"].tolist() == [0, 2, 4]
+    assert n["thresholds"].tolist() == [500.0, 100.0, 500.0, 100.0]
+    assert n["damping_values"].tolist() == [0.3, 0.4, 0.3, 0.4]
+    assert n["node_params"][0, :3].tolist() == [7.0, 2.0, 100000.0]
+    cfg = capi.sim_cfg(t0=0.0, dt_graph=0.5, dt_node=0.1)
+    with_npis, _ = oracle.graph_run(spec, cfg, 40.0)
+    spec.npis = None
+    spec._desc = None
+    without, _ = oracle.graph_run(spec, cfg, 40.0)
+    assert np.max(np.abs(with_npis - without)) > 1.0
