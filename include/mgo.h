@@ -146,6 +146,21 @@ typedef struct mgo_graph_desc {
     const double* edge_coeffs;        /* [num_edges][edge_coeff_len] */
     int32_t edge_coeff_len;           /* 10*G, or 1 (one coefficient for all compartments); 0 means 10*G */
     const mgo_dynamic_npis* dynamic_npis; /* NULL = no dynamic NPIs */
+    /* Mobility-coefficient dampings and edge dynamic NPIs (MobilityCoefficientGroup and
+     * MobilityParameters::get_dynamic_npis_infected, cpp/memilio/mobility/metapopulation_mobility_instant.h:106-290;
+     * applied by MobilityEdge::apply_mobility, :549-566 and :605-621: coefficients at time t are
+     * B - D(t) (B - M)). Every out-edge of a node shares the node's out-edge damping list and NPIs, as
+     * osecir::draw_sample gives every edge the same ones (cpp/models/ode_secir/parameter_space.h:240-246);
+     * dampings are per age group (the value of group g applies to its 10 compartments, as
+     * make_mobility_damping_vector builds them, damping_sampling.h:298-310). */
+    const double* edge_coeff_minimum;     /* [num_edges][edge_coeff_len] minimum coefficients M, or NULL (zero) */
+    const int32_t* out_damping_offsets;   /* [num_nodes+1] CSR into the out_damping arrays, or NULL */
+    const double* out_damping_times;      /* [K] */
+    const int32_t* out_damping_levels;    /* [K] */
+    const int32_t* out_damping_types;     /* [K] */
+    const double* out_damping_coeffs;     /* [K][G] damping value per age group */
+    const mgo_dynamic_npis* out_dynamic_npis; /* per node: NPIs of its out-edges (node_params: duration and base
+                                                 value are used; apply_mobility has no delay or directive), or NULL */
 } mgo_graph_desc;
 
 /* ---------------------------------------------------------------- integrators

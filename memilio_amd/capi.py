@@ -108,6 +108,13 @@ class GraphDesc(C.Structure):
         ("edge_coeffs", _dp),
         ("edge_coeff_len", C.c_int32),
         ("dynamic_npis", C.POINTER(DynamicNpis)),
+        ("edge_coeff_minimum", _dp),
+        ("out_damping_offsets", _ip),
+        ("out_damping_times", _dp),
+        ("out_damping_levels", _ip),
+        ("out_damping_types", _ip),
+        ("out_damping_coeffs", _dp),
+        ("out_dynamic_npis", C.POINTER(DynamicNpis)),
     ]
 
 

@@ -297,7 +297,29 @@ struct RunSpec {
 struct EdgeStructHost {
     std::vector<int32_t> in_ptr, out_ptr, split, seq;
     std::vector<EdgeEnt> in_ent, out_ent;
+    std::vector<double> in_min, out_min; // uniform minimum per entry (empty: the graph has no minimum)
     int n = 0, num_edges = 0;
+};
+
+// device buffers of one kind of run-time damping lists of a class (see NpiNode), with host copies of what
+// a run changes, for reset
+struct NpiBufs {
+    DevBuf nodes, thr, thr_damp, damp, list, cache_t, cache_m, work, y0;
+    std::vector<NpiNode> nodes0;
+    std::vector<double> list0, cache_t0, cache_m0;
+    void reset(hipStream_t s)
+    {
+        if (nodes0.empty()) {
+            return;
+        }
+        auto h2d = [&](DevBuf& b, const auto& v, const char* what) {
+            hip_check(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice, s), what);
+        };
+        h2d(nodes, nodes0, "reset NPI records");
+        h2d(list, list0, "reset damping lists");
+        h2d(cache_t, cache_t0, "reset damping caches");
+        h2d(cache_m, cache_m0, "reset damping caches");
+    }
 };
 
 struct ClassHost {
@@ -306,15 +328,14 @@ struct ClassHost {
     int num_nodes = 0;
     // device buffers
     DevBuf y, snap, ypre, rc, lcoef, cbase, cdelta, nodep, dtime, dmat, tnode, dtnode, steps, attempts, out,
-        ipool, epool, cvec, estructs, inst_ids;
+        ipool, epool, cvec, cvec_min, mpool, estructs, inst_ids;
     DevBuf fx_rec, fx_pool, fx_dirty;  // fix-up records of the return passes
     DevBuf node_order;                 // node-kernel lane slot -> node (empty: identity)
-    // dynamic NPIs (nodes with NF_NPI): per-node records, thresholds, damping lists and caches; host copies
-    // of what the run changes, for reset
-    DevBuf npi_nodes, npi_thr, npi_thr_damp, npi_damp, npi_list, npi_cache_t, npi_cache_m, npi_work, npi_y0;
-    std::vector<NpiNode> npi_nodes0;
-    std::vector<double> npi_list0, npi_cache_t0, npi_cache_m0;
-    std::vector<NodeP> nodep0;
+    // damping lists that change during a run: dynamic NPIs of the contact matrices (nodes with NF_NPI) and
+    // out-edge dampings / edge NPIs (nodes with NF_OUT_DAMP, with the damping factor of the last departure)
+    NpiBufs npi, oute;
+    DevBuf out_d;
+    std::vector<NodeP> nodep0; // node records as built (NPIs change their damping-cache counts)
     std::vector<int32_t> order;        // host copy, built by place_nodes
     DevBuf y0;                         // initial state, for reset
     std::vector<double> tnode0, dtnode0; // initial clocks, for reset
@@ -395,6 +416,43 @@ int day_ceil(double t)
     return int(std::ceil(t - 1e-10));
 }
 
+static void validate_npis(const mgo_dynamic_npis* dn, int num_nodes, const char* what)
+{
+    if (!dn) {
+        return;
+    }
+    if (!dn->node_offsets || !dn->node_params) {
+        fail(MGO_ERR_INVALID_VALUE, "%s: node_offsets and node_params are required", what);
+    }
+    if (dn->node_offsets[0] != 0) {
+        fail(MGO_ERR_INVALID_VALUE, "%s: node_offsets[0] must be 0", what);
+    }
+    for (int v = 0; v < num_nodes; ++v) {
+        if (dn->node_offsets[v + 1] < dn->node_offsets[v]) {
+            fail(MGO_ERR_INVALID_VALUE, "%s: node_offsets must be non-decreasing", what);
+        }
+        const double* p = dn->node_params + size_t(v) * MGO_NPI_NUM_PARAMS;
+        if (!(p[MGO_NPI_DURATION] >= 0) || !(p[MGO_NPI_DELAY] >= 0) || !std::isfinite(p[MGO_NPI_BASE_VALUE]) ||
+            !(p[MGO_NPI_DIRECTIVE_BEGIN] < p[MGO_NPI_DIRECTIVE_END])) {
+            fail(MGO_ERR_INVALID_VALUE, "%s of node %d: duration and delay must be >= 0, the base value finite and "
+                 "the directive begin before its end", what, v);
+        }
+    }
+    const int T = dn->node_offsets[num_nodes];
+    if (T > 0 && (!dn->thresholds || !dn->damping_offsets)) {
+        fail(MGO_ERR_INVALID_VALUE, "%s: thresholds and damping_offsets are required", what);
+    }
+    for (int t = 0; t < T; ++t) {
+        if (dn->damping_offsets[t + 1] < dn->damping_offsets[t] || dn->damping_offsets[t] < 0) {
+            fail(MGO_ERR_INVALID_VALUE, "%s: damping_offsets must be non-decreasing", what);
+        }
+    }
+    if (T > 0 && dn->damping_offsets[T] > dn->damping_offsets[0] &&
+        (!dn->damping_values || !dn->damping_levels || !dn->damping_types || !dn->damping_weights)) {
+        fail(MGO_ERR_INVALID_VALUE, "%s: damping arrays missing", what);
+    }
+}
+
 void validate(const mgo_graph_desc* g, const mgo_sim_cfg& cfg)
 {
     if (!g) {
@@ -415,36 +473,19 @@ void validate(const mgo_graph_desc* g, const mgo_sim_cfg& cfg)
     if (g->damping_offsets && (!g->damping_times || !g->damping_coeffs)) {
         fail(MGO_ERR_INVALID_VALUE, "damping arrays missing");
     }
-    if (const mgo_dynamic_npis* dn = g->dynamic_npis) {
-        if (!dn->node_offsets || !dn->node_params) {
-            fail(MGO_ERR_INVALID_VALUE, "dynamic NPIs: node_offsets and node_params are required");
-        }
-        if (dn->node_offsets[0] != 0) {
-            fail(MGO_ERR_INVALID_VALUE, "dynamic NPIs: node_offsets[0] must be 0");
+    validate_npis(g->dynamic_npis, g->num_nodes, "dynamic NPIs");
+    validate_npis(g->out_dynamic_npis, g->num_nodes, "edge dynamic NPIs");
+    if (g->out_damping_offsets) {
+        if (g->out_damping_offsets[0] != 0) {
+            fail(MGO_ERR_INVALID_VALUE, "out_damping_offsets[0] must be 0");
         }
         for (int v = 0; v < g->num_nodes; ++v) {
-            if (dn->node_offsets[v + 1] < dn->node_offsets[v]) {
-                fail(MGO_ERR_INVALID_VALUE, "dynamic NPIs: node_offsets must be non-decreasing");
-            }
-            const double* p = dn->node_params + size_t(v) * MGO_NPI_NUM_PARAMS;
-            if (!(p[MGO_NPI_DURATION] >= 0) || !(p[MGO_NPI_DELAY] >= 0) || !std::isfinite(p[MGO_NPI_BASE_VALUE]) ||
-                !(p[MGO_NPI_DIRECTIVE_BEGIN] < p[MGO_NPI_DIRECTIVE_END])) {
-                fail(MGO_ERR_INVALID_VALUE, "dynamic NPIs of node %d: duration and delay must be >= 0, the base value "
-                     "finite and the directive begin before its end", v);
+            if (g->out_damping_offsets[v + 1] < g->out_damping_offsets[v]) {
+                fail(MGO_ERR_INVALID_VALUE, "out_damping_offsets must be non-decreasing");
             }
         }
-        const int T = dn->node_offsets[g->num_nodes];
-        if (T > 0 && (!dn->thresholds || !dn->damping_offsets)) {
-            fail(MGO_ERR_INVALID_VALUE, "dynamic NPIs: thresholds and damping_offsets are required");
-        }
-        for (int t = 0; t < T; ++t) {
-            if (dn->damping_offsets[t + 1] < dn->damping_offsets[t] || dn->damping_offsets[t] < 0) {
-                fail(MGO_ERR_INVALID_VALUE, "dynamic NPIs: damping_offsets must be non-decreasing");
-            }
-        }
-        if (T > 0 && dn->damping_offsets[T] > dn->damping_offsets[0] &&
-            (!dn->damping_values || !dn->damping_levels || !dn->damping_types || !dn->damping_weights)) {
-            fail(MGO_ERR_INVALID_VALUE, "dynamic NPIs: damping arrays missing");
+        if (g->out_damping_offsets[g->num_nodes] > 0 && (!g->out_damping_times || !g->out_damping_coeffs)) {
+            fail(MGO_ERR_INVALID_VALUE, "out-edge damping arrays missing");
         }
     }
     const int k = cfg.integrator.kind;
@@ -516,7 +557,7 @@ void lane_coefficients(const mgo_graph_desc* g, int v, int a, double* c)
 
 // CSR of one graph: Graph::add_edge ordering (sorted by (start, end), duplicates replaced by the later
 // edge, graph.h:231-241); edges without a positive coefficient never move anyone and are dropped.
-void build_edges(const mgo_graph_desc* g, EdgeStructHost& es, std::vector<double>& cvec)
+void build_edges(const mgo_graph_desc* g, EdgeStructHost& es, std::vector<double>& cvec, std::vector<double>& cvec_min)
 {
     const int n  = g->num_nodes;
     const int G  = g->num_groups;
@@ -529,7 +570,7 @@ void build_edges(const mgo_graph_desc* g, EdgeStructHost& es, std::vector<double
     std::sort(keys.begin(), keys.end());
     struct E {
         int s, d;
-        double cval;
+        double cval, cmin;
         int cidx;
     };
     std::vector<E> edges;
@@ -541,19 +582,27 @@ void build_edges(const mgo_graph_desc* g, EdgeStructHost& es, std::vector<double
         }
         const int src   = int(keys[k].first >> 32);
         const int dst   = int(uint32_t(keys[k].first));
-        const double* c = g->edge_coeffs + size_t(keys[k].second) * len;
+        const double* c  = g->edge_coeffs + size_t(keys[k].second) * len;
+        const double* mn = g->edge_coeff_minimum ? g->edge_coeff_minimum + size_t(keys[k].second) * len : nullptr;
         bool any_pos = false, uniform = true;
         for (int k = 0; k < len; ++k) {
-            any_pos = any_pos || (c[k] > 0.0);
-            uniform = uniform && (c[k] == c[0]);
+            // an edge whose coefficients can never be positive (B - D (B - M) with B, M <= 0) never departs
+            any_pos = any_pos || (c[k] > 0.0) || (mn && mn[k] > 0.0);
+            uniform = uniform && (c[k] == c[0]) && (!mn || mn[k] == mn[0]);
         }
         if (!any_pos) {
             continue;
         }
-        E ed{src, dst, c[0], -1};
+        E ed{src, dst, c[0], mn ? mn[0] : 0.0, -1};
         if (!uniform) {
             ed.cidx = int(cvec.size() / size_t(nv));
             cvec.insert(cvec.end(), c, c + nv);
+            if (mn) {
+                cvec_min.insert(cvec_min.end(), mn, mn + nv);
+            }
+            else {
+                cvec_min.insert(cvec_min.end(), size_t(nv), 0.0);
+            }
         }
         edges.push_back(ed);
     }
@@ -571,6 +620,10 @@ void build_edges(const mgo_graph_desc* g, EdgeStructHost& es, std::vector<double
     }
     es.out_ent.resize(edges.size());
     es.in_ent.resize(edges.size());
+    if (g->edge_coeff_minimum) {
+        es.out_min.assign(edges.size(), 0.0);
+        es.in_min.assign(edges.size(), 0.0);
+    }
     es.split.assign(size_t(n), 0);
     es.seq.clear();
     std::vector<int> ofill(es.out_ptr.begin(), es.out_ptr.end() - 1), ifill(es.in_ptr.begin(), es.in_ptr.end() - 1);
@@ -580,6 +633,10 @@ void build_edges(const mgo_graph_desc* g, EdgeStructHost& es, std::vector<double
         const int ki    = ifill[size_t(e.d)]++;
         es.out_ent[size_t(ko)] = EdgeEnt{e.cval, e.d, e.cidx, ki, 0};
         es.in_ent[size_t(ki)]  = EdgeEnt{e.cval, e.s, e.cidx, ko, 0};
+        if (!es.out_min.empty()) {
+            es.out_min[size_t(ko)] = e.cmin;
+            es.in_min[size_t(ki)]  = e.cmin;
+        }
         if (e.s < e.d) {
             es.split[size_t(e.d)]++;
         }
@@ -602,39 +659,51 @@ struct NpiBuild {
     std::vector<int32_t> thr_damp;  // first damping of each threshold (+ a final end marker)
 };
 
-// DynamicNPIs of node v of graph g (desc arrays) with its contact matrix's initial dampings dl
-static void add_npi_node(NpiBuild& b, size_t node, const mgo_graph_desc* g, int v, int G, const std::vector<DampIn>& dl)
+// One damping list with its NPIs: node v's contact matrix (contact: n = G*G, NPIs g->dynamic_npis) or the
+// node's out-edge coefficients (n = G per-group values, NPIs g->out_dynamic_npis; entries of dl are G long).
+static void add_npi_node(NpiBuild& b, size_t node, const mgo_graph_desc* g, int v, int G, bool contact,
+                         const std::vector<DampIn>& dl)
 {
-    const mgo_dynamic_npis* dn = g->dynamic_npis;
+    const mgo_dynamic_npis* dn = contact ? g->dynamic_npis : g->out_dynamic_npis;
+    const int n                = contact ? G * G : G;
     NpiBuild::Node nd;
-    nd.node           = node;
-    NpiNode& r        = nd.rec;
-    const double* par = dn->node_params + size_t(v) * MGO_NPI_NUM_PARAMS;
-    r.duration        = par[MGO_NPI_DURATION];
-    r.delay           = par[MGO_NPI_DELAY];
-    r.base            = par[MGO_NPI_BASE_VALUE];
-    r.begin           = par[MGO_NPI_DIRECTIVE_BEGIN];
-    r.end             = par[MGO_NPI_DIRECTIVE_END];
-    r.value           = -DBL_MAX;
-    r.t_end           = 0.0;
-    double total      = 0.0; // Populations::get_total of the model (summed in storage order)
+    nd.node    = node;
+    NpiNode& r = nd.rec;
+    r.duration = 14.0;
+    r.delay    = 0.0;
+    r.base     = 1.0;
+    r.begin    = -DBL_MAX;
+    r.end      = DBL_MAX;
+    if (dn) {
+        const double* par = dn->node_params + size_t(v) * MGO_NPI_NUM_PARAMS;
+        r.duration        = par[MGO_NPI_DURATION];
+        r.delay           = par[MGO_NPI_DELAY];
+        r.base            = par[MGO_NPI_BASE_VALUE];
+        r.begin           = par[MGO_NPI_DIRECTIVE_BEGIN];
+        r.end             = par[MGO_NPI_DIRECTIVE_END];
+    }
+    r.value      = -DBL_MAX;
+    r.t_end      = 0.0;
+    double total = 0.0; // Populations::get_total of the model (summed in storage order)
     for (int k = 0; k < G * NC; ++k) {
         total += g->populations[size_t(v) * G * NC + size_t(k)];
     }
     r.total = total;
     // DynamicNPIs::set_threshold: sorted by value, descending; an equal value replaces the earlier one
     std::vector<std::pair<double, int>> thr;
-    for (int t = dn->node_offsets[v]; t < dn->node_offsets[v + 1]; ++t) {
-        const double val = dn->thresholds[t];
-        auto it          = thr.begin();
-        while (it != thr.end() && it->first > val) {
-            ++it;
-        }
-        if (it != thr.end() && !(val > it->first)) {
-            it->second = t;
-        }
-        else {
-            thr.insert(it, std::make_pair(val, t));
+    if (dn) {
+        for (int t = dn->node_offsets[v]; t < dn->node_offsets[v + 1]; ++t) {
+            const double val = dn->thresholds[t];
+            auto it          = thr.begin();
+            while (it != thr.end() && it->first > val) {
+                ++it;
+            }
+            if (it != thr.end() && !(val > it->first)) {
+                it->second = t;
+            }
+            else {
+                thr.insert(it, std::make_pair(val, t));
+            }
         }
     }
     r.thr_first = int32_t(b.thr.size());
@@ -651,16 +720,16 @@ static void add_npi_node(NpiBuild& b, size_t node, const mgo_graph_desc* g, int 
     }
     // damping list: the initial dampings in add order, room for MGO_NPI_LIST_EXTRA more
     r.list_cap = int32_t(dl.size()) + MGO_NPI_LIST_EXTRA;
-    nd.list.assign(size_t(r.list_cap) * size_t(npi_entry_size(G)), 0.0);
+    nd.list.assign(size_t(r.list_cap) * size_t(npi_entry_size(n)), 0.0);
     int count = 0;
     for (const DampIn& d : dl) {
-        npi_list_add(nd.list.data(), count, r.list_cap, G, d.time, d.level, d.type, d.m.data());
+        npi_list_add(nd.list.data(), count, r.list_cap, n, d.time, d.level, d.type, d.m.data());
     }
     r.list_count = count;
     nd.cache_t.assign(size_t(r.list_cap) + 2, 0.0);
-    nd.cache_m.assign((size_t(r.list_cap) + 2) * size_t(G) * G, 0.0);
-    std::vector<double> work(size_t(npi_work_size(G, r.list_cap)));
-    const int m = npi_build_cache(nd.list.data(), count, G, r.list_cap, nd.cache_t.data(), nd.cache_m.data(),
+    nd.cache_m.assign((size_t(r.list_cap) + 2) * size_t(n), 0.0);
+    std::vector<double> work(size_t(npi_work_size(n, r.list_cap)));
+    const int m = npi_build_cache(nd.list.data(), count, n, r.list_cap, nd.cache_t.data(), nd.cache_m.data(),
                                   work.data());
     if (m < 0) {
         fail(MGO_ERR_UNSUPPORTED, "node %d: more than %d damping levels", v, NPI_LEVELS);
@@ -799,64 +868,67 @@ struct Batch {
         }
     }
 
-    // device buffers of a class's dynamic NPIs; points the nodes' damping caches into them
-    void upload_npis(ClassHost& C, NpiBuild& b, std::vector<NodeP>& nodep)
+    // device buffers of one kind of run-time damping lists (n coefficients per entry); for the contact kind
+    // the nodes' damping caches point into them
+    void upload_npis(ClassHost& C, NpiBufs& B, NpiBuild& b, int n, bool contact, std::vector<NodeP>& nodep)
     {
-        C.npi_nodes0.clear();
+        B.nodes0.clear();
         if (b.nodes.empty()) {
             return;
         }
-        const int G  = C.G;
-        const size_t n = size_t(G) * G;
+        const int G = C.G;
         b.thr_damp.push_back(int32_t(b.damp.size() / size_t(3 + G)));
         size_t nl = 0, nct = 0, nw = 0;
         for (auto& nd : b.nodes) {
             nl += nd.list.size();
             nct += nd.cache_t.size();
-            nw += size_t(npi_work_size(G, nd.rec.list_cap));
+            nw += size_t(npi_work_size(n, nd.rec.list_cap));
         }
-        C.npi_list0.clear();
-        C.npi_cache_t0.clear();
-        C.npi_cache_m0.clear();
-        C.npi_list0.reserve(nl);
-        C.npi_cache_t0.reserve(nct);
-        C.npi_cache_m0.reserve(nct * n);
+        B.list0.clear();
+        B.cache_t0.clear();
+        B.cache_m0.clear();
+        B.list0.reserve(nl);
+        B.cache_t0.reserve(nct);
+        B.cache_m0.reserve(nct * size_t(n));
         for (auto& nd : b.nodes) {
-            C.npi_list0.insert(C.npi_list0.end(), nd.list.begin(), nd.list.end());
-            C.npi_cache_t0.insert(C.npi_cache_t0.end(), nd.cache_t.begin(), nd.cache_t.end());
-            C.npi_cache_m0.insert(C.npi_cache_m0.end(), nd.cache_m.begin(), nd.cache_m.end());
+            B.list0.insert(B.list0.end(), nd.list.begin(), nd.list.end());
+            B.cache_t0.insert(B.cache_t0.end(), nd.cache_t.begin(), nd.cache_t.end());
+            B.cache_m0.insert(B.cache_m0.end(), nd.cache_m.begin(), nd.cache_m.end());
         }
-        C.npi_list.upload(C.npi_list0, stream);
-        C.npi_cache_t.upload(C.npi_cache_t0, stream);
-        C.npi_cache_m.upload(C.npi_cache_m0, stream);
-        C.npi_work.alloc(nw * sizeof(double));
-        C.npi_y0.alloc(b.nodes.size() * size_t(G) * NC * sizeof(double));
-        C.npi_thr.upload(b.thr, stream);
-        C.npi_thr_damp.upload(b.thr_damp, stream);
-        C.npi_damp.upload(b.damp, stream);
-        C.npi_nodes0.assign(size_t(C.num_nodes), NpiNode{});
+        B.list.upload(B.list0, stream);
+        B.cache_t.upload(B.cache_t0, stream);
+        B.cache_m.upload(B.cache_m0, stream);
+        B.work.alloc(nw * sizeof(double));
+        if (contact) {
+            B.y0.alloc(b.nodes.size() * size_t(G) * NC * sizeof(double));
+        }
+        B.thr.upload(b.thr, stream);
+        B.thr_damp.upload(b.thr_damp, stream);
+        B.damp.upload(b.damp, stream);
+        B.nodes0.assign(size_t(C.num_nodes), NpiNode{});
         size_t ol = 0, oc = 0, ow = 0;
         for (size_t k = 0; k < b.nodes.size(); ++k) {
             NpiBuild::Node& nd = b.nodes[k];
             NpiNode r          = nd.rec;
-            r.list             = C.npi_list.as<double>() + ol;
-            r.cache_t          = C.npi_cache_t.as<double>() + oc;
-            r.cache_m          = C.npi_cache_m.as<double>() + oc * n;
-            r.work             = C.npi_work.as<double>() + ow;
-            r.y0               = C.npi_y0.as<double>() + k * size_t(G) * NC;
+            r.list             = B.list.as<double>() + ol;
+            r.cache_t          = B.cache_t.as<double>() + oc;
+            r.cache_m          = B.cache_m.as<double>() + oc * size_t(n);
+            r.work             = B.work.as<double>() + ow;
+            r.y0               = contact ? B.y0.as<double>() + k * size_t(G) * NC : nullptr;
             ol += nd.list.size();
             oc += nd.cache_t.size();
-            ow += size_t(npi_work_size(G, r.list_cap));
-            C.npi_nodes0[nd.node] = r;
-            NodeP& np             = nodep[nd.node];
-            np.damp_time          = r.cache_t;
-            np.damp_mat           = r.cache_m;
-            np.damp_cnt           = r.cache_count;
+            ow += size_t(npi_work_size(n, r.list_cap));
+            B.nodes0[nd.node] = r;
+            if (contact) {
+                NodeP& np    = nodep[nd.node];
+                np.damp_time = r.cache_t;
+                np.damp_mat  = r.cache_m;
+                np.damp_cnt  = r.cache_count;
+            }
         }
-        C.npi_nodes.upload(C.npi_nodes0, stream);
-        C.nodep0 = nodep;
-        device_bytes += C.npi_list.bytes + C.npi_cache_t.bytes + C.npi_cache_m.bytes + C.npi_work.bytes +
-                        C.npi_y0.bytes + C.npi_thr.bytes + C.npi_thr_damp.bytes + C.npi_damp.bytes + C.npi_nodes.bytes;
+        B.nodes.upload(B.nodes0, stream);
+        device_bytes += B.list.bytes + B.cache_t.bytes + B.cache_m.bytes + B.work.bytes + B.y0.bytes + B.thr.bytes +
+                        B.thr_damp.bytes + B.damp.bytes + B.nodes.bytes;
     }
 
     void build_class(size_t ci, std::vector<RunSpec>& runs, std::vector<InstDev>& idev)
@@ -869,15 +941,16 @@ struct Batch {
         std::vector<NodeP> nodep(N);
         std::vector<int32_t> ipool, inst_ids;
         std::vector<EdgeEnt> epool;
-        std::vector<double> cvec;
+        std::vector<double> cvec, cvec_min, mpool;
         struct StructOff {
             size_t in_ptr, out_ptr, split, seq, in_ent, out_ent;
+            int64_t in_min = -1, out_min = -1; // offsets into mpool, -1 = no minimum
             int n, num_edges;
         };
         std::vector<StructOff> soffs;
         std::vector<size_t> damp_off(N, 0);
-        NpiBuild npi;
-        std::map<std::tuple<const void*, const void*, const void*, int, int>, int> struct_cache;
+        NpiBuild npi, outd;
+        std::map<std::tuple<const void*, const void*, const void*, const void*, int, int>, int> struct_cache;
         int64_t out_total = 0;
         int64_t fx_total  = 0;
         int64_t pool_slots = 0;
@@ -943,9 +1016,28 @@ struct Batch {
                 np.damp_cnt    = 0;
                 np.damp_time   = nullptr;
                 np.damp_mat    = nullptr;
+                // out-edge coefficient dampings and edge NPIs (shared by the node's out-edges)
+                {
+                    std::vector<DampIn> ol;
+                    if (g->out_damping_offsets) {
+                        for (int d = g->out_damping_offsets[v]; d < g->out_damping_offsets[v + 1]; ++d) {
+                            DampIn di;
+                            di.time  = g->out_damping_times[d];
+                            di.level = g->out_damping_levels ? g->out_damping_levels[d] : 0;
+                            di.type  = g->out_damping_types ? g->out_damping_types[d] : 0;
+                            di.m.assign(g->out_damping_coeffs + size_t(d) * G, g->out_damping_coeffs + size_t(d + 1) * G);
+                            ol.push_back(di);
+                        }
+                    }
+                    const mgo_dynamic_npis* on = g->out_dynamic_npis;
+                    if (!r.single && (!ol.empty() || (on && on->node_offsets[v + 1] > on->node_offsets[v]))) {
+                        add_npi_node(outd, node, g, v, G, false, ol);
+                        np.flags |= NF_OUT_DAMP;
+                    }
+                }
                 const mgo_dynamic_npis* dn = g->dynamic_npis;
                 if (dn && dn->node_offsets[v + 1] > dn->node_offsets[v]) {
-                    add_npi_node(npi, node, g, v, G, dl);
+                    add_npi_node(npi, node, g, v, G, true, dl);
                     np.flags |= NF_NPI | NF_DAMPED;
                 }
                 else if (!dl.empty()) {
@@ -963,7 +1055,8 @@ struct Batch {
             I.has_edges = false;
             if (!r.single && g->num_edges > 0) {
                 auto key = std::make_tuple((const void*)g->edge_start, (const void*)g->edge_end,
-                                           (const void*)g->edge_coeffs, g->num_edges, g->num_nodes);
+                                           (const void*)g->edge_coeffs, (const void*)g->edge_coeff_minimum,
+                                           g->num_edges, g->num_nodes);
                 auto it  = struct_cache.find(key);
                 int sidx;
                 if (it != struct_cache.end()) {
@@ -971,7 +1064,7 @@ struct Batch {
                 }
                 else {
                     EdgeStructHost es;
-                    build_edges(g, es, cvec);
+                    build_edges(g, es, cvec, cvec_min);
                     StructOff so{};
                     so.n         = es.n;
                     so.num_edges = es.num_edges;
@@ -987,6 +1080,12 @@ struct Batch {
                     epool.insert(epool.end(), es.in_ent.begin(), es.in_ent.end());
                     so.out_ent = epool.size();
                     epool.insert(epool.end(), es.out_ent.begin(), es.out_ent.end());
+                    if (!es.in_min.empty()) {
+                        so.in_min = int64_t(mpool.size());
+                        mpool.insert(mpool.end(), es.in_min.begin(), es.in_min.end());
+                        so.out_min = int64_t(mpool.size());
+                        mpool.insert(mpool.end(), es.out_min.begin(), es.out_min.end());
+                    }
                     sidx = int(soffs.size());
                     soffs.push_back(so);
                     struct_cache[key] = sidx;
@@ -1046,7 +1145,14 @@ struct Batch {
                 nodep[node].damp_mat  = C.dmat.as<double>() + damp_off[node] * G * G;
             }
         }
-        upload_npis(C, npi, nodep);
+        upload_npis(C, C.npi, npi, G * G, true, nodep);
+        upload_npis(C, C.oute, outd, G, false, nodep);
+        if (!C.oute.nodes0.empty()) {
+            C.out_d.alloc(N * size_t(G) * sizeof(double));
+            hip_check(hipMemsetAsync(C.out_d.p, 0, C.out_d.bytes, stream), "damping factors");
+            device_bytes += C.out_d.bytes;
+        }
+        C.nodep0 = nodep;
         up(C.nodep, nodep);
         up(C.tnode, tnode);
         up(C.dtnode, dtnode);
@@ -1059,6 +1165,8 @@ struct Batch {
         up(C.ipool, ipool);
         up(C.epool, epool);
         up(C.cvec, cvec);
+        up(C.cvec_min, cvec_min);
+        up(C.mpool, mpool);
         std::vector<EdgeStructDev> estructs(soffs.size());
         for (size_t k = 0; k < soffs.size(); ++k) {
             const auto& so          = soffs[k];
@@ -1072,6 +1180,9 @@ struct Batch {
             sd.in_ent               = eb + so.in_ent;
             sd.out_ent              = eb + so.out_ent;
             sd.cvec                 = C.cvec.as<double>();
+            sd.cvec_min             = C.cvec_min.as<double>();
+            sd.in_min               = so.in_min >= 0 ? C.mpool.as<double>() + so.in_min : nullptr;
+            sd.out_min              = so.out_min >= 0 ? C.mpool.as<double>() + so.out_min : nullptr;
             sd.n                    = so.n;
             sd.num_edges            = so.num_edges;
         }
@@ -1135,10 +1246,15 @@ struct Batch {
         d.fx_rec    = C.fx_rec.as<FxRec>();
         d.fx_pool   = C.fx_pool.as<double>();
         d.fx_dirty  = C.fx_dirty.as<int32_t>();
-        d.npi          = C.npi_nodes0.empty() ? nullptr : C.npi_nodes.as<NpiNode>();
-        d.npi_thr      = C.npi_thr.as<double>();
-        d.npi_thr_damp = C.npi_thr_damp.as<int32_t>();
-        d.npi_damp     = C.npi_damp.as<double>();
+        d.npi          = C.npi.nodes0.empty() ? nullptr : C.npi.nodes.as<NpiNode>();
+        d.npi_thr      = C.npi.thr.as<double>();
+        d.npi_thr_damp = C.npi.thr_damp.as<int32_t>();
+        d.npi_damp     = C.npi.damp.as<double>();
+        d.out_npi      = C.oute.nodes0.empty() ? nullptr : C.oute.nodes.as<NpiNode>();
+        d.out_thr      = C.oute.thr.as<double>();
+        d.out_thr_damp = C.oute.thr_damp.as<int32_t>();
+        d.out_damp     = C.oute.damp.as<double>();
+        d.out_d        = C.out_d.as<double>();
         C.seg_host = d;
         std::vector<SegDev> segs{d};
         std::vector<int32_t> bseg, bfirst;
@@ -1155,7 +1271,7 @@ struct Batch {
         ld.num_blocks  = int(bseg.size());
         ld.G           = G;
         ld.kind        = C.kind;
-        ld.npi         = C.npi_nodes0.empty() ? 0 : 1;
+        ld.npi         = (C.npi.nodes0.empty() && C.oute.nodes0.empty()) ? 0 : 1;
     }
 
     // launch tables of a cycle: one per class. (Batches keep per-G kernels so that a run's arithmetic is
@@ -1312,17 +1428,16 @@ struct Batch {
             hip_check(hipMemsetAsync(C.steps.p, 0, C.steps.bytes, stream), "reset steps");
             hip_check(hipMemsetAsync(C.attempts.p, 0, C.attempts.bytes, stream), "reset attempts");
             C.clear_fixup_records(stream);
-            if (!C.npi_nodes0.empty()) {
-                // damping lists, caches and NPI state as built (dynamic NPIs change them during a run)
-                auto h2d = [&](DevBuf& b, const auto& v, const char* what) {
-                    hip_check(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice, stream),
-                              what);
-                };
-                h2d(C.npi_nodes, C.npi_nodes0, "reset NPI records");
-                h2d(C.npi_list, C.npi_list0, "reset damping lists");
-                h2d(C.npi_cache_t, C.npi_cache_t0, "reset damping caches");
-                h2d(C.npi_cache_m, C.npi_cache_m0, "reset damping caches");
-                h2d(C.nodep, C.nodep0, "reset node records");
+            // damping lists, caches and NPI state as built (dynamic NPIs change them during a run)
+            C.npi.reset(stream);
+            C.oute.reset(stream);
+            if (!C.npi.nodes0.empty()) {
+                hip_check(hipMemcpyAsync(C.nodep.p, C.nodep0.data(), C.nodep0.size() * sizeof(NodeP),
+                                         hipMemcpyHostToDevice, stream),
+                          "reset node records");
+            }
+            if (C.out_d.p) {
+                hip_check(hipMemsetAsync(C.out_d.p, 0, C.out_d.bytes, stream), "reset damping factors");
             }
         }
         hip_check(hipMemcpyAsync(inst_dev.p, inst0.data(), inst0.size() * sizeof(InstDev), hipMemcpyHostToDevice,

@@ -95,6 +95,7 @@ enum : int32_t {
     NF_DAMPED      = 4, // contact matrix has dampings
     NF_DETECTION   = 8, // commuter detection interval non-empty
     NF_NPI         = 16, // dynamic NPIs: the damping list lives in an NpiNode and changes during the run
+    NF_OUT_DAMP    = 32, // out-edge coefficient dampings or edge NPIs (SegDev::out_npi)
 };
 
 // Dynamic NPIs of one node (osecir::Simulation::advance, models/ode_secir/model.h:289-367). The damping
@@ -204,6 +205,9 @@ struct EdgeStructDev {
     const EdgeEnt* in_ent;
     const EdgeEnt* out_ent;
     const double* cvec;      // non-uniform coefficient vectors [k][10*G]
+    const double* in_min;    // minimum coefficient of each uniform in-entry, or NULL (zero)
+    const double* out_min;   // minimum coefficient of each uniform out-entry, or NULL (zero)
+    const double* cvec_min;  // minimum coefficient vectors [k][10*G] (parallel to cvec), or NULL (zero)
     int32_t n;
     int32_t num_edges;
 };
@@ -234,6 +238,14 @@ struct SegDev {
     const double* npi_thr;         // threshold values
     const int32_t* npi_thr_damp;   // [threshold + 1] first damping of each threshold in npi_damp
     const double* npi_damp;        // [damping][3 + G] value, level, type, group weights
+    // out-edge coefficient dampings and edge NPIs (nodes with NF_OUT_DAMP): per-group damping lists (entries of
+    // G values) shared by a node's out-edges, their thresholds and dampings as above, and the damping factor
+    // D of every lane at the last departure (coefficients B - D (B - M)); out_d is NULL without out-dampings
+    NpiNode* out_npi;
+    const double* out_thr;
+    const int32_t* out_thr_damp;
+    const double* out_damp;
+    double* out_d;                 // [node][g]
 };
 
 // One kernel launch over the segments of a class.

@@ -577,9 +577,9 @@ __device__ double npi_piece(const SegDev& c, Lane<G>& L, InstDev& I, const doubl
                         const int gi = nn->thr_first + k;
                         const int d0 = c.npi_thr_damp[gi], d1 = c.npi_thr_damp[gi + 1];
                         int count    = nn->list_count;
-                        bool ok      = npi_implement(nn->list, count, nn->list_cap, G, c.npi_damp + d0 * (3 + G),
-                                                     d1 - d0, tsd, te + 1.0, nn->work);
-                        const int m  = ok ? npi_build_cache(nn->list, count, G, nn->list_cap, nn->cache_t,
+                        bool ok      = npi_implement(nn->list, count, nn->list_cap, G, true,
+                                                     c.npi_damp + d0 * (3 + G), d1 - d0, tsd, te + 1.0, nn->work);
+                        const int m  = ok ? npi_build_cache(nn->list, count, G * G, nn->list_cap, nn->cache_t,
                                                             nn->cache_m, nn->work)
                                           : -1;
                         if (m < 0) {
@@ -604,6 +604,78 @@ __device__ double npi_piece(const SegDev& c, Lane<G>& L, InstDev& I, const doubl
         dt_eff = (t_end - t < dt_eff) ? t_end - t : dt_eff;
     }
     return dt_eff;
+}
+
+// Out-edge dampings of the lane's node at the cycle's edge action (MobilityEdge::apply_mobility at t = c_t1,
+// metapopulation_mobility_instant.h:549-566): the edge NPI check on the node's last state (no delay, no
+// directive; begin t, end t + duration, per-group mobility dampings), then on a departure the damping factor
+// D of the lane's age group at t (Dampings::get_matrix_at, damping.h:381-396) for the edge kernels.
+template <int G>
+__device__ void out_damping(const SegDev& c, Lane<G>& L, InstDev& I, const double (&y)[NC])
+{
+    NpiNode* od    = c.out_npi + L.node;
+    const double t = I.c_t1;
+    if (od->thr_count > 0) {
+        const double sum     = group_sum<G>(y[ISY], L.base);
+        const double inf_rel = sum / od->total * od->base;
+        int k                = 0;
+        while (k < od->thr_count && !(inf_rel > c.out_thr[od->thr_first + k])) {
+            ++k;
+        }
+        const double te    = od->t_end;
+        const bool expired = !(t - te < -(1e-10 + DBL_MIN * fmax(fabs(t), fabs(te))));
+        bool changed       = false;
+        if (k < od->thr_count) {
+            const double thr = c.out_thr[od->thr_first + k];
+            if (thr > od->value || expired) {
+                changed = true;
+                if (L.age == 0) {
+                    const double t_end = t + od->duration;
+                    od->value          = thr;
+                    od->t_end          = t_end;
+                    const int gi       = od->thr_first + k;
+                    const int d0 = c.out_thr_damp[gi], d1 = c.out_thr_damp[gi + 1];
+                    int count    = od->list_count;
+                    const bool ok = npi_implement(od->list, count, od->list_cap, G, false, c.out_damp + d0 * (3 + G),
+                                                  d1 - d0, t, t_end, od->work);
+                    const int m   = ok ? npi_build_cache(od->list, count, G, od->list_cap, od->cache_t, od->cache_m,
+                                                         od->work)
+                                       : -1;
+                    if (m < 0) {
+                        I.status = MGO_ERR_OUT_OF_RANGE; // damping list or level capacity exceeded
+                    }
+                    else {
+                        od->list_count  = count;
+                        od->cache_count = m;
+                    }
+                }
+            }
+        }
+        if (changed) {
+            __threadfence(); // the group's other lanes read the cache written by its first lane
+        }
+    }
+    if (I.c_action == 1) {
+        const int cnt    = *static_cast<volatile int32_t*>(&od->cache_count);
+        const double* tk = od->cache_t;
+        int k            = 1;
+        while (k < cnt - 1 && !(tk[k] > t)) {
+            ++k;
+        }
+        const double xr = tk[k], xl = xr - 1.0;
+        const double Dl = od->cache_m[(k - 1) * G + L.age], Dr = od->cache_m[k * G + L.age];
+        double d;
+        if (t <= xl) {
+            d = Dl;
+        }
+        else if (t >= xr) {
+            d = Dr;
+        }
+        else {
+            d = 0.5 * (Dl - Dr) * cos(MGO_PI / (xr - xl) * (t - xl)) + 0.5 * (Dl + Dr);
+        }
+        c.out_d[size_t(L.node) * G + L.age] = d;
+    }
 }
 
 // ------------------------------------------------------------------------------- node phase
@@ -844,6 +916,9 @@ __device__ __forceinline__ void node_body(const LaunchDev& ld, InstDev* __restri
         c.attempts[L.node] += natt;
     }
     const size_t off = (size_t(L.node) * G + L.age) * NC;
+    if (NPI && (L.np.flags & NF_OUT_DAMP) && I.c_action != 0) {
+        out_damping<G>(c, L, I, y);
+    }
     if (I.c_action == 1) {
         // departure row: SimulationNode::m_last_state, the ISy mobility factor and the commuter
         // nondetection factor at the departure time
@@ -910,21 +985,39 @@ __device__ __forceinline__ void load_row(const double* snap, size_t lane, double
 }
 
 // traveler vector of one edge from its source's departure row (metapopulation_mobility_instant.h:608-615:
-// snapshot * coefficients * mobility factors, then test_commuters' nondetection on INS and ISy if with_nd)
-__device__ __forceinline__ void traveler_vector(const double* cvec, const EdgeEnt& e, const double (&row)[SNAP_STRIDE],
-                                                int G, int age, bool with_nd, double (&m)[NC])
+// snapshot * coefficients * mobility factors, then test_commuters' nondetection on INS and ISy if with_nd).
+// The coefficients at the departure time are B - D (B - M) (DampingMatrixExpression::get_matrix_at,
+// contact_matrix.h:190-194) with the source's out-edge damping factor D; mn: the entry's uniform minimum
+// (entry index k of the in- or out-entries), or NULL.
+__device__ __forceinline__ void traveler_vector(const EdgeStructDev& es, const EdgeEnt& e, const double* mn, int k,
+                                                double D, const double (&row)[SNAP_STRIDE], int G, int age,
+                                                bool with_nd, double (&m)[NC])
 {
     if (e.cidx < 0) {
+        double cv = e.cval;
+        if (D != 0.0) {
+            cv = e.cval - D * (e.cval - (mn ? mn[k] : 0.0));
+        }
 #pragma unroll
         for (int i = 0; i < NC; ++i) {
-            m[i] = row[i] * e.cval;
+            m[i] = row[i] * cv;
         }
     }
     else {
-        const double* cv = cvec + size_t(e.cidx) * G * NC + size_t(age) * NC;
+        const size_t off = size_t(e.cidx) * G * NC + size_t(age) * NC;
+        const double* cv = es.cvec + off;
+        if (D != 0.0) {
+            const double* mv = es.cvec_min ? es.cvec_min + off : nullptr;
 #pragma unroll
-        for (int i = 0; i < NC; ++i) {
-            m[i] = row[i] * cv[i];
+            for (int i = 0; i < NC; ++i) {
+                m[i] = row[i] * (cv[i] - D * (cv[i] - (mv ? mv[i] : 0.0)));
+            }
+        }
+        else {
+#pragma unroll
+            for (int i = 0; i < NC; ++i) {
+                m[i] = row[i] * cv[i];
+            }
         }
     }
     m[ISY] = m[ISY] * row[SNAP_FAC];
@@ -1007,7 +1100,6 @@ __device__ __forceinline__ void depart_body(const LaunchDev& ld, InstDev* __rest
     const int out_b = es.out_ptr[v], out_e = es.out_ptr[v + 1];
     const EdgeEnt* ine      = es.in_ent;
     const EdgeEnt* oute     = es.out_ent;
-    const double* cvec      = es.cvec;
     const size_t lane_off   = size_t(node) * G + age;
     const size_t src_base   = size_t(I.node_base) * G + age;   // lane of local node 0
     const size_t src_step   = size_t(I.node_stride) * G;       // lanes between local nodes
@@ -1021,9 +1113,10 @@ __device__ __forceinline__ void depart_body(const LaunchDev& ld, InstDev* __rest
         for (; k < end; ++k) {
             const EdgeEnt e = ine[k];
             double r[SNAP_STRIDE];
-            load_row(c.snap, src_base + size_t(e.nbr) * src_step, r);
+            const size_t sl = src_base + size_t(e.nbr) * src_step;
+            load_row(c.snap, sl, r);
             double m[NC];
-            traveler_vector(cvec, e, r, G, age, true, m);
+            traveler_vector(es, e, es.in_min, k, c.out_d ? c.out_d[sl] : 0.0, r, G, age, true, m);
 #pragma unroll
             for (int i = 0; i < NC; ++i) {
                 y[i] += m[i];
@@ -1035,9 +1128,10 @@ __device__ __forceinline__ void depart_body(const LaunchDev& ld, InstDev* __rest
         double sv[SNAP_STRIDE];
         load_row(c.snap, lane_off, sv);
         const double nd_v = sv[SNAP_ND];
+        const double D_v  = c.out_d ? c.out_d[lane_off] : 0.0;
         for (int k = out_b; k < out_e; ++k) {
             double m[NC];
-            traveler_vector(cvec, oute[k], sv, G, age, false, m);
+            traveler_vector(es, oute[k], es.out_min, k, D_v, sv, G, age, false, m);
             if (nd_v != 1.0) {
                 // test_commuters on the source: detected commuters stay home as confirmed cases
                 y[INS] -= m[INS] * (1 - nd_v);
@@ -1121,7 +1215,6 @@ __device__ __forceinline__ void return_body(const LaunchDev& ld, InstDev* __rest
     const int out_b = es.out_ptr[v], out_e = es.out_ptr[v + 1];
     const EdgeEnt* ine      = es.in_ent;
     const EdgeEnt* oute     = es.out_ent;
-    const double* cvec      = es.cvec;
     const size_t lane_off   = size_t(node) * G + age;
     const size_t src_base   = size_t(I.node_base) * G + age;   // lane of local node 0
     const size_t src_step   = size_t(I.node_stride) * G;       // lanes between local nodes
@@ -1140,9 +1233,10 @@ __device__ __forceinline__ void return_body(const LaunchDev& ld, InstDev* __rest
     auto in_edge = [&](int k) {
         const EdgeEnt e = ine[k];
         double row[SNAP_STRIDE];
-        load_row(c.snap, src_base + size_t(e.nbr) * src_step, row);
+        const size_t sl = src_base + size_t(e.nbr) * src_step;
+        load_row(c.snap, sl, row);
         double m[NC];
-        traveler_vector(cvec, e, row, G, age, true, m);
+        traveler_vector(es, e, es.in_min, k, c.out_d ? c.out_d[sl] : 0.0, row, G, age, true, m);
         return_euler(rc_v, h, m);
         const bool any = group_any<G>(fixup_negative(y, m), base);
         const int32_t ps = FIRST ? -1 : record_slot(recs + 2 * k, lo, tag - 1);
@@ -1193,6 +1287,7 @@ __device__ __forceinline__ void return_body(const LaunchDev& ld, InstDev* __rest
     if (out_b < out_e) {
         double sv[SNAP_STRIDE];
         load_row(c.snap, lane_off, sv);
+        const double D_v = c.out_d ? c.out_d[lane_off] : 0.0;
         for (int k = out_b; k < out_e; ++k) {
             const EdgeEnt e = oute[k];
             double m[NC];
@@ -1204,7 +1299,7 @@ __device__ __forceinline__ void return_body(const LaunchDev& ld, InstDev* __rest
                 }
             }
             else {
-                traveler_vector(cvec, e, sv, G, age, true, m);
+                traveler_vector(es, e, es.out_min, k, D_v, sv, G, age, true, m);
                 return_euler(c.rc + (src_base + size_t(e.nbr) * src_step) * RC_STRIDE, h, m);
             }
 #pragma unroll
@@ -1285,7 +1380,6 @@ __device__ __forceinline__ void replay_body(const LaunchDev& ld, InstDev* __rest
     const EdgeStructDev& es = c.estructs[I.estruct];
     const int32_t* seq      = es.seq;
     const EdgeEnt* oute     = es.out_ent;
-    const double* cvec      = es.cvec;
     const double h          = I.c_step;
     if (a < G) {
         for (int k = 0; k < es.num_edges; ++k) {
@@ -1295,7 +1389,7 @@ __device__ __forceinline__ void replay_body(const LaunchDev& ld, InstDev* __rest
             double row[SNAP_STRIDE];
             load_row(c.snap, soff, row);
             double m[NC];
-            traveler_vector(cvec, e, row, G, a, true, m);
+            traveler_vector(es, e, es.out_min, seq[3 * k + 2], c.out_d ? c.out_d[soff] : 0.0, row, G, a, true, m);
             return_euler(c.rc + doff * RC_STRIDE, h, m);
             double* yd = c.y + doff * NC;
             double* ys = c.y + soff * NC;

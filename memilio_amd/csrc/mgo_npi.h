@@ -1,10 +1,12 @@
 // mgo_npi.h - damping lists of contact matrices that change during a run (dynamic NPIs), shared by the
 // host engine (initial lists and caches) and the node kernel (NPI implementation at run time).
 //
-// A node with dynamic NPIs keeps its contact matrix's damping list on the device:
-//   list   [cap][3 + G*G]   time, level, type, coefficients (row major), sorted by (time, type, level)
-//   cache  [cap + 2]        accumulated damping matrices and their times (Dampings::update_cache), the
-//                           representation the node kernel reads through NodeP::damp_time / damp_mat
+// A node with dynamic NPIs keeps its contact matrix's damping list on the device, a node whose out-edges have
+// coefficient dampings or NPIs keeps their (shared) list; n coefficients per entry (G*G for the contact
+// matrix, G for the per-group mobility dampings):
+//   list   [cap][3 + n]     time, level, type, coefficients, sorted by (time, type, level)
+//   cache  [cap + 2]        accumulated damping coefficients and their times (Dampings::update_cache), the
+//                           representation the kernels read (NodeP::damp_time / damp_mat for the contact matrix)
 // The functions below restate cpp/memilio/epidemiology/damping.h (add_, remove, update_cache) and
 // cpp/memilio/epidemiology/dynamic_npis.h (implement_dynamic_npis) on these flat arrays. They run on one
 // thread (the host, or one lane of the node's wavefront group) and never allocate.
@@ -26,16 +28,16 @@ namespace mgo
 
 constexpr int NPI_LEVELS = 8; // distinct damping levels per contact matrix (update_cache's sum_by_level)
 
-MGO_HD inline int npi_entry_size(int G)
+MGO_HD inline int npi_entry_size(int n)
 {
-    return 3 + G * G;
+    return 3 + n;
 }
 
-// workspace doubles one node needs: value, active, active_end, candidate, combined [5][G*G], per-level sums
-// [NPI_LEVELS][G*G], active_by_type [cap][3]
-MGO_HD inline int npi_work_size(int G, int cap)
+// workspace doubles one list needs: value, active, active_end, candidate, combined [5][n], per-level sums
+// [NPI_LEVELS][n], active_by_type [cap][3]
+MGO_HD inline int npi_work_size(int n, int cap)
 {
-    return (5 + NPI_LEVELS) * G * G + 3 * cap;
+    return (5 + NPI_LEVELS) * n + 3 * cap;
 }
 
 // key order of Dampings::add_ (damping.h:602-613): (time, type, level)
@@ -52,11 +54,11 @@ MGO_HD inline bool npi_key_less(double ta, int ya, int la, double tb, int yb, in
 
 // Dampings::add_ with insert_sorted_replace (utils/stl_util.h:82-96): an entry with an equal key is
 // replaced, else the new entry is inserted at its sorted position. false: the list is full.
-MGO_HD inline bool npi_list_add(double* list, int& count, int cap, int G, double time, int level, int type,
+MGO_HD inline bool npi_list_add(double* list, int& count, int cap, int n, double time, int level, int type,
                                 const double* coeffs)
 {
 #pragma clang fp contract(off)
-    const int es = npi_entry_size(G), n = G * G;
+    const int es = npi_entry_size(n);
     int pos      = 0;
     while (pos < count) {
         const double* e = list + pos * es;
@@ -91,9 +93,9 @@ MGO_HD inline bool npi_list_add(double* list, int& count, int cap, int G, double
 }
 
 // Dampings::remove (damping.h:336-341)
-MGO_HD inline void npi_list_remove(double* list, int& count, int G, int idx)
+MGO_HD inline void npi_list_remove(double* list, int& count, int n, int idx)
 {
-    const int es = npi_entry_size(G);
+    const int es = npi_entry_size(n);
     for (int q = idx; q + 1 < count; ++q) {
         for (int k = 0; k < es; ++k) {
             list[q * es + k] = list[(q + 1) * es + k];
@@ -103,9 +105,9 @@ MGO_HD inline void npi_list_remove(double* list, int& count, int G, int idx)
 }
 
 // get_active_damping (dynamic_npis.h:291-303): the last entry of (level, type) with time <= t, or -1
-MGO_HD inline int npi_list_active(const double* list, int count, int G, int level, int type, double t)
+MGO_HD inline int npi_list_active(const double* list, int count, int n, int level, int type, double t)
 {
-    const int es = npi_entry_size(G);
+    const int es = npi_entry_size(n);
     for (int q = count - 1; q >= 0; --q) {
         const double* e = list + q * es;
         if (int(e[1]) == level && int(e[2]) == type && e[0] <= t) {
@@ -125,11 +127,11 @@ MGO_HD inline bool npi_time_equal(double a, double b)
 // Dampings::update_cache (damping.h:570-600) with update_active_dampings (damping.h:616-658) and
 // inclusive_exclusive_sum (damping.h:549-566). Returns the number of cache entries, or -1 when the list
 // has more than NPI_LEVELS distinct levels.
-MGO_HD inline int npi_build_cache(const double* list, int count, int G, int cap, double* cache_t, double* cache_m,
+MGO_HD inline int npi_build_cache(const double* list, int count, int n, int cap, double* cache_t, double* cache_m,
                                   double* work)
 {
 #pragma clang fp contract(off)
-    const int es = npi_entry_size(G), n = G * G;
+    const int es = npi_entry_size(n);
     double* combined = work + 4 * n;
     double* sums     = work + 5 * n;                   // [NPI_LEVELS][n]
     double* abt      = work + (5 + NPI_LEVELS) * n;    // active_by_type: [cap][level, type, entry]
@@ -204,14 +206,16 @@ MGO_HD inline int npi_build_cache(const double* list, int count, int G, int cap,
     return m + 1;
 }
 
-// implement_dynamic_npis (dynamic_npis.h:318-392) on the node's one contact matrix with the dampings of one
-// threshold (damp: [ndamp][3 + G] value, level, type, group weights), make_contact_damping_matrix
-// (damping_sampling.h:282-288) of value * weights. false: the list is full.
-MGO_HD inline bool npi_implement(double* list, int& count, int cap, int G, const double* damp, int ndamp,
-                                 double begin, double end, double* work)
+// implement_dynamic_npis (dynamic_npis.h:318-392) on one damping list with the dampings of one threshold
+// (damp: [ndamp][3 + G] value, level, type, group weights). contact: the list is a contact matrix's
+// (n = G*G, make_contact_damping_matrix of value * weights, damping_sampling.h:282-288); else per-group
+// mobility dampings (n = G, value * weights, the distinct values of make_mobility_damping_vector).
+// false: the list is full.
+MGO_HD inline bool npi_implement(double* list, int& count, int cap, int G, bool contact, const double* damp,
+                                 int ndamp, double begin, double end, double* work)
 {
 #pragma clang fp contract(off)
-    const int es = npi_entry_size(G), n = G * G;
+    const int n = contact ? G * G : G, es = npi_entry_size(n);
     double* value      = work;
     double* active     = work + n;
     double* active_end = work + 2 * n;
@@ -219,15 +223,22 @@ MGO_HD inline bool npi_implement(double* list, int& count, int cap, int G, const
     for (int d = 0; d < ndamp; ++d) {
         const double* s = damp + d * (3 + G);
         const int lev = int(s[1]), typ = int(s[2]);
-        for (int i = 0; i < G; ++i) {
-            for (int j = 0; j < G; ++j) {
-                const double gi = s[0] * s[3 + i];
-                const double gj = s[0] * s[3 + j];
-                value[i * G + j] = 1.0 - sqrt((1.0 - gi) * (1.0 - gj));
+        if (contact) {
+            for (int i = 0; i < G; ++i) {
+                for (int j = 0; j < G; ++j) {
+                    const double gi = s[0] * s[3 + i];
+                    const double gj = s[0] * s[3 + j];
+                    value[i * G + j] = 1.0 - sqrt((1.0 - gi) * (1.0 - gj));
+                }
             }
         }
-        const int ia = npi_list_active(list, count, G, lev, typ, begin);
-        const int ie = npi_list_active(list, count, G, lev, typ, end);
+        else {
+            for (int i = 0; i < G; ++i) {
+                value[i] = s[0] * s[3 + i];
+            }
+        }
+        const int ia = npi_list_active(list, count, n, lev, typ, begin);
+        const int ie = npi_list_active(list, count, n, lev, typ, end);
         for (int k = 0; k < n; ++k) {
             active[k]     = ia >= 0 ? list[ia * es + 3 + k] : 0.0;
             active_end[k] = ie >= 0 ? list[ie * es + 3 + k] : 0.0;
@@ -241,7 +252,7 @@ MGO_HD inline bool npi_implement(double* list, int& count, int cap, int G, const
             for (int k = 0; k < n; ++k) {
                 cand[k] = (value[k] < active[k]) ? active[k] : value[k]; // max(value, active)
             }
-            if (!npi_list_add(list, count, cap, G, begin, lev, typ, cand)) {
+            if (!npi_list_add(list, count, cap, n, begin, lev, typ, cand)) {
                 return false;
             }
             implemented = true;
@@ -256,7 +267,7 @@ MGO_HD inline bool npi_implement(double* list, int& count, int cap, int G, const
                 implemented = true;
             }
         }
-        if (implemented && !npi_list_add(list, count, cap, G, end, lev, typ, active_end)) {
+        if (implemented && !npi_list_add(list, count, cap, n, end, lev, typ, active_end)) {
             return false;
         }
     }
@@ -273,7 +284,7 @@ MGO_HD inline bool npi_implement(double* list, int& count, int cap, int G, const
                     same = same && (ep[3 + k] == ec[3 + k]);
                 }
                 if (same) {
-                    npi_list_remove(list, count, G, cur);
+                    npi_list_remove(list, count, n, cur);
                 }
                 break;
             }

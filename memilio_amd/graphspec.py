@@ -62,6 +62,9 @@ class GraphSpec:
         self.edge_end = np.zeros(0, np.int32)
         self.edge_coeffs = np.zeros(0)   # [E] (one value per edge) or [E][10G]
         self.npis = None                 # dynamic NPI arrays (set_dynamic_npis)
+        self.edge_coeff_minimum = None   # [E] or [E][10G] minimum mobility coefficients
+        self.out_dampings = None         # per-node out-edge damping arrays (set_out_dampings)
+        self.out_npis = None             # per-node out-edge NPI arrays (set_out_dynamic_npis)
         self._desc = None
 
     # ---------------------------------------------------------------- building
@@ -90,6 +93,33 @@ class GraphSpec:
         per_node: list (len num_nodes) of None or dicts with keys ``thresholds`` (list of (value, dampings),
         dampings = list of (value, level, type, group_weights[G])), and optionally ``duration`` (14),
         ``delay`` (0), ``base_value`` (1), ``directive_begin`` (lowest), ``directive_end`` (max)."""
+        self.npis = self._npi_arrays(per_node)
+        self._desc = None
+
+    def set_out_dynamic_npis(self, per_node):
+        """Dynamic NPIs of every out-edge of a node (MobilityParameters::set_dynamic_npis_infected); same
+        per-node form as set_dynamic_npis (apply_mobility uses duration and base_value)."""
+        self.out_npis = self._npi_arrays(per_node)
+        self._desc = None
+
+    def set_out_dampings(self, per_node):
+        """Dampings of the mobility coefficients of every out-edge of a node: per node a list of
+        (time, level, type, values[G]) (the value of group g applies to its 10 compartments)."""
+        G = self.G
+        offs, times, levels, types, coeffs = [0], [], [], [], []
+        for lst in per_node:
+            for (t, lev, typ, c) in (lst or []):
+                times.append(float(t))
+                levels.append(int(lev))
+                types.append(int(typ))
+                coeffs.append(np.broadcast_to(np.asarray(c, dtype=np.float64), (G,)))
+            offs.append(len(times))
+        self.out_dampings = {"offsets": np.asarray(offs, np.int32), "times": np.asarray(times, np.float64),
+                             "levels": np.asarray(levels, np.int32), "types": np.asarray(types, np.int32),
+                             "coeffs": np.ascontiguousarray(np.stack(coeffs) if coeffs else np.zeros((0, G)))}
+        self._desc = None
+
+    def _npi_arrays(self, per_node):
         G = self.G
         node_offsets, params, thresholds, damp_offsets = [0], [], [], [0]
         values, levels, types, weights = [], [], [], []
@@ -111,7 +141,7 @@ class GraphSpec:
                     weights.append(np.broadcast_to(np.asarray(w, dtype=np.float64), (G,)))
                 damp_offsets.append(len(values))
             node_offsets.append(len(thresholds))
-        self.npis = {
+        return {
             "node_offsets": np.asarray(node_offsets, np.int32),
             "node_params": np.ascontiguousarray(np.stack(params), dtype=np.float64),
             "thresholds": np.asarray(thresholds, np.float64),
@@ -121,7 +151,6 @@ class GraphSpec:
             "damping_types": np.asarray(types, np.int32),
             "damping_weights": np.ascontiguousarray(np.stack(weights) if weights else np.zeros((0, G))),
         }
-        self._desc = None
 
     def set_edges(self, start, end, coeffs):
         """coeffs: [E] (one coefficient for every compartment of the edge) or [E][10G]."""
@@ -174,13 +203,28 @@ class GraphSpec:
             d.edge_end = capi.iptr(self.edge_end) if self.num_edges else None
             d.edge_coeffs = capi.dptr(self.edge_coeffs) if self.num_edges else None
             d.edge_coeff_len = 1 if self.edge_coeffs.ndim == 1 else NC * self.G
-            if self.npis is not None:
-                n = self.npis
-                self._npi_struct = capi.DynamicNpis(
+            def npi_struct(n):
+                return capi.DynamicNpis(
                     capi.iptr(n["node_offsets"]), capi.dptr(n["node_params"]), capi.dptr(n["thresholds"]),
                     capi.iptr(n["damping_offsets"]), capi.dptr(n["damping_values"]), capi.iptr(n["damping_levels"]),
                     capi.iptr(n["damping_types"]), capi.dptr(n["damping_weights"]))
+            if self.npis is not None:
+                self._npi_struct = npi_struct(self.npis)
                 d.dynamic_npis = C.pointer(self._npi_struct)
+            if self.out_npis is not None:
+                self._out_npi_struct = npi_struct(self.out_npis)
+                d.out_dynamic_npis = C.pointer(self._out_npi_struct)
+            if self.edge_coeff_minimum is not None and self.num_edges:
+                self.edge_coeff_minimum = np.ascontiguousarray(self.edge_coeff_minimum, dtype=np.float64)
+                assert self.edge_coeff_minimum.shape == self.edge_coeffs.shape
+                d.edge_coeff_minimum = capi.dptr(self.edge_coeff_minimum)
+            if self.out_dampings is not None:
+                o = self.out_dampings
+                d.out_damping_offsets = capi.iptr(o["offsets"])
+                d.out_damping_times = capi.dptr(o["times"])
+                d.out_damping_levels = capi.iptr(o["levels"])
+                d.out_damping_types = capi.iptr(o["types"])
+                d.out_damping_coeffs = capi.dptr(o["coeffs"])
             self._desc = d
         return self._desc
 

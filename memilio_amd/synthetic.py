@@ -167,3 +167,18 @@ def with_npis(spec, thresholds=((200.0, 0.3), (800.0, 0.6)), duration=14.0, dela
         cfg["directive_end"] = directive_end
     spec.set_dynamic_npis([cfg] * spec.num_nodes)
     return spec
+
+
+def with_edge_dampings(spec, dampings=((5.0, 0, 0, 0.3), (12.0, 1, 0, 0.5)), thresholds=((300.0, 0.4),),
+                       duration=7.0, base_value=100000.0, minimum=0.2):
+    """Mobility-coefficient dampings and edge dynamic NPIs on every out-edge (as osecir::draw_sample sets
+    them): dampings (time, level, type, value for every group), NPIs of one damping per threshold, minimum
+    coefficients = minimum * baseline. Returns spec."""
+    G = spec.G
+    spec.set_out_dampings([[(t, lev, typ, np.full(G, v)) for (t, lev, typ, v) in dampings]] * spec.num_nodes)
+    if thresholds:
+        spec.set_out_dynamic_npis([{"thresholds": [(thr, [(val, 0, 0, np.ones(G))]) for thr, val in thresholds],
+                                    "duration": duration, "base_value": base_value}] * spec.num_nodes)
+    spec.edge_coeff_minimum = minimum * np.asarray(spec.edge_coeffs, dtype=np.float64)
+    spec._desc = None
+    return spec

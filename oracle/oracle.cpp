@@ -268,6 +268,17 @@ static Vec make_contact_damping_matrix(const Vec& g)
     return m;
 }
 
+// make_mobility_damping_vector (damping_sampling.h:298-310): [g_0 (rows / G times), g_1, ...]
+static Vec make_mobility_damping_vector(size_t rows, const Vec& g)
+{
+    Vec v(rows);
+    const size_t per = rows / g.size();
+    for (size_t k = 0; k < rows; ++k) {
+        v[k] = g[k / per];
+    }
+    return v;
+}
+
 // get_active_damping (dynamic_npis.h:291-303): coefficients of the last damping of (level, type) with
 // time <= t, zeros if none
 static Vec get_active_damping(const Dampings& d, int level, int type, double t)
@@ -308,7 +319,9 @@ static void implement_dynamic_npis(std::vector<Dampings*>& group, const std::vec
             for (size_t k = 0; k < g.size(); ++k) {
                 g[k] = npi.value * npi.weights[k];
             }
-            Vec value = make_matrix == 1 ? make_contact_damping_matrix(g) : g;
+            Vec value = make_matrix == 1 ? make_contact_damping_matrix(g)
+                        : make_matrix == 2 ? make_mobility_damping_vector(size_t(expr.rows()), g)
+                                           : g;
             bool implemented = false;
             bool any_less    = false;
             for (size_t k = 0; k < value.size(); ++k) {
@@ -1141,19 +1154,49 @@ struct SimulationNode {
 // MobilityEdge with constant coefficients (metapopulation_mobility_instant.h:295-623); edge dynamic NPIs
 // and saved commuter indices are not on this path.
 struct MobilityEdge {
-    Vec coeffs;
+    Vec coeffs;         // baseline B of the MobilityCoefficients
+    Vec minimum;        // minimum M
+    Dampings dampings;  // vector-shaped dampings of the coefficients
+    DynamicNPIs npis;   // MobilityParameters::get_dynamic_npis_infected
+    double npi_value = std::numeric_limits<double>::lowest(), npi_end = 0.0; // m_dynamic_npi
     TimeSeries mobile;
     Vec return_times;
     bool return_mobile_population = false;
     explicit MobilityEdge(const Vec& c)
         : coeffs(c)
+        , minimum(c.size(), 0.0)
+        , dampings(int(c.size()), 1)
         , mobile(int(c.size()))
     {
+    }
+
+    // DampingMatrixExpression::get_matrix_at (contact_matrix.h:190-194) of the coefficient vector
+    Vec coeffs_at(double t) const
+    {
+        Vec c(coeffs.size());
+        for (size_t j = 0; j < c.size(); ++j) {
+            c[j] = coeffs[j] - (dampings.at(t, int(j), 0) * (coeffs[j] - minimum[j]));
+        }
+        return c;
     }
 
     void apply_mobility(double t, double dt, SimulationNode& from, SimulationNode& to, orc_stats* st)
     {
         const int n = mobile.n;
+        // edge dynamic NPIs (:549-566): checked on the source's last state at every call
+        if (!npis.thresholds.empty()) {
+            auto* om = dynamic_cast<const OsecirModel*>(from.sim.model.get());
+            const double inf_rel = Simulation::infections_relative(*om, from.last_state.data()) * npis.base;
+            const int k          = npis.max_exceeded(inf_rel);
+            const bool expired   = floating_point_greater_equal(t, npi_end, 1e-10);
+            if (k >= 0 && (npis.thresholds[size_t(k)].first > npi_value || expired)) {
+                const double t_end = t + npis.duration;
+                npi_value          = npis.thresholds[size_t(k)].first;
+                npi_end            = t_end;
+                std::vector<Dampings*> group{&dampings};
+                implement_dynamic_npis(group, npis.thresholds[size_t(k)].second, t, t_end, 2);
+            }
+        }
         // returns (:568-603)
         for (int i = mobile.size() - 1; i >= 0; --i) {
             if (return_times[size_t(i)] <= t) {
@@ -1211,8 +1254,9 @@ struct MobilityEdge {
             }
         }
         // departures (:605-621)
+        const Vec cat     = coeffs_at(t);
         bool any_positive = false;
-        for (double c : coeffs) {
+        for (double c : cat) {
             any_positive = any_positive || (c > 0.0);
         }
         if (!return_mobile_population && any_positive) {
@@ -1220,7 +1264,7 @@ struct MobilityEdge {
             from.sim.model->mobility_factors(from.last_state.data(), factors.data());
             Vec m(n);
             for (int j = 0; j < n; ++j) {
-                m[j] = from.last_state[j] * coeffs[j] * factors[j];
+                m[j] = from.last_state[j] * cat[j] * factors[j];
             }
             mobile.add_time_point(t, m.data());
             return_times.push_back(t + dt);
@@ -1254,7 +1298,11 @@ struct GraphSim {
 
     void add_edge(int s, int e, const Vec& c)
     {
-        EdgeEntry item{s, e, MobilityEdge(c)};
+        add_edge(s, e, MobilityEdge(c));
+    }
+    void add_edge(int s, int e, const MobilityEdge& me)
+    {
+        EdgeEntry item{s, e, me};
         auto less = [](const EdgeEntry& a, const EdgeEntry& b) {
             return a.start == b.start ? a.end < b.end : a.start < b.start;
         };
@@ -1427,11 +1475,45 @@ static void build_graph(const mgo_graph_desc* g, const mgo_sim_cfg* cfg, GraphSi
     }
     const size_t n   = size_t(g->num_groups) * NC;
     const size_t len = (g->edge_coeff_len == 1) ? 1 : n;
+    const int G = g->num_groups;
     for (int e = 0; e < g->num_edges; ++e) {
         // MobilityEdge(Eigen::VectorX) coefficients; a single value stands for VectorX::Constant(n, c)
         Vec c = (len == 1) ? Vec(n, g->edge_coeffs[e])
                            : Vec(g->edge_coeffs + size_t(e) * n, g->edge_coeffs + size_t(e + 1) * n);
-        sim.add_edge(g->edge_start[e], g->edge_end[e], c);
+        MobilityEdge me(c);
+        if (g->edge_coeff_minimum) {
+            me.minimum = (len == 1) ? Vec(n, g->edge_coeff_minimum[e])
+                                    : Vec(g->edge_coeff_minimum + size_t(e) * n,
+                                          g->edge_coeff_minimum + size_t(e + 1) * n);
+        }
+        // the source node's out-edge dampings (per age group, make_mobility_damping_vector) and NPIs
+        const int src = g->edge_start[e];
+        if (g->out_damping_offsets) {
+            for (int k = g->out_damping_offsets[src]; k < g->out_damping_offsets[src + 1]; ++k) {
+                Vec gv(g->out_damping_coeffs + size_t(k) * G, g->out_damping_coeffs + size_t(k + 1) * G);
+                me.dampings.add(Damping{make_mobility_damping_vector(n, gv), g->out_damping_levels[k],
+                                        g->out_damping_types[k], g->out_damping_times[k]});
+            }
+        }
+        if (const mgo_dynamic_npis* dn = g->out_dynamic_npis) {
+            const double* par = dn->node_params + size_t(src) * MGO_NPI_NUM_PARAMS;
+            me.npis.duration  = par[MGO_NPI_DURATION];
+            me.npis.base      = par[MGO_NPI_BASE_VALUE];
+            for (int t = dn->node_offsets[src]; t < dn->node_offsets[src + 1]; ++t) {
+                std::vector<NpiDamping> ds;
+                for (int d = dn->damping_offsets[t]; d < dn->damping_offsets[t + 1]; ++d) {
+                    NpiDamping nd;
+                    nd.value = dn->damping_values[d];
+                    nd.level = dn->damping_levels[d];
+                    nd.type  = dn->damping_types[d];
+                    nd.mats  = {0};
+                    nd.weights.assign(dn->damping_weights + size_t(d) * G, dn->damping_weights + size_t(d + 1) * G);
+                    ds.push_back(nd);
+                }
+                me.npis.set_threshold(dn->thresholds[t], ds);
+            }
+        }
+        sim.add_edge(g->edge_start[e], g->edge_end[e], me);
     }
 }
 
@@ -1745,6 +1827,41 @@ int32_t orc_implement_npis(int32_t rows, int32_t cols, int32_t num_mats, const i
         out_offsets[m + 1] = n;
     }
     return n;
+}
+
+// The edge dynamic-NPI check of MobilityEdge::apply_mobility (metapopulation_mobility_instant.h:549-566) over
+// a sequence of calls (times[c], infections relative of the source inf_rel[c]; the known-answer scenario of
+// cpp/tests/test_dynamic_npis.cpp:319-376): one threshold of one damping (value, level 0, type 0, weights
+// [num_weights]) on a coefficient vector of n entries. Writes the number of dampings after each call.
+int32_t orc_edge_npi_calls(int32_t n, double threshold, double base, double duration, double value,
+                           int32_t num_weights, const double* weights, int32_t num_calls, const double* times,
+                           const double* inf_rel, int32_t* out_num_dampings)
+{
+    Dampings d(n, 1);
+    DynamicNPIs npis;
+    npis.duration = duration;
+    npis.base     = base;
+    NpiDamping s;
+    s.value = value;
+    s.level = 0;
+    s.type  = 0;
+    s.mats  = {0};
+    s.weights.assign(weights, weights + num_weights);
+    npis.set_threshold(threshold, {s});
+    double npi_value = std::numeric_limits<double>::lowest(), npi_end = 0.0;
+    for (int c = 0; c < num_calls; ++c) {
+        const double t  = times[c];
+        const int k     = npis.max_exceeded(inf_rel[c] * npis.base);
+        const bool exp_ = floating_point_greater_equal(t, npi_end, 1e-10);
+        if (k >= 0 && (npis.thresholds[size_t(k)].first > npi_value || exp_)) {
+            npi_value = npis.thresholds[size_t(k)].first;
+            npi_end   = t + npis.duration;
+            std::vector<Dampings*> group{&d};
+            implement_dynamic_npis(group, npis.thresholds[size_t(k)].second, t, npi_end, 2);
+        }
+        out_num_dampings[c] = int32_t(d.list().size());
+    }
+    return MGO_OK;
 }
 
 // osecir::Simulation::advance(tmax) with dynamic NPIs over the reference tests' MockSimulation (state constant

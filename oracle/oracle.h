@@ -88,6 +88,12 @@ int32_t orc_implement_npis(int32_t rows, int32_t cols, int32_t num_mats, const i
 int32_t orc_npi_frozen_run(const mgo_graph_desc* model, double t0, double tmax, int32_t num_query,
                            const double* query_times, double* out_c00, int32_t* out_num_dampings);
 
+/* Edge dynamic-NPI checks of MobilityEdge::apply_mobility over a sequence of calls (times, infections
+ * relative of the source); number of dampings of the coefficient vector after each call. */
+int32_t orc_edge_npi_calls(int32_t n, double threshold, double base, double duration, double value,
+                           int32_t num_weights, const double* weights, int32_t num_calls, const double* times,
+                           const double* inf_rel, int32_t* out_num_dampings);
+
 /* Effective contact matrix entry of node 0 at time t (ContactMatrixGroup::get_matrix_at(t)(i, j)). */
 double orc_contact_at(const mgo_graph_desc* model, double t, int32_t i, int32_t j);
 

@@ -64,8 +64,24 @@ def lib():
                                          _ip, _dp, _ip, _ip, _dp]
         L.orc_npi_frozen_run.restype = C.c_int32
         L.orc_npi_frozen_run.argtypes = [C.POINTER(capi.GraphDesc), C.c_double, C.c_double, C.c_int32, _dp, _dp, _ip]
+        L.orc_edge_npi_calls.restype = C.c_int32
+        L.orc_edge_npi_calls.argtypes = [C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int32, _dp,
+                                         C.c_int32, _dp, _dp, _ip]
         _lib = L
     return _lib
+
+
+def edge_npi_calls(n, threshold, base, duration, value, weights, times, inf_rel):
+    """Number of coefficient dampings after each apply_mobility NPI check (see orc_edge_npi_calls)."""
+    w = np.ascontiguousarray(weights, dtype=np.float64)
+    t = np.ascontiguousarray(times, dtype=np.float64)
+    r = np.ascontiguousarray(inf_rel, dtype=np.float64)
+    out = np.zeros(len(t), np.int32)
+    st = lib().orc_edge_npi_calls(n, threshold, base, duration, value, len(w), capi.dptr(w), len(t), capi.dptr(t),
+                                  capi.dptr(r), capi.iptr(out))
+    if st != 0:
+        raise RuntimeError(f"orc_edge_npi_calls: {st}")
+    return out.tolist()
 
 
 def implement_npis(rows, cols, lists, npis, begin, end, make_matrix=0, capacity=256):

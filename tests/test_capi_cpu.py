@@ -33,7 +33,7 @@ def test_version_string():
 def test_struct_sizes():
     # plain C layout of the descriptors (x86-64): pointers 8, int32 4
     # 2 int32 + 10 pointers + int32 (padded) + 3 pointers + int32 (padded) + the dynamic NPI pointer
-    assert C.sizeof(capi.GraphDesc) == 8 + 10 * 8 + 8 + 3 * 8 + 8 + 8
+    assert C.sizeof(capi.GraphDesc) == 8 + 10 * 8 + 8 + 3 * 8 + 8 + 8 + 7 * 8
     assert C.sizeof(capi.DynamicNpis) == 8 * 8
     assert C.sizeof(capi.IntegratorCfg) == 40
     assert C.sizeof(capi.SimCfg) == 32 + 40

@@ -125,3 +125,45 @@ def test_npi_serving(oracle):
     assert np.array_equal(served, batched)
     ref, _ = oracle.graph_run(spec, cfg, 30.0)
     assert rel_err(served, ref) < 1e-6
+
+
+# ------------------------------------------------------------------------------- out-edge dampings, edge NPIs
+def _edge_graph(seed, n=30, G=3, **kw):
+    return synthetic.with_edge_dampings(synthetic.graph(num_nodes=n, G=G, seed=seed, mean_degree=6.0), **kw)
+
+
+def test_edge_dampings_fixed_step(ctx, oracle):
+    """Mobility-coefficient dampings (with minimum coefficients) and edge dynamic NPIs: fixed step 1e-12."""
+    spec = _edge_graph(64)
+    cfg = synthetic.fixed_cfg(0.1)
+    ref, _ = oracle.graph_run(spec, cfg, 40.0)
+    got = _device_graph(ctx, spec, cfg, 40.0)
+    assert rel_err(got, ref) < 1e-12
+
+
+def test_edge_dampings_adaptive_with_node_npis(ctx, oracle):
+    """Edge dampings and NPIs together with node NPIs (off-grid), per-edge coefficient vectors, adaptive."""
+    spec = synthetic.with_npis(_edge_graph(65, G=4, dampings=((3.5, 0, 0, 0.6), (8.0, 0, 1, 0.2))), **OFF_GRID)
+    c = spec.edge_coeff_matrix() * np.linspace(0.5, 1.5, spec.edge_coeffs.shape[0])[:, None]
+    spec.set_edges(spec.edge_start, spec.edge_end, c)
+    spec.edge_coeff_minimum = 0.1 * c
+    spec._desc = None
+    cfg = synthetic.ck_cfg(1e-8)
+    ref, _ = oracle.graph_run(spec, cfg, 35.0)
+    got = _device_graph(ctx, spec, cfg, 35.0)
+    assert rel_err(got, ref) < 1e-6
+
+
+def test_edge_dampings_batch_reset(ctx):
+    """Edge damping lists and NPI states are restored by a batch reset."""
+    specs = [_edge_graph(66 + k, n=15 + 5 * k) for k in range(3)]
+    cfgs = [synthetic.ck_cfg(1e-8)] * 3
+    b = Batch(ctx, specs, cfgs, [30.0] * 3)
+    b.run()
+    first = b.read_daily()
+    b.reset()
+    b.run()
+    second = b.read_daily()
+    b.close()
+    for x, y in zip(first, second):
+        assert np.array_equal(x, y)

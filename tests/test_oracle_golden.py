@@ -267,3 +267,20 @@ def test_npi_scenario_triggers(oracle):
         a, _ = oracle.graph_run(g, synthetic.fixed_cfg(0.1), 60.0)
         b, _ = oracle.graph_run(g0, synthetic.fixed_cfg(0.1), 60.0)
         assert np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-6)) > 1e-2
+
+
+def test_npi_mobility_edge(oracle):
+    """test_dynamic_npis.cpp:319-376: the edge's NPI check at each apply_mobility call (source state (0, 1)
+    at t = 0.5, (0.01, 0.99) at 3, (0.02, 0.98) at 4.5 and 6; threshold 1500 per 100,000, duration 5)."""
+    counts = oracle.edge_npi_calls(2, 0.015 * 100000, 100000, 5.0, 1.0, [1.0, 1.0], [0.5, 3.0, 4.5, 6.0],
+                                   [0.0, 0.01, 0.02, 0.02])
+    assert counts == [0, 0, 2, 2]
+
+
+def test_mobility_dampings_scenario_triggers(oracle):
+    """The graph scenario of the device edge-damping tests: out-edge dampings and edge NPIs change the run."""
+    g = synthetic.with_edge_dampings(synthetic.graph(num_nodes=30, G=3, seed=64, mean_degree=6.0))
+    g0 = synthetic.graph(num_nodes=30, G=3, seed=64, mean_degree=6.0)
+    a, _ = oracle.graph_run(g, synthetic.fixed_cfg(0.1), 40.0)
+    b, _ = oracle.graph_run(g0, synthetic.fixed_cfg(0.1), 40.0)
+    assert np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-6)) > 1e-3
