@@ -1181,27 +1181,146 @@ private:
     std::vector<Edge<EdgePropertyT>> m_edges;
 };
 
-// MobilityParameters: one coefficient per compartment (10*G), MobilityEdge owns them
+// MobilityCoefficients (metapopulation_mobility_instant.h:106-110): baseline B, minimum M and dampings of one
+// coefficient vector (10 * G values); the coefficients at time t are B - D(t) (B - M).
+struct MobilityDamping {
+    Vector coeffs;
+    int level = 0;
+    int type  = 0;
+    double time = 0.0;
+};
+
+class MobilityCoefficients
+{
+public:
+    MobilityCoefficients() = default;
+    template <class V>
+    explicit MobilityCoefficients(const V& baseline)
+        : m_baseline(detail::to_vector(baseline))
+        , m_minimum(m_baseline.size(), 0.0)
+    {
+    }
+    template <class V, class W>
+    MobilityCoefficients(const V& baseline, const W& minimum)
+        : m_baseline(detail::to_vector(baseline))
+        , m_minimum(detail::to_vector(minimum))
+    {
+    }
+    Vector& get_baseline()
+    {
+        return m_baseline;
+    }
+    const Vector& get_baseline() const
+    {
+        return m_baseline;
+    }
+    Vector& get_minimum()
+    {
+        return m_minimum;
+    }
+    const Vector& get_minimum() const
+    {
+        return m_minimum;
+    }
+    // a scalar value damps every entry
+    template <class V>
+    void add_damping(const V& coeffs, DampingLevel level, DampingType type, SimulationTime t)
+    {
+        MobilityDamping d;
+        if constexpr (std::is_arithmetic_v<V>) {
+            d.coeffs = Vector(m_baseline.size(), double(coeffs));
+        }
+        else {
+            d.coeffs = detail::to_vector(coeffs);
+        }
+        d.level = level.get();
+        d.type  = type.get();
+        d.time  = t.get();
+        m_dampings.push_back(d);
+    }
+    const std::vector<MobilityDamping>& get_dampings() const
+    {
+        return m_dampings;
+    }
+
+private:
+    Vector m_baseline, m_minimum;
+    std::vector<MobilityDamping> m_dampings;
+};
+
+// MobilityCoefficientGroup: coefficient vectors whose values add up; add_damping damps every one
+class MobilityCoefficientGroup
+{
+public:
+    MobilityCoefficientGroup(size_t num_matrices, size_t num_elements)
+        : m_mats(num_matrices, MobilityCoefficients(Vector(num_elements, 0.0)))
+    {
+    }
+    MobilityCoefficients& operator[](size_t k)
+    {
+        return m_mats.at(k);
+    }
+    const MobilityCoefficients& operator[](size_t k) const
+    {
+        return m_mats.at(k);
+    }
+    size_t get_num_matrices() const
+    {
+        return m_mats.size();
+    }
+    template <class V>
+    void add_damping(const V& coeffs, DampingLevel level, DampingType type, SimulationTime t)
+    {
+        for (auto& m : m_mats) {
+            m.add_damping(coeffs, level, type, t);
+        }
+    }
+
+private:
+    std::vector<MobilityCoefficients> m_mats;
+};
+
+// MobilityParameters (metapopulation_mobility_instant.h:113-290): the coefficient group and the edge's dynamic
+// NPIs; MobilityEdge owns them
 template <class FP = double>
 class MobilityParameters
 {
 public:
+    explicit MobilityParameters(const MobilityCoefficientGroup& coeffs)
+        : m_coeffs(coeffs)
+    {
+    }
     template <class V>
+        requires(!std::is_same_v<std::decay_t<V>, MobilityCoefficientGroup>)
     explicit MobilityParameters(const V& coeffs)
-        : m_coeffs(detail::to_vector(coeffs))
+        : m_coeffs(1, 0)
     {
+        m_coeffs[0] = MobilityCoefficients(coeffs);
     }
-    const Vector& get_coefficients() const
+    const MobilityCoefficientGroup& get_coefficients() const
     {
         return m_coeffs;
     }
-    Vector& get_coefficients()
+    MobilityCoefficientGroup& get_coefficients()
     {
         return m_coeffs;
+    }
+    const DynamicNPIs<FP>& get_dynamic_npis_infected() const
+    {
+        return m_npis;
+    }
+    DynamicNPIs<FP>& get_dynamic_npis_infected()
+    {
+        return m_npis;
+    }
+    void set_dynamic_npis_infected(const DynamicNPIs<FP>& v)
+    {
+        m_npis = v;
     }
 
 private:
-    Vector m_coeffs;
+    MobilityCoefficientGroup m_coeffs;
+    DynamicNPIs<FP> m_npis;
 };
 
 template <class FP = double>
@@ -1370,6 +1489,13 @@ struct GraphArrays {
     std::vector<double> npi_params, npi_thr, npi_vals, npi_weights;
     std::vector<int32_t> npi_offs, npi_doffs, npi_levels, npi_types;
     std::unique_ptr<mgo_dynamic_npis> npis; // on the heap: desc keeps pointing at it when the arrays move
+    // out-edge dampings and edge NPIs per source node (add_edge), edge minimum coefficients
+    std::vector<double> emin;
+    std::vector<std::vector<double>> out_lists, out_npis;
+    std::vector<char> out_set;
+    std::vector<double> od_times, od_coeffs, onpi_params, onpi_thr, onpi_vals, onpi_weights;
+    std::vector<int32_t> od_offs, od_levels, od_types, onpi_offs, onpi_doffs, onpi_levels, onpi_types;
+    std::unique_ptr<mgo_dynamic_npis> out_npi_desc;
     mgo_integrator_cfg integrator{};
     double dt_node = 0.1;
     mgo_graph_desc desc{};
@@ -1506,12 +1632,145 @@ struct GraphArrays {
 
     void add_edge(size_t s, size_t e, const Vector& c)
     {
-        if (c.size() != size_t(G) * MGO_NUM_COMPARTMENTS) {
-            throw Error(MGO_ERR_INVALID_VALUE, "mobility coefficients need 10 * G values");
+        add_edge(s, e, MobilityParameters<double>(c));
+    }
+
+    // the folded coefficient group (B, M), and the source node's out-edge dampings (per age group) and edge
+    // NPIs, which all out-edges of a node share on the device
+    template <class FP>
+    void add_edge(size_t s, size_t e, const MobilityParameters<FP>& p)
+    {
+        const size_t nv             = size_t(G) * MGO_NUM_COMPARTMENTS;
+        const MobilityCoefficientGroup& grp = p.get_coefficients();
+        int damped                  = -1;
+        for (size_t k = 0; k < grp.get_num_matrices(); ++k) {
+            if (grp[k].get_baseline().size() != nv || grp[k].get_minimum().size() != nv) {
+                throw Error(MGO_ERR_INVALID_VALUE, "mobility coefficients need 10 * G values");
+            }
+            if (!grp[k].get_dampings().empty()) {
+                if (damped >= 0) {
+                    throw Error(MGO_ERR_UNSUPPORTED, "more than one damped mobility coefficient vector in an edge");
+                }
+                damped = int(k);
+            }
+        }
+        std::vector<double> B(nv, 0.0), M(nv, 0.0);
+        for (size_t k = 0; k < grp.get_num_matrices(); ++k) {
+            for (size_t i = 0; i < nv; ++i) {
+                B[i] += grp[k].get_baseline()[i];
+                M[i] += (damped < 0 || int(k) == damped) ? grp[k].get_minimum()[i] : grp[k].get_baseline()[i];
+            }
+        }
+        // out-edge damping list of the source as per-group values
+        std::vector<double> lst;
+        if (damped >= 0) {
+            for (const MobilityDamping& d : grp[size_t(damped)].get_dampings()) {
+                lst.push_back(d.time);
+                lst.push_back(d.level);
+                lst.push_back(d.type);
+                for (int a = 0; a < G; ++a) {
+                    const double v = d.coeffs[size_t(a) * MGO_NUM_COMPARTMENTS];
+                    for (int c = 1; c < MGO_NUM_COMPARTMENTS; ++c) {
+                        if (d.coeffs[size_t(a) * MGO_NUM_COMPARTMENTS + size_t(c)] != v) {
+                            throw Error(MGO_ERR_UNSUPPORTED, "mobility dampings must be uniform within an age group");
+                        }
+                    }
+                    lst.push_back(v);
+                }
+            }
+        }
+        const DynamicNPIs<FP>& dn = p.get_dynamic_npis_infected();
+        std::vector<double> npi_key;
+        for (const auto& t : dn.get_thresholds()) {
+            npi_key.push_back(t.first);
+            for (const auto& d : t.second) {
+                npi_key.push_back(d.get_value());
+                npi_key.push_back(d.get_level().get());
+                npi_key.push_back(d.get_type().get());
+                for (size_t a = 0; a < d.get_group_weights().size(); ++a) {
+                    npi_key.push_back(d.get_group_weights()[a]);
+                }
+            }
+            npi_key.push_back(std::numeric_limits<double>::quiet_NaN()); // threshold separator
+        }
+        npi_key.push_back(dn.get_duration().get());
+        npi_key.push_back(dn.get_base_value());
+        if (out_lists.size() < size_t(n)) {
+            out_lists.resize(size_t(n));
+            out_npis.resize(size_t(n));
+            out_set.resize(size_t(n), 0);
+        }
+        auto same = [](const std::vector<double>& a, const std::vector<double>& b) {
+            if (a.size() != b.size()) {
+                return false;
+            }
+            for (size_t i = 0; i < a.size(); ++i) {
+                if (!(a[i] == b[i]) && !(std::isnan(a[i]) && std::isnan(b[i]))) {
+                    return false;
+                }
+            }
+            return true;
+        };
+        if (!out_set[s]) {
+            out_set[s]   = 1;
+            out_lists[s] = lst;
+            out_npis[s]  = npi_key;
+        }
+        else if (!same(out_lists[s], lst) || !same(out_npis[s], npi_key)) {
+            throw Error(MGO_ERR_UNSUPPORTED, "out-edges of a node differ in their dampings or dynamic NPIs; the "
+                                             "device shares one list per node");
         }
         estart.push_back(int32_t(s));
         eend.push_back(int32_t(e));
-        ecoeffs.insert(ecoeffs.end(), c.data(), c.data() + c.size());
+        ecoeffs.insert(ecoeffs.end(), B.begin(), B.end());
+        emin.insert(emin.end(), M.begin(), M.end());
+    }
+
+    // per-node out-edge damping and NPI arrays from the lists add_edge recorded (out_npis holds per threshold
+    // value, then (value, level, type, G weights) per damping, then NaN; finally duration and base value)
+    void finish_out_dampings()
+    {
+        od_offs.assign(1, 0);
+        onpi_offs.assign(1, 0);
+        onpi_doffs.assign(1, 0);
+        od_times.clear(), od_coeffs.clear(), od_levels.clear(), od_types.clear();
+        onpi_params.clear(), onpi_thr.clear(), onpi_vals.clear(), onpi_weights.clear(), onpi_levels.clear(),
+            onpi_types.clear();
+        const size_t es = 3 + size_t(G);
+        for (int v = 0; v < n; ++v) {
+            const bool set = size_t(v) < out_set.size() && out_set[size_t(v)];
+            if (set) {
+                const auto& l = out_lists[size_t(v)];
+                for (size_t k = 0; k + es <= l.size(); k += es) {
+                    od_times.push_back(l[k]);
+                    od_levels.push_back(int32_t(l[k + 1]));
+                    od_types.push_back(int32_t(l[k + 2]));
+                    od_coeffs.insert(od_coeffs.end(), l.begin() + long(k + 3), l.begin() + long(k + es));
+                }
+            }
+            od_offs.push_back(int32_t(od_times.size()));
+            double duration = 14.0, base = 1.0;
+            if (set) {
+                const auto& q = out_npis[size_t(v)];
+                duration      = q[q.size() - 2];
+                base          = q[q.size() - 1];
+                size_t i      = 0;
+                while (i + 2 < q.size()) {
+                    onpi_thr.push_back(q[i++]);
+                    while (!std::isnan(q[i])) {
+                        onpi_vals.push_back(q[i]);
+                        onpi_levels.push_back(int32_t(q[i + 1]));
+                        onpi_types.push_back(int32_t(q[i + 2]));
+                        onpi_weights.insert(onpi_weights.end(), q.begin() + long(i + 3), q.begin() + long(i + 3 + size_t(G)));
+                        i += 3 + size_t(G);
+                    }
+                    ++i; // separator
+                    onpi_doffs.push_back(int32_t(onpi_vals.size()));
+                }
+            }
+            onpi_params.insert(onpi_params.end(), {duration, 0.0, base, -DBL_MAX, DBL_MAX});
+            onpi_offs.push_back(int32_t(onpi_thr.size()));
+        }
     }
 
     const mgo_graph_desc& finish()
@@ -1534,6 +1793,31 @@ struct GraphArrays {
         desc.edge_end         = eend.empty() ? nullptr : eend.data();
         desc.edge_coeffs      = ecoeffs.empty() ? nullptr : ecoeffs.data();
         desc.edge_coeff_len   = G * MGO_NUM_COMPARTMENTS;
+        finish_out_dampings();
+        bool any_min = false;
+        for (double m : emin) {
+            any_min = any_min || m != 0.0;
+        }
+        desc.edge_coeff_minimum = any_min ? emin.data() : nullptr;
+        if (!od_times.empty()) {
+            desc.out_damping_offsets = od_offs.data();
+            desc.out_damping_times   = od_times.data();
+            desc.out_damping_levels  = od_levels.data();
+            desc.out_damping_types   = od_types.data();
+            desc.out_damping_coeffs  = od_coeffs.data();
+        }
+        if (!onpi_thr.empty()) {
+            out_npi_desc                  = std::make_unique<mgo_dynamic_npis>();
+            out_npi_desc->node_offsets    = onpi_offs.data();
+            out_npi_desc->node_params     = onpi_params.data();
+            out_npi_desc->thresholds      = onpi_thr.data();
+            out_npi_desc->damping_offsets = onpi_doffs.data();
+            out_npi_desc->damping_values  = onpi_vals.empty() ? nullptr : onpi_vals.data();
+            out_npi_desc->damping_levels  = onpi_levels.empty() ? nullptr : onpi_levels.data();
+            out_npi_desc->damping_types   = onpi_types.empty() ? nullptr : onpi_types.data();
+            out_npi_desc->damping_weights = onpi_weights.empty() ? nullptr : onpi_weights.data();
+            desc.out_dynamic_npis         = out_npi_desc.get();
+        }
         if (!npi_thr.empty()) {
             npis                  = std::make_unique<mgo_dynamic_npis>();
             npis->node_offsets    = npi_offs.data();
@@ -1563,7 +1847,7 @@ GraphArrays graph_arrays(const OsecirGraph<FP>& g)
         a.add_core(sim.get_integrator_core(), double(sim.get_dt()));
     }
     for (const auto& e : g.edges()) {
-        a.add_edge(e.start_node_idx, e.end_node_idx, e.property.get_parameters().get_coefficients());
+        a.add_edge(e.start_node_idx, e.end_node_idx, e.property.get_parameters());
     }
     a.finish();
     return a;

@@ -177,6 +177,9 @@ class _FixedValue(UncertainValue):
         box = [v]
         super().__init__(lambda: box[0], lambda x: box.__setitem__(0, float(x)))
 
+    def __deepcopy__(self, memo):
+        return _FixedValue(self.value)
+
 
 class DynamicNPIs:
     """mio::DynamicNPIs (epidemiology/dynamic_npis.h:36-242; binding dynamic_npis.h:35-87): thresholds (cases
@@ -205,11 +208,71 @@ class DynamicNPIs:
         return list(self._thresholds)
 
 
+class MobilityDamping:
+    """mio::VectorDamping of mobility coefficients (binding simulation.cpp:97-99): coefficients (10 * G values
+    or a scalar), time t, level, type."""
+
+    def __init__(self, coeffs, t=0.0, level=0, type=0):
+        self.coeffs = np.atleast_1d(np.asarray(coeffs, dtype=np.float64)).ravel().copy()
+        self.time = float(t)
+        self.level = int(level)
+        self.type = int(type)
+
+
+class MobilityCoefficients:
+    """mio::MobilityCoefficients (metapopulation_mobility_instant.h:106-110): baseline, minimum, dampings;
+    the coefficients at time t are baseline - D(t) (baseline - minimum)."""
+
+    def __init__(self, baseline, minimum=None):
+        self.baseline = np.asarray(baseline, dtype=np.float64).ravel().copy()
+        self.minimum = np.zeros_like(self.baseline) if minimum is None else \
+            np.asarray(minimum, dtype=np.float64).ravel().copy()
+        self.dampings = []
+
+    def add_damping(self, damping):
+        self.dampings.append(damping)
+
+    def clear_dampings(self):
+        self.dampings = []
+
+
+class MobilityCoefficientGroup:
+    """mio::MobilityCoefficientGroup: coefficient vectors whose values add up; add_damping damps every one."""
+
+    def __init__(self, num_matrices, num_elements):
+        self.matrices = [MobilityCoefficients(np.zeros(num_elements)) for _ in range(num_matrices)]
+
+    def __getitem__(self, k):
+        return self.matrices[k]
+
+    def __setitem__(self, k, m):
+        self.matrices[k] = m
+
+    def __len__(self):
+        return len(self.matrices)
+
+    @property
+    def num_matrices(self):
+        return len(self.matrices)
+
+    def add_damping(self, damping):
+        for m in self.matrices:
+            m.add_damping(damping)
+
+
 class MobilityParameters:
-    """mio::MobilityParameters: one coefficient per compartment and age group (10 * G)."""
+    """mio::MobilityParameters (binding mobility/metapopulation_mobility_instant.cpp:30-41): `coefficients` is
+    the MobilityCoefficientGroup (constructed from a vector of 10 * G values or a group);
+    `dynamic_npis_infected` the edge's DynamicNPIs (MobilityParameters::get_dynamic_npis_infected)."""
 
     def __init__(self, coeffs):
-        self.coefficients = np.asarray(coeffs, dtype=np.float64).ravel().copy()
+        if isinstance(coeffs, MobilityCoefficientGroup):
+            self.coefficients = coeffs
+        else:
+            v = np.asarray(coeffs, dtype=np.float64).ravel()
+            self.coefficients = MobilityCoefficientGroup(1, v.size)
+            self.coefficients[0] = MobilityCoefficients(v)
+        self.dynamic_npis_infected = DynamicNPIs()
 
 
 class TimeSeries:

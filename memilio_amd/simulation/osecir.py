@@ -340,7 +340,8 @@ class _Graph:
         """Edges stay sorted by (start, end); an existing edge is replaced (graph.h:231-241)."""
         if not (0 <= start < self.num_nodes and 0 <= end < self.num_nodes):
             raise IndexError("add_edge: node index out of range")
-        prop = coeffs if isinstance(coeffs, MobilityParameters) else MobilityParameters(coeffs)
+        # the graph keeps its own copy, as Graph::add_edge copies the edge property (graph.h:231-241)
+        prop = copy.deepcopy(coeffs) if isinstance(coeffs, MobilityParameters) else MobilityParameters(coeffs)
         for k, e in enumerate(self._edges):
             if (e.start_node_idx, e.end_node_idx) == (start, end):
                 self._edges[k] = _Edge(start, end, prop)
@@ -419,21 +420,73 @@ def _spec_from(models, edges):
         spec.contact_minimum = minima
     if any_damp:
         spec.set_dampings(per_node_damp)
-    npi_cfgs = []
-    for m in models:
-        dn = m.parameters.DynamicNPIsInfectedSymptoms
-        npi_cfgs.append({"thresholds": [(thr, [(float(d.value.value), d.level, d.type, d.group_weights) for d in ds])
-                                        for thr, ds in dn.threshold],
-                         "duration": dn.duration, "delay": dn.implementation_delay, "base_value": dn.base_value,
-                         "directive_begin": dn.directive_begin, "directive_end": dn.directive_end})
+    npi_cfgs = [_npi_cfg(m.parameters.DynamicNPIsInfectedSymptoms) for m in models]
     if any(c["thresholds"] for c in npi_cfgs):
         spec.set_dynamic_npis(npi_cfgs)
     if edges:
-        s = np.array([e[0] for e in edges], np.int32)
-        d = np.array([e[1] for e in edges], np.int32)
-        c = np.stack([np.broadcast_to(np.asarray(e[2], np.float64), (NC * G,)) for e in edges])
-        spec.set_edges(s, d, c)
+        _edges_into(spec, edges, [m.populations for m in models])
     return spec
+
+
+def _npi_cfg(dn):
+    return {"thresholds": [(thr, [(float(d.value.value), d.level, d.type, d.group_weights) for d in ds])
+                           for thr, ds in dn.threshold],
+            "duration": dn.duration, "delay": dn.implementation_delay, "base_value": dn.base_value,
+            "directive_begin": dn.directive_begin, "directive_end": dn.directive_end}
+
+
+def _edges_into(spec, edges, populations):
+    """Edges (start, end, MobilityParameters or coefficient vector) into the descriptor: baseline B and minimum
+    M of the folded coefficient group, and per source node the out-edge dampings (per age group) and edge
+    NPIs, which every out-edge of a node must share on the device."""
+    G, n = spec.G, spec.num_nodes
+    s = np.array([e[0] for e in edges], np.int32)
+    d = np.array([e[1] for e in edges], np.int32)
+    B, M, out = [], [], [None] * n
+    for (start, _, prop) in edges:
+        if not isinstance(prop, MobilityParameters):
+            B.append(np.broadcast_to(np.asarray(prop, np.float64), (NC * G,)))
+            M.append(np.zeros(NC * G))
+            key = ((), None)
+        else:
+            grp = prop.coefficients
+            damped = [k for k in range(grp.num_matrices) if grp[k].dampings]
+            if len(damped) > 1:
+                raise ValueError("more than one damped mobility coefficient vector in an edge is not supported")
+            b = sum(np.broadcast_to(grp[k].baseline, (NC * G,)) for k in range(grp.num_matrices))
+            if damped:
+                k0 = damped[0]
+                m = grp[k0].minimum + sum(grp[k].baseline for k in range(grp.num_matrices) if k != k0)
+            else:
+                m = sum(grp[k].minimum for k in range(grp.num_matrices))
+            B.append(b)
+            M.append(np.broadcast_to(m, (NC * G,)))
+            lst = []
+            for dm in (grp[damped[0]].dampings if damped else []):
+                v = np.broadcast_to(dm.coeffs, (NC * G,)).reshape(G, NC)
+                if not np.all(v == v[:, :1]):
+                    raise ValueError("mobility dampings must be uniform within an age group on the device")
+                lst.append((dm.time, dm.level, dm.type, tuple(v[:, 0])))
+            dn = prop.dynamic_npis_infected
+            npis = _npi_cfg(dn) if dn.threshold else None
+            key = (tuple(lst), repr(npis))
+        if out[start] is None:
+            out[start] = (key, prop)
+        elif out[start][0] != key:
+            raise ValueError(f"out-edges of node {start} differ in their dampings or dynamic NPIs; the device "
+                             "shares one list per node")
+    spec.set_edges(s, d, np.stack(B))
+    Ms = np.stack(M)
+    if Ms.any():
+        spec.edge_coeff_minimum = Ms
+    if any(o is not None and o[0][0] for o in out):
+        spec.set_out_dampings([[(t, lev, typ, np.asarray(v)) for (t, lev, typ, v) in o[0][0]] if o else []
+                               for o in out])
+    if any(o is not None and isinstance(o[1], MobilityParameters) and o[1].dynamic_npis_infected.threshold
+           for o in out):
+        spec.set_out_dynamic_npis([_npi_cfg(o[1].dynamic_npis_infected)
+                                   if o is not None and isinstance(o[1], MobilityParameters) else None for o in out])
+    spec._desc = None
 
 
 def _integrator_of(graph, integrator):
@@ -472,7 +525,7 @@ class MobilitySimulation:
             dts = {n.dt for n in nodes}
             if len(dts) != 1:
                 raise ValueError("nodes of one graph need the same initial step on the device")
-            edges = [(e.start_node_idx, e.end_node_idx, e.property.coefficients) for e in self.graph._edges]
+            edges = [(e.start_node_idx, e.end_node_idx, e.property) for e in self.graph._edges]
             self._spec = _spec_from([n.model for n in nodes], edges)
             cfg = capi.sim_cfg(t0=self.t0, dt_graph=self.dt, dt_node=dts.pop(),
                                integrator=_integrator_of(self.graph, self._integrator))
@@ -530,7 +583,7 @@ class GraphParameterStudy:
                 sg.add_node(nd.id, nd.property, self.t0, self.dt)
             for e in g._edges:
                 sg.add_edge(e.start_node_idx, e.end_node_idx, e.property)
-            edges = [(e.start_node_idx, e.end_node_idx, e.property.coefficients) for e in sg._edges]
+            edges = [(e.start_node_idx, e.end_node_idx, e.property) for e in sg._edges]
             specs.append(_spec_from([sg.get_node(v).property.model for v in range(sg.num_nodes)], edges))
             cfgs.append(capi.sim_cfg(t0=self.t0, dt_graph=self.dt, dt_node=self.dt,
                                      integrator=(integrator or ControlledStepperWrapper()).cfg()))

@@ -359,7 +359,34 @@ int main()
         std::printf("dynamic NPIs, fixed dt: max rel err %.3e\n", e);
         CHECK(e < 1e-12);
     }
-    // 6. errors surface as mio::Error with the C status
+    // 6. damped mobility coefficients (minimum, per-group dampings) and edge dynamic NPIs on every edge, as
+    //    osecir::draw_sample gives them (cpp/models/ode_secir/parameter_space.h:240-246)
+    {
+        auto pg = random_params_graph(14, 2, 31);
+        mio::DynamicNPIs<double> npis;
+        npis.set_duration(mio::SimulationTime(6.0));
+        npis.set_base_value(100000.0);
+        npis.set_threshold(40.0, {mio::DampingSampling<double>(0.6, mio::DampingLevel(0), mio::DampingType(0),
+                                                               mio::SimulationTime(0), {0},
+                                                               mio::Vector::Constant(2, 1.0))});
+        for (auto& e : pg.edges()) {
+            auto& grp   = e.property.get_coefficients();
+            auto& b     = grp[0].get_baseline();
+            auto& mn    = grp[0].get_minimum();
+            for (size_t i = 0; i < b.size(); ++i) {
+                mn[i] = 0.25 * b[i];
+            }
+            grp.add_damping(0.4, mio::DampingLevel(0), mio::DampingType(0), mio::SimulationTime(4.0));
+            grp.add_damping(0.7, mio::DampingLevel(1), mio::DampingType(0), mio::SimulationTime(11.0));
+            e.property.set_dynamic_npis_infected(npis);
+        }
+        auto sim = mio::make_mobility_sim<double>(0.0, 0.5, sample_graph(pg, 0.0, 0.1, 5, true));
+        sim.advance(25.0);
+        const double e = compare_graph(sim.get_graph(), 0.0, 0.5, 25.0);
+        std::printf("mobility dampings and edge NPIs, fixed dt: max rel err %.3e\n", e);
+        CHECK(e < 1e-12);
+    }
+    // 7. errors surface as mio::Error with the C status
     {
         Graph g;
         g.add_node(0, mio::osecir::Model<double>(2), 0.0);

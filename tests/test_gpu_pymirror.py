@@ -18,7 +18,7 @@ def rel_err(a, b):
 
 def _spec(graph):
     nodes = [graph.get_node(v).property for v in range(graph.num_nodes)]
-    edges = [(e.start_node_idx, e.end_node_idx, e.property.coefficients) for e in graph._edges]
+    edges = [(e.start_node_idx, e.end_node_idx, e.property) for e in graph._edges]
     return osecir._spec_from([n.model for n in nodes], edges)
 
 
@@ -89,4 +89,23 @@ def test_mobility_simulation_dynamic_npis(oracle):
     sim.advance(40)
     cfg = capi.sim_cfg(t0=0.0, dt_graph=0.5, dt_node=0.1)
     ref, _ = oracle.graph_run(spec, cfg, 40.0)
+    assert rel_err(_results(sim.graph), ref) < 1e-6
+
+
+def test_mobility_simulation_edge_dampings(oracle):
+    """The mobility example with damped mobility coefficients (minimum, per-group damping) and edge NPIs."""
+    graph = ps.mobility_example_graph()
+    for e in graph._edges:
+        grp = e.property.coefficients
+        grp[0].minimum = 0.2 * grp[0].baseline
+        grp.add_damping(mio.MobilityDamping(np.full(10, 0.5), t=3.0, level=0, type=0))
+        dn = e.property.dynamic_npis_infected
+        dn.duration = 5.0
+        dn.base_value = 100000
+        dn.set_threshold(100.0, [mio.DampingSampling(0.7, 1, 0, 0.0, [0], np.ones(1))])
+    spec = _spec(graph)
+    sim = osecir.MobilitySimulation(graph, 0.0, dt=0.5)
+    sim.advance(30)
+    cfg = capi.sim_cfg(t0=0.0, dt_graph=0.5, dt_node=0.1)
+    ref, _ = oracle.graph_run(spec, cfg, 30.0)
     assert rel_err(_results(sim.graph), ref) < 1e-6

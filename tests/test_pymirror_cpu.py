@@ -2,6 +2,7 @@
 edge order, apply_constraints as the reference, and the descriptor it hands to the engine (checked by
 running the oracle on it against the same set-up built directly)."""
 import numpy as np
+import pytest
 
 import memilio_amd.simulation as mio
 import memilio_amd.simulation.osecir as osecir
@@ -53,7 +54,7 @@ def test_graph_edges_sorted_and_replaced():
     assert g.num_nodes == 3 and g.num_edges == 2
     assert g.get_num_out_edges(0) == 1 and g.get_num_out_edges(1) == 0
     assert (g.get_edge(0).start_node_idx, g.get_edge(0).end_node_idx) == (0, 1)
-    assert g.get_edge(0).property.coefficients[0] == 2.0
+    assert g.get_edge(0).property.coefficients[0].baseline[0] == 2.0
 
 
 def test_descriptor_matches_direct_setup(oracle):
@@ -61,7 +62,7 @@ def test_descriptor_matches_direct_setup(oracle):
     against the C descriptor directly."""
     graph = ps.mobility_example_graph()
     nodes = [graph.get_node(v).property for v in range(graph.num_nodes)]
-    edges = [(e.start_node_idx, e.end_node_idx, e.property.coefficients) for e in graph._edges]
+    edges = [(e.start_node_idx, e.end_node_idx, e.property) for e in graph._edges]
     spec = osecir._spec_from([n.model for n in nodes], edges)
     assert spec.num_nodes == 2 and spec.num_edges == 2
     assert spec.populations[0, 0, 0] == 10000 - 240 and spec.populations[1, 0, 0] == 2000
@@ -91,7 +92,7 @@ def test_dynamic_npis_descriptor(oracle):
         assert [t for t, _ in dn.threshold] == [500.0, 100.0]
         assert dn.threshold[1][1][0].value.value == 0.4
     nodes = [graph.get_node(v).property for v in range(graph.num_nodes)]
-    edges = [(e.start_node_idx, e.end_node_idx, e.property.coefficients) for e in graph._edges]
+    edges = [(e.start_node_idx, e.end_node_idx, e.property) for e in graph._edges]
     spec = osecir._spec_from([n.model for n in nodes], edges)
     n = spec.npis
     assert n["node_offsets"].tolist() == [0, 2, 4]
@@ -104,3 +105,42 @@ def test_dynamic_npis_descriptor(oracle):
     spec._desc = None
     without, _ = oracle.graph_run(spec, cfg, 40.0)
     assert np.max(np.abs(with_npis - without)) > 1.0
+
+
+def test_mobility_dampings_descriptor(oracle):
+    """MobilityParameters with a damped coefficient group (minimum, per-group dampings) and edge NPIs become
+    per-node out-edge damping arrays; out-edges of one node that differ are refused."""
+    graph = ps.mobility_example_graph()
+    for e in graph._edges:
+        grp = e.property.coefficients
+        grp[0].minimum = 0.2 * grp[0].baseline
+        grp.add_damping(mio.MobilityDamping(np.full(10, 0.5), t=3.0, level=0, type=0))
+        dn = e.property.dynamic_npis_infected
+        dn.duration = 5.0
+        dn.base_value = 100000
+        dn.set_threshold(100.0, [mio.DampingSampling(0.7, 1, 0, 0.0, [0], np.ones(1))])
+    nodes = [graph.get_node(v).property for v in range(graph.num_nodes)]
+    edges = [(e.start_node_idx, e.end_node_idx, e.property) for e in graph._edges]
+    spec = osecir._spec_from([n.model for n in nodes], edges)
+    assert spec.out_dampings["offsets"].tolist() == [0, 1, 2]
+    assert spec.out_dampings["coeffs"][:, 0].tolist() == [0.5, 0.5]
+    assert spec.out_npis["thresholds"].tolist() == [100.0, 100.0]
+    np.testing.assert_allclose(spec.edge_coeff_minimum, 0.2 * spec.edge_coeffs)
+    cfg = capi.sim_cfg(t0=0.0, dt_graph=0.5, dt_node=0.1)
+    damped, _ = oracle.graph_run(spec, cfg, 20.0)
+    spec.out_dampings = spec.out_npis = spec.edge_coeff_minimum = None
+    spec._desc = None
+    plain, _ = oracle.graph_run(spec, cfg, 20.0)
+    assert np.max(np.abs(damped - plain)) > 1e-3
+    # a node whose out-edges differ cannot share one list
+    graph.get_edge(0).property.coefficients.add_damping(mio.MobilityDamping(np.full(10, 0.1), t=7.0))
+    graph.add_edge(0, 1, graph.get_edge(0).property)
+    extra = osecir.MobilityGraph()
+    for v in range(graph.num_nodes):
+        extra.add_node(v, graph.get_node(v).property.model, 0.0, 0.1)
+    extra.add_node(2, graph.get_node(0).property.model, 0.0, 0.1)
+    extra.add_edge(0, 1, graph.get_edge(0).property)
+    extra.add_edge(0, 2, mio.MobilityParameters(np.full(10, 0.01)))
+    with pytest.raises(ValueError):
+        osecir._spec_from([extra.get_node(v).property.model for v in range(3)],
+                          [(e.start_node_idx, e.end_node_idx, e.property) for e in extra._edges])
