@@ -284,3 +284,105 @@ def test_mobility_dampings_scenario_triggers(oracle):
     a, _ = oracle.graph_run(g, synthetic.fixed_cfg(0.1), 40.0)
     b, _ = oracle.graph_run(g0, synthetic.fixed_cfg(0.1), 40.0)
     assert np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-6)) > 1e-3
+
+
+# ------------------------------------------------------------------ mobility factors and commuter testing
+def _departure(oracle, spec, t, c=0.1):
+    """(source, destination) after one departure of a G-group node pair with all coefficients c."""
+    out = oracle.edge_scenario(spec, t, np.full(spec.state_size(), c))
+    return out[0][0], out[0][1]
+
+
+def _factor_model(G=1, ttc=45.0, beta=(0.25,), max_beta=(0.5,), ins=(100.0,)):
+    """test_odesecir.cpp:1052-1063: T_E 3, T_INS 2, RecoveredPerInfectedNoSymptoms 0.1, INS 100 (split over
+    the groups); 40 symptomatic and 1000 susceptible per group added so that the ISy factor shows in the
+    travelers (the factors depend on INS only)."""
+    from memilio_amd.graphspec import GraphSpec, S, INS, ISY
+    spec = GraphSpec(1, G)
+    a = spec.age_params[0]
+    a[capi.A_TIME_EXPOSED] = 3.0
+    a[capi.A_TIME_INFECTED_NO_SYMPTOMS] = 2.0
+    a[capi.A_RECOVERED_PER_INFECTED_NO_SYMPTOMS] = 0.1
+    a[capi.A_RISK_OF_INFECTION_FROM_SYMPTOMATIC] = beta
+    a[capi.A_MAX_RISK_OF_INFECTION_FROM_SYMPTOMATIC] = max_beta
+    spec.node_scalars[0, capi.P_TEST_AND_TRACE_CAPACITY] = ttc
+    spec.populations[0, :, INS] = ins
+    spec.populations[0, :, ISY] = 40.0
+    spec.populations[0, :, S] = 1000.0
+    return spec
+
+
+def _isy_factors(oracle, spec):
+    from memilio_amd.graphspec import ISY, NC
+    src, _ = _departure(oracle, spec, 0.0)
+    isy = src.reshape(spec.G, NC)[:, ISY]
+    return (40.0 - isy) / (0.1 * 40.0)
+
+
+def test_mobility_factors_known_answers(oracle):
+    """get_mobility_factors (test_odesecir.cpp:1052-1088) seen through a departure: test-and-trace load
+    (1 - 0.1) / 2 * 100 = 45; capacity 45 gives the symptomatic factor beta, capacity 9 (x 5 = 45) gives
+    max beta, capacity 20 a value strictly between; every other compartment travels with factor 1."""
+    f = _isy_factors(oracle, _factor_model(ttc=45.0))
+    assert f[0] == pytest.approx(0.25, rel=1e-12)
+    f = _isy_factors(oracle, _factor_model(ttc=9.0))
+    assert f[0] == pytest.approx(0.5, rel=1e-12)
+    f_mid = _isy_factors(oracle, _factor_model(ttc=20.0))
+    assert 0.25 < f_mid[0] < 0.5
+    from memilio_amd.graphspec import INS, ISY, S, NC
+    spec = _factor_model(ttc=20.0)
+    src, dst = _departure(oracle, spec, 0.0)
+    for k in (S, INS):
+        assert src[k] == pytest.approx(spec.populations[0, 0, k] * 0.9, rel=1e-15)
+        assert dst[k] == pytest.approx(spec.populations[0, 0, k] * 1.1, rel=1e-15)
+    assert src.sum() + dst.sum() == pytest.approx(2 * spec.populations[0].sum(), rel=1e-15)
+    # more groups: the load is the total over the groups, the factor each group's own
+    f2 = _isy_factors(oracle, _factor_model(G=2, ttc=20.0, beta=(0.25, 0.25), max_beta=(0.5, 0.5),
+                                            ins=(60.0, 40.0)))
+    assert f2 == pytest.approx([f_mid[0]] * 2, rel=1e-12)
+    f2 = _isy_factors(oracle, _factor_model(G=2, ttc=45.0, beta=(0.25, 0.1), max_beta=(0.5, 0.7),
+                                            ins=(60.0, 40.0)))
+    assert f2 == pytest.approx([0.25, 0.1], rel=1e-12)
+    f2 = _isy_factors(oracle, _factor_model(G=2, ttc=9.0, beta=(0.25, 0.1), max_beta=(0.5, 0.7),
+                                            ins=(60.0, 40.0)))
+    assert f2 == pytest.approx([0.5, 0.7], rel=1e-12)
+
+
+@pytest.mark.parametrize("t,nondet", [(3.125, 0.4), (25.0, 1.0)], ids=["inside", "outside"])
+def test_commuter_testing_known_answers(oracle, t, nondet):
+    """test_commuters (test_odesecir.cpp:1086-1117) on departure: inside the detection window [0, 20) a
+    fraction 1 - 0.4 of the departing INS and ISy travelers is detected and stays home as INSC / ISyC; the
+    travelers keep 0.4 of them. Outside the window nothing is detected."""
+    from memilio_amd.graphspec import GraphSpec, S, INS, INSC, ISY, ISYC, NC
+    spec = GraphSpec(1, 2)
+    spec.node_scalars[0, capi.P_COMMUTER_NONDETECTION] = 0.4
+    spec.node_scalars[0, capi.P_START_COMMUTER_DETECTION] = 0.0
+    spec.node_scalars[0, capi.P_END_COMMUTER_DETECTION] = 20.0
+    spec.populations[0] = np.array([[800.0, 30.0, 50.0, 20.0, 40.0, 10.0, 5.0, 2.0, 100.0, 1.0],
+                                    [600.0, 20.0, 70.0, 15.0, 60.0, 25.0, 8.0, 3.0, 80.0, 2.0]])
+    c = 0.1
+    y = spec.populations[0]
+    src, dst = _departure(oracle, spec, t, c)
+    src, dst = src.reshape(2, NC), dst.reshape(2, NC)
+    for g in range(2):
+        mob = c * y[g]       # default parameters: every mobility factor is 1
+        assert dst[g, INS] == pytest.approx(y[g, INS] + mob[INS] * nondet, rel=1e-14)
+        assert dst[g, ISY] == pytest.approx(y[g, ISY] + mob[ISY] * nondet, rel=1e-14)
+        assert src[g, INSC] == pytest.approx(y[g, INSC] + mob[INS] * (1 - nondet) - mob[INSC], rel=1e-14, abs=1e-12)
+        assert src[g, ISYC] == pytest.approx(y[g, ISYC] + mob[ISY] * (1 - nondet) - mob[ISYC], rel=1e-14, abs=1e-12)
+        assert src[g, INS] == pytest.approx(y[g, INS] - mob[INS], rel=1e-14)
+        assert src[g, S] == pytest.approx(y[g, S] - mob[S], rel=1e-15)
+    assert src.sum() + dst.sum() == pytest.approx(2 * y.sum(), rel=1e-15)
+
+
+def test_return_fixup_properties(oracle):
+    """Return fix-up (metapopulation_mobility_instant.h:576-596), no known answers in the reference's tests:
+    on a graph where it triggers, every node keeps its total at every output day (the supernumerous returns
+    are moved inside the traveler vector, not created) and no compartment is negative afterwards."""
+    spec = setups.fixup_graph()
+    for cfg in (synthetic.fixed_cfg(0.1), synthetic.ck_cfg(1e-10)):
+        got, st = oracle.graph_run(spec, cfg, 20.0)
+        assert st.negative_fixups > 0
+        per_node = got.sum(axis=2)
+        assert np.max(np.abs(per_node - per_node[0]) / per_node[0]) < 1e-12
+        assert got.min() >= -1e-9

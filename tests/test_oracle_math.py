@@ -1,0 +1,293 @@
+"""Pins of the oracle that do not need the reference: closed forms, invariants, special cases, brute force.
+
+The golden fixtures (test_oracle_golden.py) pin the oracle on the reference's own outputs; the checks here
+pin what those leave open, each chosen so that a plausible restatement mistake fails it:
+  - closed form: with TransmissionProbabilityOnContact = 0 the osecir system (cpp/models/ode_secir/model.h:
+    82-214) is linear, y' = A y per age group, and scipy's matrix exponential gives the exact solution (a
+    dropped or swapped rate of one of the 14 linear flows fails it);
+  - invariants: the sum of all compartments is conserved per node without edges and over the whole graph
+    with edges (a traveler not returned, or returned twice, fails it); nothing goes negative;
+  - special cases: zero mobility coefficients equal independent nodes (cpp/tests/test_mobility.cpp:30-127);
+    identical age groups with a uniform contact matrix split a single-group run evenly (a transposed contact
+    index or a per-group instead of a total test-and-trace / ICU load fails it);
+  - brute force: the oracle's adaptive solution against scipy's DOP853 at rtol 1e-13 on a numpy restatement
+    of model.h's right-hand side written for this test, with seasonality, the test-and-trace and the ICU
+    smoothers engaged.
+"""
+import numpy as np
+import pytest
+from scipy.integrate import solve_ivp
+from scipy.linalg import expm
+
+from memilio_amd import capi, synthetic
+from memilio_amd.graphspec import GraphSpec, S, E, INS, INSC, ISY, ISYC, ISEV, ICR, R, D, NC
+
+FLOOR = 1e-6
+
+
+def rel_err(a, b, floor=FLOOR):
+    a, b = np.asarray(a), np.asarray(b)
+    assert a.shape == b.shape, (a.shape, b.shape)
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), floor)))
+
+
+def _tight_ck(tol=1e-12):
+    return synthetic.ck_cfg(tol)
+
+
+# ------------------------------------------------------------------------------------------ closed form
+def linear_generator(spec, node, g):
+    """10x10 generator of the flows of age group g of `node` when no one gets infected (model.h:154-213),
+    ICU capacity not reached: column = source compartment, row = target."""
+    a = spec.age_params[node, :, g]
+    te, tins, tisy, tsev, tcr = (a[capi.A_TIME_EXPOSED], a[capi.A_TIME_INFECTED_NO_SYMPTOMS],
+                                 a[capi.A_TIME_INFECTED_SYMPTOMS], a[capi.A_TIME_INFECTED_SEVERE],
+                                 a[capi.A_TIME_INFECTED_CRITICAL])
+    rins = a[capi.A_RECOVERED_PER_INFECTED_NO_SYMPTOMS]
+    sev = a[capi.A_SEVERE_PER_INFECTED_SYMPTOMS]
+    crit = a[capi.A_CRITICAL_PER_SEVERE]
+    dsev = a[capi.A_DEATHS_PER_SEVERE]
+    dcr = a[capi.A_DEATHS_PER_CRITICAL]
+    flows = [(E, INS, 1.0 / te),
+             (INS, ISY, (1.0 - rins) / tins), (INS, R, rins / tins),
+             (INSC, ISYC, (1.0 - rins) / tins), (INSC, R, rins / tins),
+             (ISY, ISEV, sev / tisy), (ISY, R, (1.0 - sev) / tisy),
+             (ISYC, ISEV, sev / tisy), (ISYC, R, (1.0 - sev) / tisy),
+             (ISEV, ICR, crit / tsev), (ISEV, R, (1.0 - crit - dsev) / tsev), (ISEV, D, dsev / tsev),
+             (ICR, D, dcr / tcr), (ICR, R, (1.0 - dcr) / tcr)]
+    A = np.zeros((NC, NC))
+    for src, dst, rate in flows:
+        A[src, src] -= rate
+        A[dst, src] += rate
+    return A
+
+
+def no_transmission_graph(num_nodes=6, G=3, seed=5):
+    """Nodes of the synthetic workload with transmission probability 0, deaths per severe > 0 (so every one
+    of the 14 linear flows is non-zero) and every compartment populated."""
+    spec = GraphSpec(num_nodes, G)
+    synthetic._node_params(spec, np.random.default_rng(seed))
+    rng = np.random.default_rng(seed + 1)
+    spec.age_params[:, capi.A_TRANSMISSION_PROBABILITY_ON_CONTACT, :] = 0.0
+    spec.age_params[:, capi.A_DEATHS_PER_SEVERE, :] = rng.uniform(0.02, 0.1, (num_nodes, G))
+    spec.populations[:] = rng.uniform(1e3, 1e5, (num_nodes, G, NC))
+    return spec
+
+
+def closed_form_daily(spec, days):
+    """exp(A t) y0 per node and group at the given days: [day][node][10G]."""
+    n, G = spec.num_nodes, spec.G
+    out = np.zeros((len(days), n, G * NC))
+    for v in range(n):
+        for g in range(G):
+            A = linear_generator(spec, v, g)
+            y0 = spec.populations[v, g]
+            for k, t in enumerate(days):
+                out[k, v, g * NC:(g + 1) * NC] = expm(A * t) @ y0
+    return out
+
+
+def test_linear_generator_matches_brute_force_rhs():
+    """The test's two restatements agree: the linear generator equals the full numpy right-hand side at
+    transmission probability 0 (guards the closed-form test against a mistake shared with neither)."""
+    spec = no_transmission_graph(num_nodes=2, G=2)
+    for v in range(2):
+        y = spec.populations[v].ravel()
+        f = secir_rhs_numpy(spec, v, 3.7, y)
+        lin = np.concatenate([linear_generator(spec, v, g) @ spec.populations[v, g] for g in range(2)])
+        assert np.allclose(f, lin, rtol=1e-13, atol=1e-9)
+
+
+def test_closed_form_adaptive(oracle):
+    """Oracle graph run (no edges) with the adaptive Cash-Karp core at 1e-12 against exp(A t) y0, every
+    compartment, every day."""
+    spec = no_transmission_graph()
+    tmax = 30.0
+    got, _ = oracle.graph_run(spec, _tight_ck(), tmax)
+    ref = closed_form_daily(spec, np.arange(got.shape[0], dtype=np.float64))
+    assert rel_err(got, ref) < 1e-9
+
+
+@pytest.mark.parametrize("kind,dt,tol", [(capi.INTEGRATOR_CASH_KARP54_FIXED, 0.1, 1e-8),
+                                         (capi.INTEGRATOR_RKF45, None, 1e-8)])
+def test_closed_form_single_model(oracle, kind, dt, tol):
+    """osecir::simulate's time series against exp(A t) y0 at every time point (fixed-step Cash-Karp and the
+    RKF45 RKIntegratorCore)."""
+    spec = no_transmission_graph(num_nodes=1, G=2, seed=9)
+    if dt is not None:
+        cfg = synthetic.fixed_cfg(dt)
+    else:
+        cfg = capi.sim_cfg(dt_node=0.1, integrator=capi.integrator_cfg(kind, 1e-12, 1e-12, 1e-8, 1.0))
+    times, vals = oracle.simulate(spec, cfg, 25.0)
+    assert times[-1] == pytest.approx(25.0, abs=1e-10)
+    ref = np.stack([np.concatenate([expm(linear_generator(spec, 0, g) * t) @ spec.populations[0, g]
+                                    for g in range(2)]) for t in times])
+    assert rel_err(vals, ref) < tol
+
+
+# ------------------------------------------------------------------------------------------ invariants
+def test_conservation_without_edges(oracle):
+    """Without edges, each node keeps its total population (Dead included) and nothing goes negative."""
+    spec = synthetic.graph(num_nodes=12, G=3, seed=21, mean_degree=0.0)
+    assert spec.num_edges == 0
+    got, _ = oracle.graph_run(spec, synthetic.ck_cfg(1e-8), 60.0)
+    totals = got.sum(axis=2)                       # [day][node]
+    assert rel_err(totals, np.broadcast_to(totals[0], totals.shape), floor=1.0) < 1e-12
+    assert got.min() > -1e-8
+
+
+def test_conservation_with_edges(oracle):
+    """With mobility, every node keeps its total at every output day: the travelers are home at integer days
+    and the Euler step of the traveler vector conserves its sum (test_mobility.cpp:196-214), so the whole
+    graph does too; the course differs from the edge-free run; nothing goes negative."""
+    spec = synthetic.graph(num_nodes=20, G=3, seed=22, mean_degree=6.0)
+    alone = synthetic.graph(num_nodes=20, G=3, seed=22, mean_degree=0.0)
+    assert spec.num_edges > 0
+    for cfg in (synthetic.ck_cfg(1e-8), synthetic.fixed_cfg(0.1)):
+        got, st = oracle.graph_run(spec, cfg, 40.0)
+        assert st.returns > 0
+        per_node = got.sum(axis=2)
+        assert rel_err(per_node, np.broadcast_to(per_node[0], per_node.shape), floor=1.0) < 1e-12
+        total = got.sum(axis=(1, 2))
+        assert rel_err(total, np.full_like(total, total[0]), floor=1.0) < 1e-12
+        free, _ = oracle.graph_run(alone, cfg, 40.0)
+        assert rel_err(got, free) > 1e-3
+        assert got.min() > -1e-8
+
+
+# ------------------------------------------------------------------------------------------ special cases
+def test_zero_coefficients_equal_independent_nodes(oracle):
+    """Edges with zero coefficients change nothing (bit for bit against the edge-free graph), and the graph's
+    nodes end where single simulations of them end (test_mobility.cpp:30-127 compares the last values: the
+    graph stops at every half day, so the step sequences, and the daily interpolants, differ)."""
+    base = synthetic.graph(num_nodes=4, G=2, seed=23, mean_degree=2.0)
+    assert base.num_edges > 0
+    zero = synthetic.graph(num_nodes=4, G=2, seed=23, mean_degree=2.0)
+    zero.set_edges(base.edge_start, base.edge_end, np.zeros(base.num_edges))
+    alone = synthetic.graph(num_nodes=4, G=2, seed=23, mean_degree=0.0)
+    cfg = synthetic.ck_cfg(1e-10)
+    a, _ = oracle.graph_run(zero, cfg, 10.0)
+    b, _ = oracle.graph_run(alone, cfg, 10.0)
+    assert np.array_equal(a, b)
+    for v in range(4):
+        single = GraphSpec(1, 2)
+        single.node_scalars[0] = alone.node_scalars[v]
+        single.age_params[0] = alone.age_params[v]
+        single.contact_baseline[0] = alone.contact_baseline[v]
+        single.populations[0] = alone.populations[v]
+        ts, vs = oracle.simulate(single, cfg, 10.0)
+        tg, vg = oracle.graph_node_series(zero, cfg, 10.0, v)
+        assert ts[-1] == tg[-1] == 10.0
+        assert rel_err(vg[-1], vs[-1]) < 1e-9
+
+
+@pytest.mark.parametrize("icu,ttc", [(capi.DBL_MAX, capi.DBL_MAX), (2e3, 5e3)], ids=["defaults", "capacities"])
+def test_uniform_contacts_split_evenly(oracle, icu, ttc):
+    """G identical groups with contact matrix c/G everywhere and a third of the population each run exactly
+    as a third of the single-group model with contact c (the test-and-trace and ICU loads are totals over
+    the groups, so they agree as well). Fixed step: the two runs take the same steps."""
+    G = 3
+    one = synthetic.single_model(seed=24, G=1)
+    one.node_scalars[0, capi.P_ICU_CAPACITY] = icu
+    one.node_scalars[0, capi.P_TEST_AND_TRACE_CAPACITY] = ttc
+    one.populations[0, 0] = [3e5, 3e3, 1.5e3, 0.0, 9e2, 0.0, 300.0, 90.0, 0.0, 0.0]
+    split = GraphSpec(1, G)
+    split.node_scalars[0] = one.node_scalars[0]
+    split.age_params[0] = np.repeat(one.age_params[0], G, axis=1)
+    split.contact_baseline[0] = one.contact_baseline[0, 0, 0] / G
+    split.populations[0] = one.populations[0, 0] / G
+    cfg = synthetic.fixed_cfg(0.1)
+    ref, _ = oracle.simulate_daily(one, cfg, 50.0)
+    got, _ = oracle.simulate_daily(split, cfg, 50.0)
+    got = got.reshape(got.shape[0], G, NC)
+    for g in range(G):
+        assert rel_err(G * got[:, g, :], ref, floor=1e-3) < 1e-12
+    assert rel_err(ref[-1], one.populations[0, 0], floor=1.0) > 1e-2   # the epidemic moved
+
+
+# ------------------------------------------------------------------------------------------ brute force
+def _smoother_cosine(x, xl, xr, yl, yr):
+    """cpp/memilio/math/smoother.h:46-60."""
+    if x <= xl:
+        return yl
+    if x >= xr:
+        return yr
+    return 0.5 * (yl - yr) * np.cos(np.pi / (xr - xl) * (x - xl)) + 0.5 * (yl + yr)
+
+
+def secir_rhs_numpy(spec, v, t, y):
+    """dy/dt of node v (no dampings, pop = y) following cpp/models/ode_secir/model.h:82-214."""
+    G = spec.G
+    a = spec.age_params[v]
+    s = spec.node_scalars[v]
+    Y = y.reshape(G, NC)
+    te, tins, tisy, tsev, tcr = (a[capi.A_TIME_EXPOSED], a[capi.A_TIME_INFECTED_NO_SYMPTOMS],
+                                 a[capi.A_TIME_INFECTED_SYMPTOMS], a[capi.A_TIME_INFECTED_SEVERE],
+                                 a[capi.A_TIME_INFECTED_CRITICAL])
+    rins = a[capi.A_RECOVERED_PER_INFECTED_NO_SYMPTOMS]
+    ttc, icu = float(s[capi.P_TEST_AND_TRACE_CAPACITY]), float(s[capi.P_ICU_CAPACITY])
+    tt_required = float(np.sum((1.0 - rins) / tins * Y[:, INS]))
+    icu_occ = float(np.sum(Y[:, ICR]))
+    season = 1.0 + s[capi.P_SEASONALITY] * np.sin(np.pi * ((s[capi.P_START_DAY] + t) / 182.5 + 0.5))
+    N = Y[:, [S, E, INS, ISY, ISEV, ICR, R]].sum(axis=1)
+    risk = np.array([_smoother_cosine(tt_required, ttc, ttc * float(s[capi.P_TEST_AND_TRACE_MAX_RISK]),
+                                      a[capi.A_RISK_OF_INFECTION_FROM_SYMPTOMATIC, j],
+                                      a[capi.A_MAX_RISK_OF_INFECTION_FROM_SYMPTOMATIC, j]) for j in range(G)])
+    pressure = (a[capi.A_RELATIVE_TRANSMISSION_NO_SYMPTOMS] * Y[:, INS] + risk * Y[:, ISY]) / N
+    C = season * spec.contact_baseline[v]
+    dY = np.zeros_like(Y)
+    for i in range(G):
+        f_se = Y[i, S] * a[capi.A_TRANSMISSION_PROBABILITY_ON_CONTACT, i] * float(C[i] @ pressure)
+        crit = _smoother_cosine(icu_occ, 0.9 * icu, icu, a[capi.A_CRITICAL_PER_SEVERE, i], 0.0)
+        crit_full = a[capi.A_CRITICAL_PER_SEVERE, i]
+        dsev = a[capi.A_DEATHS_PER_SEVERE, i]
+        sev = a[capi.A_SEVERE_PER_INFECTED_SYMPTOMS, i]
+        dcr = a[capi.A_DEATHS_PER_CRITICAL, i]
+        f = {(S, E): f_se, (E, INS): Y[i, E] / te[i],
+             (INS, ISY): (1 - rins[i]) / tins[i] * Y[i, INS], (INS, R): rins[i] / tins[i] * Y[i, INS],
+             (INSC, ISYC): (1 - rins[i]) / tins[i] * Y[i, INSC], (INSC, R): rins[i] / tins[i] * Y[i, INSC],
+             (ISY, ISEV): sev / tisy[i] * Y[i, ISY], (ISY, R): (1 - sev) / tisy[i] * Y[i, ISY],
+             (ISYC, ISEV): sev / tisy[i] * Y[i, ISYC], (ISYC, R): (1 - sev) / tisy[i] * Y[i, ISYC],
+             (ISEV, ICR): crit / tsev[i] * Y[i, ISEV],
+             (ISEV, R): (1 - crit_full - dsev) / tsev[i] * Y[i, ISEV],
+             (ISEV, D): (dsev + (crit_full - crit)) / tsev[i] * Y[i, ISEV],
+             (ICR, D): dcr / tcr[i] * Y[i, ICR], (ICR, R): (1 - dcr) / tcr[i] * Y[i, ICR]}
+        for (src, dst), val in f.items():
+            dY[i, src] -= val
+            dY[i, dst] += val
+    return dY.ravel()
+
+
+def test_adaptive_against_dop853(oracle):
+    """The oracle's adaptive Cash-Karp at 1e-12 against DOP853 at rtol 1e-13 on the numpy right-hand side,
+    at every point of the oracle's time series: seasonality, the test-and-trace smoother (capacity below
+    the load) and the ICU smoother (occupancy crossing 90% of capacity) all engaged."""
+    spec = synthetic.single_model(seed=25, G=3)
+    spec.node_scalars[0, capi.P_SEASONALITY] = 0.3
+    spec.node_scalars[0, capi.P_START_DAY] = 40.0
+    spec.age_params[0, capi.A_DEATHS_PER_SEVERE, :] = 0.05
+    spec.populations[0, :, E] *= 20.0
+    spec.populations[0, :, S] -= spec.populations[0, :, E] * (1.0 - 1.0 / 20.0)
+    y0 = spec.populations[0].ravel().copy()
+    tmax = 60.0
+    # capacities that the course crosses: read the loads off a first run with infinite capacities
+    sol = solve_ivp(lambda t, y: secir_rhs_numpy(spec, 0, t, y), (0.0, tmax), y0, method="DOP853", rtol=1e-10,
+                    atol=1e-6, dense_output=True)
+    Yt = sol.y.T.reshape(-1, spec.G, NC)
+    rins = spec.age_params[0, capi.A_RECOVERED_PER_INFECTED_NO_SYMPTOMS]
+    tins = spec.age_params[0, capi.A_TIME_INFECTED_NO_SYMPTOMS]
+    tt_load = ((1.0 - rins) / tins * Yt[:, :, INS]).sum(axis=1)
+    icu_load = Yt[:, :, ICR].sum(axis=1)
+    spec.node_scalars[0, capi.P_TEST_AND_TRACE_CAPACITY] = 0.3 * tt_load.max()
+    spec.node_scalars[0, capi.P_ICU_CAPACITY] = 0.7 * icu_load.max()
+    spec._desc = None
+
+    times, vals = oracle.simulate(spec, _tight_ck(1e-12), tmax)
+    ref = solve_ivp(lambda t, y: secir_rhs_numpy(spec, 0, t, y), (0.0, tmax), y0, method="DOP853", rtol=1e-13,
+                    atol=1e-9, t_eval=times)
+    assert ref.success
+    assert rel_err(vals, ref.y.T, floor=1e-3) < 1e-8
+    # the smoothers were engaged
+    Y = vals.reshape(-1, spec.G, NC)
+    assert (Y[:, :, ICR].sum(axis=1) > 0.9 * spec.node_scalars[0, capi.P_ICU_CAPACITY]).any()
+    assert (((1.0 - rins) / tins * Y[:, :, INS]).sum(axis=1) > spec.node_scalars[0, capi.P_TEST_AND_TRACE_CAPACITY]).any()
