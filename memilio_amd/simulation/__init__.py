@@ -153,10 +153,16 @@ class ContactMatrixGroup:
 
 
 class UncertainContactMatrix:
-    """mio::UncertainContactMatrix: `cont_freq_mat` is the ContactMatrixGroup."""
+    """mio::UncertainContactMatrix (epidemiology/uncertain_matrix.h:40-60): `cont_freq_mat` is the
+    ContactMatrixGroup; `dampings`, `school_holiday_damping` and `school_holidays` are the parameter-study
+    sampling inputs the reference keeps beside it (kept and written to files, not sampled here)."""
 
     def __init__(self, group):
         self.cont_freq_mat = group
+        G = group[0].num_groups if len(group) else 1
+        self.dampings = []
+        self.school_holiday_damping = DampingSampling(0.0, 0, 0, 0.0, [], np.zeros(G))
+        self.school_holidays = []
 
 
 class DampingSampling:

@@ -5,7 +5,7 @@ bindings/mobility/graph_simulation.h:35-52, bindings/compartments/parameter_stud
 InfectionState, Model (parameters by name, populations indexed by (AgeGroup, InfectionState),
 apply_constraints), ModelGraph, MobilityGraph, MobilitySimulation(graph, t0, dt).advance(tmax),
 GraphParameterStudy(graph, t0, tmax, dt, num_runs).run([handle]), simulate(t0, tmax, dt, model),
-interpolate_simulation_result.
+interpolate_simulation_result, write_graph / read_graph (parameter-graph JSON files).
 
 Differences a caller sees: results are daily (the values interpolate_simulation_result would give for the
 reference's step-wise series); GraphParameterStudy.run takes an optional `sample(graph, run_idx)` callback
@@ -26,7 +26,7 @@ from . import (AgeGroup, ContactMatrix, ContactMatrixGroup, DynamicNPIs, Mobilit
 
 __all__ = ["InfectionState", "Model", "ModelGraph", "MobilityGraph", "MobilitySimulation", "GraphParameterStudy",
            "Simulation", "simulate", "interpolate_simulation_result", "ControlledStepperWrapper",
-           "RKIntegratorCore", "ExplicitStepperWrapper", "EulerIntegratorCore"]
+           "RKIntegratorCore", "ExplicitStepperWrapper", "EulerIntegratorCore", "write_graph", "read_graph"]
 
 
 class InfectionState(enum.IntEnum):
@@ -141,6 +141,7 @@ class Parameters:
         object.__setattr__(self, "ContactPatterns",
                            UncertainContactMatrix(ContactMatrixGroup(1, self._G)))
         object.__setattr__(self, "DynamicNPIsInfectedSymptoms", DynamicNPIs())
+        object.__setattr__(self, "_dist", {})   # distributions read from files, written back unchanged
 
     def __getattr__(self, name):
         if name in _SCALARS:
@@ -191,6 +192,8 @@ class Parameters:
         object.__setattr__(p, "_sc", self._sc.copy())
         object.__setattr__(p, "_age", self._age.copy())
         object.__setattr__(p, "ContactPatterns", copy.deepcopy(self.ContactPatterns, memo))
+        object.__setattr__(p, "DynamicNPIsInfectedSymptoms", copy.deepcopy(self.DynamicNPIsInfectedSymptoms, memo))
+        object.__setattr__(p, "_dist", copy.deepcopy(self._dist, memo))
         return p
 
     def apply_constraints(self):
@@ -236,6 +239,7 @@ class Populations:
     def __init__(self, num_groups):
         self._G = int(num_groups)
         self._v = np.zeros((self._G, NC))
+        self._dist = {}   # distributions read from files, by flat index
 
     @staticmethod
     def _idx(key):
@@ -597,3 +601,20 @@ class GraphParameterStudy:
         for run, sg in enumerate(graphs):
             handle_result_func(sg, run)
         return None
+
+
+# ----------------------------------------------------------------------------------------- graph files
+def write_graph(graph, directory):
+    """osecir.write_graph (binding io/mobility_io.h:34-45 over mio::write_graph, mobility_io.h:70-122): the
+    ModelGraph as JSON files GraphNode<i>.json / GraphEdges_node<i>.json in `directory`."""
+    from ._json import IOF_None
+    from ._json import write_graph as _write
+    _write(graph, directory, IOF_None)
+
+
+def read_graph(directory):
+    """osecir.read_graph (binding io/mobility_io.h:47-58 over mio::read_graph, mobility_io.h:124-192): the
+    ModelGraph written by write_graph (or by the reference). A missing directory or file raises ValueError,
+    a malformed file RuntimeError."""
+    from ._json import read_graph as _read
+    return _read(directory, 0, True)

@@ -1,0 +1,325 @@
+"""Parameter-graph JSON files (osecir.write_graph / read_graph), CPU only.
+
+Pinned on the reference's serializer known answers (cpp/tests/test_json_serializer.cpp: uncertain value,
+custom index array, matrix, tuple, parameter set), on its graph IO tests (pycode/memilio-simulation/tests/
+test_mobility.py:79-106, cpp/tests/test_save_parameters.cpp:205-290), on a node file written by hand from
+the reference's serializers (not by this writer), and on round trips whose device descriptor, and the
+oracle's run of it, are unchanged bit for bit.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import memilio_amd.simulation as mio
+import memilio_amd.simulation.osecir as osecir
+from memilio_amd.simulation import _json as mj
+from tests import pymirror_setups as ps
+
+
+# ------------------------------------------------------------------------------ serializer known answers
+def test_uncertain_value_known_answers():
+    """test_json_serializer.cpp:442-500."""
+    assert mj.uv_to_json(2.0) == {"Value": 2.0}
+    normal = {"Type": "Normal", "Mean": 0.5, "StandardDev": 0.1, "LowerBound": -1.0, "UpperBound": 2.0,
+              "Quantile": 2.5758, "PredefinedSamples": []}
+    assert mj.uv_to_json(2.0, distribution=normal) == {"Value": 2.0, "Distribution": normal}
+    assert mj.uv_to_json(2.0, mj.IOF_OmitDistributions, normal) == {"Value": 2.0}
+    assert mj.uv_from_json({"Value": 2.0}) == (2.0, None)
+    assert mj.uv_from_json({"Value": 2.0, "Distribution": normal}) == (2.0, normal)
+    assert mj.uv_from_json({"Value": 2.0, "Distribution": normal}, mj.IOF_OmitDistributions) == (2.0, None)
+
+
+def test_array_matrix_tuple_known_answers():
+    """test_json_serializer.cpp:215-228 (tuple), :277-293 (custom index array), :531-550 (matrix)."""
+    js = mj.array1_to_json(np.array([1.0, 2.0]))
+    assert js == {"Dimensions": 2, "Elements": [{"Value": 1.0}, {"Value": 2.0}]}
+    vals, _ = mj.array1_from_json(js)
+    assert vals.tolist() == [1.0, 2.0]
+    m = np.array([[1.0, 2.0, 3.0], [4.0, 5.0, 6.0]])
+    assert mj.matrix_to_json(m) == {"Rows": 2, "Columns": 3, "Elements": [1.0, 2.0, 3.0, 4.0, 5.0, 6.0]}
+    assert np.array_equal(mj.matrix_from_json(mj.matrix_to_json(m)), m)
+    assert mj.matrix_to_json(np.array([1.0, 2.0])) == {"Rows": 2, "Columns": 1, "Elements": [1.0, 2.0]}
+    # a two-category index: MultiIndex {"Indices": tuple}, tuple elements "Element<k>"
+    js = mj.array2_to_json(np.arange(20.0).reshape(2, 10))
+    assert js["Dimensions"] == {"Indices": {"Element0": 2, "Element1": 10}}
+    assert [e["Value"] for e in js["Elements"]] == list(np.arange(20.0))
+    with pytest.raises(mj.JsonFormatError, match="Dimensions"):
+        mj.array1_from_json({"Dimensions": 3, "Elements": [{"Value": 1.0}]})
+
+
+def test_parameter_set_names():
+    """ParameterSet::serialize keys every parameter by its tag's name() (parameter_set.h:335-343,
+    models/ode_secir/parameters.h:47-382); StartDay is a plain number, the other scalars uncertain values."""
+    js = mj.model_to_json(osecir.Model(2))
+    par = js["Parameters"]
+    assert set(par) == {"StartDay", "Seasonality", "ICUCapacity", "TestAndTraceCapacity",
+                        "TestAndTraceCapacityMaxRisk", "ContactPatterns", "DynamicNPIsInfectedSymptoms",
+                        "TimeExposed", "TimeInfectedNoSymptoms", "TimeInfectedSymptoms", "TimeInfectedSevere",
+                        "TimeInfectedCritical", "TransmissionProbabilityOnContact", "RelativeTransmissionNoSymptoms",
+                        "RecoveredPerInfectedNoSymptoms", "RiskOfInfectionFromSymptomatic",
+                        "MaxRiskOfInfectionFromSymptomatic", "SeverePerInfectedSymptoms", "CriticalPerSevere",
+                        "DeathsPerSevere", "DeathsPerCritical"}
+    assert par["StartDay"] == 0.0
+    assert par["TestAndTraceCapacityMaxRisk"] == {"Value": 5.0}
+    assert set(par["ContactPatterns"]) == {"ContactMatrix", "SchoolHolidayDamping", "SchoolHolidays", "Dampings"}
+    assert set(mj.model_to_json(osecir.Model(2), mj.IOF_OmitDistributions)["Parameters"]["ContactPatterns"]) == \
+        {"ContactMatrix"}
+    assert set(par["DynamicNPIsInfectedSymptoms"]) == {"Thresholds", "Duration", "Delay", "BaseValue",
+                                                       "DirectiveBegin", "DirectiveEnd"}
+
+
+# ------------------------------------------------------------------------------ the reference's graph IO tests
+def test_write_read_graph_simple(tmp_path):
+    """pycode/memilio-simulation/tests/test_mobility.py:79-106."""
+    model = osecir.Model(1)
+    model.parameters.TestAndTraceCapacity.value = 42
+    model.apply_constraints()
+    graph = osecir.ModelGraph()
+    graph.add_node(0, model)
+    graph.add_node(1, model)
+    graph.add_edge(0, 1, 0.1 * np.ones(10))
+    graph.add_edge(1, 0, 0.1 * np.ones(10))
+    osecir.write_graph(graph, str(tmp_path))
+    g_read = osecir.read_graph(str(tmp_path))
+    assert graph.num_nodes == g_read.num_nodes
+    assert graph.num_edges == g_read.num_edges
+    assert g_read.get_node(0).property.parameters.TestAndTraceCapacity.value == 42
+    assert sorted(os.listdir(tmp_path)) == ["GraphEdges_node0.json", "GraphEdges_node1.json", "GraphNode0.json",
+                                            "GraphNode1.json"]
+
+
+def _save_parameters_model():
+    """test_save_parameters.cpp:207-245 (three groups)."""
+    G = 3
+    m = osecir.Model(G)
+    p = m.parameters
+    S = osecir.InfectionState
+    fact = 1.0 / G
+    for g in range(G):
+        A = mio.AgeGroup(g)
+        p.TimeExposed[A] = 3.2
+        p.TimeInfectedNoSymptoms[A] = 2.
+        p.TimeInfectedSymptoms[A] = 5.
+        p.TimeInfectedSevere[A] = 10.
+        p.TimeInfectedCritical[A] = 8.
+        m.populations[A, S.Exposed] = fact * 100
+        m.populations[A, S.InfectedNoSymptoms] = fact * 50
+        m.populations[A, S.InfectedSymptoms] = fact * 50
+        m.populations[A, S.InfectedSevere] = fact * 20
+        m.populations[A, S.InfectedCritical] = fact * 10
+        m.populations[A, S.Recovered] = fact * 10
+        m.populations[A, S.Dead] = 0
+        m.populations.set_difference_from_group_total_AgeGroup((A, S.Susceptible), fact * 10000)
+        p.TransmissionProbabilityOnContact[A] = 0.05
+        p.RelativeTransmissionNoSymptoms[A] = 0.67
+        p.RecoveredPerInfectedNoSymptoms[A] = 0.09
+        p.RiskOfInfectionFromSymptomatic[A] = 0.25
+        p.SeverePerInfectedSymptoms[A] = 0.2
+        p.CriticalPerSevere[A] = 0.25
+        p.DeathsPerCritical[A] = 0.3
+    p.ContactPatterns.cont_freq_mat[0] = mio.ContactMatrix(np.full((G, G), fact * 10))
+    return m
+
+
+def test_read_graph_without_edges(tmp_path):
+    """test_save_parameters.cpp:205-290: a graph without edges, read back with IOF_OmitDistributions and
+    without edges, has the written parameters. Reading its edges fails: write_graph writes no edge file for
+    a node without out-edges and read_graph expects one per node (mobility_io.h:96-116, 168-171)."""
+    m = _save_parameters_model()
+    graph = osecir.ModelGraph()
+    graph.add_node(0, m)
+    graph.add_node(1, m)
+    mj.write_graph(graph, str(tmp_path))
+    g = mj.read_graph(str(tmp_path), mj.IOF_OmitDistributions, False)
+    assert g.num_nodes == 2 and g.num_edges == 0
+    for v in range(2):
+        p = g.get_node(v).property.parameters
+        for name in osecir._AGES:
+            assert np.array_equal(p._age[osecir._AGES[name]], m.parameters._age[osecir._AGES[name]]), name
+        assert np.array_equal(g.get_node(v).property.populations._v, m.populations._v)
+        assert np.array_equal(p.ContactPatterns.cont_freq_mat[0].baseline, np.full((3, 3), (1.0 / 3) * 10))
+    with pytest.raises(ValueError):
+        osecir.read_graph(str(tmp_path))
+    with pytest.raises(ValueError):
+        osecir.read_graph(str(tmp_path / "missing"))
+
+
+# ------------------------------------------------------------------------------ a file written from the format
+def _uv(v):
+    return {"Value": v}
+
+
+def _hand_written_node():
+    """GraphNode0.json of a one-group osecir model, typed from the reference's serializers (not produced by
+    this module's writer): dampings, a dynamic NPI and a distribution on one value."""
+    ages = {"TimeExposed": 3.2, "TimeInfectedNoSymptoms": 2.0, "TimeInfectedSymptoms": 6.0,
+            "TimeInfectedSevere": 12.0, "TimeInfectedCritical": 8.0, "TransmissionProbabilityOnContact": 0.9,
+            "RelativeTransmissionNoSymptoms": 0.67, "RecoveredPerInfectedNoSymptoms": 0.09,
+            "RiskOfInfectionFromSymptomatic": 0.25, "MaxRiskOfInfectionFromSymptomatic": 0.5,
+            "SeverePerInfectedSymptoms": 0.2, "CriticalPerSevere": 0.25, "DeathsPerSevere": 0.01,
+            "DeathsPerCritical": 0.3}
+    par = {k: {"Dimensions": 1, "Elements": [_uv(v)]} for k, v in ages.items()}
+    par["TimeExposed"]["Elements"][0]["Distribution"] = {
+        "Type": "Uniform", "LowerBound": 3.0, "UpperBound": 3.4, "PredefinedSamples": []}
+    par.update({"StartDay": 60.0, "Seasonality": _uv(0.2), "ICUCapacity": _uv(1e4),
+                "TestAndTraceCapacity": _uv(35.0), "TestAndTraceCapacityMaxRisk": _uv(5.0)})
+    mat = lambda v: {"Rows": 1, "Columns": 1, "Elements": [v]}  # noqa: E731
+    par["ContactPatterns"] = {
+        "ContactMatrix": {"Matrices": [{"Baseline": mat(10.0), "Minimum": mat(1.0),
+                                        "Dampings": [{"Time": 30.0, "Type": 0, "Level": 0, "Coeffs": mat(0.7)}]}]},
+        "SchoolHolidayDamping": {"Time": 0.0, "Type": 0, "Level": 0, "Value": _uv(0.0), "MatrixIndices": [],
+                                 "GroupWeights": {"Rows": 1, "Columns": 1, "Elements": [0.0]}},
+        "SchoolHolidays": [{"Element0": 10.0, "Element1": 20.0}],
+        "Dampings": []}
+    par["DynamicNPIsInfectedSymptoms"] = {
+        "Thresholds": [{"Element0": 500.0, "Element1": [
+            {"Time": 0.0, "Type": 0, "Level": 1, "Value": _uv(0.4), "MatrixIndices": [0],
+             "GroupWeights": {"Rows": 1, "Columns": 1, "Elements": [1.0]}}]}],
+        "Duration": 14.0, "Delay": 2.0, "BaseValue": 100000.0, "DirectiveBegin": 5.0, "DirectiveEnd": 90.0}
+    pops = [9000.0, 100.0, 50.0, 0.0, 50.0, 0.0, 20.0, 10.0, 10.0, 0.0]
+    return {"NodeId": 7, "Model": {"Parameters": par, "Populations": {
+        "Dimensions": {"Indices": {"Element0": 1, "Element1": 10}}, "Elements": [_uv(v) for v in pops]}}}
+
+
+def test_read_hand_written_file(tmp_path):
+    node = _hand_written_node()
+    (tmp_path / "GraphNode0.json").write_text(json.dumps(node))
+    (tmp_path / "GraphNode1.json").write_text(json.dumps(dict(node, NodeId=8)))
+    edge = {"StartNodeIndex": 0, "EndNodeIndex": 1, "Parameters": {
+        "Coefficients": {"Matrices": [{"Baseline": {"Rows": 10, "Columns": 1, "Elements": [0.1] * 10},
+                                       "Minimum": {"Rows": 10, "Columns": 1, "Elements": [0.02] * 10},
+                                       "Dampings": [{"Time": 4.0, "Type": 0, "Level": 0,
+                                                     "Coeffs": {"Rows": 10, "Columns": 1,
+                                                                "Elements": [0.5] * 10}}]}]},
+        "DynamicNPIs": {"Thresholds": [], "Duration": 14.0, "Delay": 0.0, "BaseValue": 1.0,
+                        "DirectiveBegin": -1.7976931348623157e308, "DirectiveEnd": 1.7976931348623157e308}}}
+    (tmp_path / "GraphEdges_node0.json").write_text(json.dumps([edge]))
+    (tmp_path / "GraphEdges_node1.json").write_text(json.dumps([dict(edge, StartNodeIndex=1, EndNodeIndex=0)]))
+    g = osecir.read_graph(str(tmp_path))
+    assert [g.get_node(v).id for v in range(2)] == [7, 8]
+    m = g.get_node(0).property
+    p = m.parameters
+    A0 = mio.AgeGroup(0)
+    S = osecir.InfectionState
+    assert p.TimeExposed[A0].value == 3.2 and p.DeathsPerSevere[A0].value == 0.01
+    assert p.MaxRiskOfInfectionFromSymptomatic[A0].value == 0.5
+    assert p.StartDay.value == 60.0 and p.Seasonality.value == 0.2 and p.ICUCapacity.value == 1e4
+    assert p.TestAndTraceCapacity.value == 35.0
+    assert m.populations[A0, S.Exposed].value == 100.0 and m.populations[A0, S.InfectedCritical].value == 10.0
+    cm = p.ContactPatterns.cont_freq_mat[0]
+    assert cm.baseline.tolist() == [[10.0]] and cm.minimum.tolist() == [[1.0]]
+    assert [(d.time, d.level, d.type, d.coeffs.tolist()) for d in cm.dampings] == [(30.0, 0, 0, [[0.7]])]
+    assert p.ContactPatterns.school_holidays == [(10.0, 20.0)]
+    dn = p.DynamicNPIsInfectedSymptoms
+    assert (dn.duration, dn.implementation_delay, dn.base_value, dn.directive_begin, dn.directive_end) == \
+        (14.0, 2.0, 1e5, 5.0, 90.0)
+    (thr, ds), = dn.threshold
+    assert thr == 500.0 and ds[0].value.value == 0.4 and ds[0].level == 1 and ds[0].matrix_indices == [0]
+    e = g.get_edge(0).property
+    c = e.coefficients[0]
+    assert c.baseline.tolist() == [0.1] * 10 and c.minimum.tolist() == [0.02] * 10
+    assert [(d.time, d.coeffs.tolist()) for d in c.dampings] == [(4.0, [0.5] * 10)]
+    # written back: the same objects, the distribution kept
+    out = tmp_path / "out"
+    osecir.write_graph(g, str(out))
+    again = json.loads((out / "GraphNode0.json").read_text())
+    assert again == node
+    assert json.loads((out / "GraphEdges_node0.json").read_text()) == [edge]
+
+
+def test_malformed_files(tmp_path):
+    node = _hand_written_node()
+    bad = dict(node, NodeId="seven")
+    (tmp_path / "GraphNode0.json").write_text(json.dumps(bad))
+    with pytest.raises(RuntimeError, match="NodeId"):
+        mj.read_graph(str(tmp_path), read_edges=False)
+    del node["Model"]["Parameters"]["TimeExposed"]
+    (tmp_path / "GraphNode0.json").write_text(json.dumps(node))
+    with pytest.raises(RuntimeError, match="TimeExposed"):
+        mj.read_graph(str(tmp_path), read_edges=False)
+    node = _hand_written_node()
+    (tmp_path / "GraphNode0.json").write_text(json.dumps(node))
+    (tmp_path / "GraphEdges_node0.json").write_text(json.dumps([{"StartNodeIndex": 0, "EndNodeIndex": 3,
+                                                                 "Parameters": {}}]))
+    with pytest.raises(RuntimeError, match="EndNodeIndex not in range"):
+        mj.read_graph(str(tmp_path))
+
+
+# ------------------------------------------------------------------------------ round trips
+def _descriptor_arrays(spec):
+    keys = ["node_scalars", "age_params", "contact_baseline", "contact_minimum", "populations", "edge_start",
+            "edge_end", "edge_coeffs", "edge_coeff_minimum", "damping_offsets", "damping_times", "damping_levels",
+            "damping_types", "damping_coeffs"]
+    out = {k: getattr(spec, k) for k in keys}
+    for grp in ("npis", "out_npis", "out_dampings"):
+        d = getattr(spec, grp)
+        for k, v in (d or {}).items():
+            out[f"{grp}.{k}"] = v
+    return out
+
+
+def _feature_graph():
+    """Several contact matrices (one damped), dynamic NPIs, damped edges with minimum and edge NPIs, two age
+    groups, distinct nodes."""
+    graph = osecir.ModelGraph()
+    for v in range(4):
+        m = ps.study_model(G=2, seed=v)
+        grp = m.parameters.ContactPatterns.cont_freq_mat
+        extra = mio.ContactMatrixGroup(2, 2)
+        extra[0] = grp[0]
+        extra[1] = mio.ContactMatrix(np.full((2, 2), 0.5 + v))
+        m.parameters.ContactPatterns.cont_freq_mat = extra
+        dn = m.parameters.DynamicNPIsInfectedSymptoms
+        dn.duration, dn.implementation_delay, dn.base_value = 9.0, 1.5, 1e4
+        dn.set_threshold(50.0 + v, [mio.DampingSampling(0.3, 1, 0, 0.0, [0], np.array([1.0, 0.5]))])
+        graph.add_node(10 + v, m)
+    for v in range(4):
+        mp = mio.MobilityParameters(np.linspace(0.01, 0.03, 20) * (v + 1))
+        mp.coefficients[0].minimum = 0.1 * mp.coefficients[0].baseline
+        mp.coefficients.add_damping(mio.MobilityDamping(np.repeat([0.4, 0.6], 10), t=5.5, level=0, type=1))
+        mp.dynamic_npis_infected.set_threshold(80.0, [mio.DampingSampling(0.5, 0, 0, 0.0, [0], np.ones(2))])
+        graph.add_edge(v, (v + 1) % 4, mp)
+    return graph
+
+
+def test_round_trip_keeps_descriptor_and_run(tmp_path, oracle):
+    graph = _feature_graph()
+    osecir.write_graph(graph, str(tmp_path))
+    back = osecir.read_graph(str(tmp_path))
+    assert [back.get_node(v).id for v in range(4)] == [10, 11, 12, 13]
+
+    def spec_of(g):
+        return osecir._spec_from([g.get_node(v).property for v in range(g.num_nodes)],
+                                 [(e.start_node_idx, e.end_node_idx, e.property) for e in g._edges])
+
+    a, b = spec_of(graph), spec_of(back)
+    da, db = _descriptor_arrays(a), _descriptor_arrays(b)
+    assert da.keys() == db.keys()
+    for k in da:
+        if da[k] is None:
+            assert db[k] is None, k
+        else:
+            assert np.array_equal(np.asarray(da[k]), np.asarray(db[k])), k
+    cfg = osecir.ControlledStepperWrapper(1e-8, 1e-8).cfg()
+    from memilio_amd import capi
+    sc = capi.sim_cfg(t0=0.0, dt_graph=0.5, dt_node=0.1, integrator=cfg)
+    ra, _ = oracle.graph_run(a, sc, 20.0)
+    rb, _ = oracle.graph_run(b, sc, 20.0)
+    assert np.array_equal(ra, rb)
+    # a second write of what was read gives the same files
+    osecir.write_graph(back, str(tmp_path / "again"))
+    for name in sorted(os.listdir(tmp_path)):
+        if name.endswith(".json"):
+            assert (tmp_path / name).read_text() == (tmp_path / "again" / name).read_text(), name
+
+
+def test_model_deepcopy_keeps_dynamic_npis():
+    """Graph.add_node copies the model (graph.h:214-219); the copy keeps its dynamic NPIs."""
+    m = osecir.Model(1)
+    m.parameters.DynamicNPIsInfectedSymptoms.set_threshold(10.0, [mio.DampingSampling(0.5, 0, 0, 0.0, [0],
+                                                                                      np.ones(1))])
+    g = osecir.ModelGraph()
+    g.add_node(0, m)
+    assert [t for t, _ in g.get_node(0).property.parameters.DynamicNPIsInfectedSymptoms.threshold] == [10.0]
