@@ -35,8 +35,10 @@
 //   * ParameterStudy::run executes every simulation create_simulation returns in one device batch, then
 //     calls process_result in run order. Parameter sampling stays with create_simulation.
 //   * Errors are thrown as mgo_mio::Error (status + message) instead of the reference's asserts.
-//   * Edge result tracking (MobilityEdge save_indices / get_mobility_results) and dynamic NPIs are not
-//     provided; passing save indices is accepted and ignored.
+//   * Edge result tracking (MobilityEdge save_indices / get_mobility_results) is not provided; passing save
+//     indices is accepted and ignored.
+//
+// Parameter-graph JSON files (mio::write_graph / read_graph) are in mgo_mio_io.hpp.
 //
 // The names live in namespace mgo_mio; unless MGO_MIO_NO_ALIAS is defined, `mio` is an alias of it.
 #ifndef MGO_MIO_HPP

@@ -1,0 +1,1258 @@
+// mgo_mio_io.hpp - parameter-graph JSON files for the mio:: layer of mgo_mio.hpp: mio::write_graph and
+// mio::read_graph of Graph<osecir::Model, MobilityParameters>.
+//
+// Reference interfaces (paths relative to the memilio source tree):
+//   write_graph / read_graph            cpp/memilio/io/mobility_io.h:70-192 (GraphNode<i>.json with
+//                                       {"NodeId", "Model"}, GraphEdges_node<i>.json with a list of
+//                                       {"StartNodeIndex", "EndNodeIndex", "Parameters"})
+//   IOResult, IOStatus, IOFlags          cpp/memilio/io/io.h:66-90, 197-354
+//   object layouts                       the JSON serializers of the reference's types: uncertain values
+//                                       {"Value"[, "Distribution"]}, arrays {"Dimensions", "Elements"},
+//                                       matrices {"Rows", "Columns", "Elements" row-major}, tuples
+//                                       {"Element<k>"}, dampings, contact matrices, damping samplings, dynamic
+//                                       NPIs, the osecir model {"Parameters", "Populations"} and mobility
+//                                       parameters {"Coefficients", "DynamicNPIs"} (see memilio_amd/simulation/
+//                                       _json.py for the same layouts with their reference lines)
+//
+// The reference builds these files with jsoncpp; this header has its own small JSON reader and writer so
+// that it needs nothing beyond the standard library. Differences a caller sees: distributions of uncertain
+// values are not kept (the mio:: layer holds plain values; reading skips them), and the parameter-study
+// inputs of the contact patterns (school holidays and their damping, contact dampings to sample) are
+// written with their default values and skipped when read.
+#ifndef MGO_MIO_IO_HPP
+#define MGO_MIO_IO_HPP
+
+#include <cerrno>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <filesystem>
+#include <fstream>
+#include <optional>
+#include <sstream>
+
+#include "mgo_mio.hpp"
+
+namespace mgo_mio
+{
+
+enum IOFlags
+{
+    IOF_None              = 0,
+    IOF_OmitDistributions = 1 << 0,
+    IOF_OmitValues        = 1 << 1,
+    IOF_IncludeTypeInfo   = 1 << 2,
+};
+
+// StatusCode values the graph files can fail with (io.h:49-62)
+enum class StatusCode
+{
+    OK = 0,
+    UnknownError,
+    OutOfRange,
+    InvalidValue,
+    InvalidFileFormat,
+    KeyNotFound,
+    InvalidType,
+    FileNotFound,
+};
+
+class IOStatus
+{
+public:
+    IOStatus() = default;
+    IOStatus(StatusCode code, std::string msg)
+        : m_code(code)
+        , m_msg(std::move(msg))
+    {
+    }
+    StatusCode code() const
+    {
+        return m_code;
+    }
+    const std::string& message() const
+    {
+        return m_msg;
+    }
+    bool is_ok() const
+    {
+        return m_code == StatusCode::OK;
+    }
+    std::string formatted_message() const
+    {
+        return "[StatusCode " + std::to_string(int(m_code)) + "] " + m_msg;
+    }
+
+private:
+    StatusCode m_code = StatusCode::OK;
+    std::string m_msg;
+};
+
+// Result or error of an IO call; true when it holds a value (the call shapes of io.h's IOResult).
+template <class T>
+class IOResult
+{
+public:
+    IOResult(T v)
+        : m_value(std::move(v))
+        , m_ok(true)
+    {
+    }
+    IOResult(IOStatus e)
+        : m_status(std::move(e))
+    {
+    }
+    explicit operator bool() const
+    {
+        return m_ok;
+    }
+    bool has_value() const
+    {
+        return m_ok;
+    }
+    bool has_error() const
+    {
+        return !m_ok;
+    }
+    T& value()
+    {
+        if (!m_ok) {
+            throw Error(MGO_ERR_INVALID_VALUE, m_status.formatted_message());
+        }
+        return *m_value;
+    }
+    const IOStatus& error() const
+    {
+        return m_status;
+    }
+
+private:
+    std::optional<T> m_value;
+    IOStatus m_status;
+    bool m_ok = false;
+};
+
+template <>
+class IOResult<void>
+{
+public:
+    IOResult() = default;
+    IOResult(IOStatus e)
+        : m_status(std::move(e))
+        , m_ok(false)
+    {
+    }
+    explicit operator bool() const
+    {
+        return m_ok;
+    }
+    bool has_value() const
+    {
+        return m_ok;
+    }
+    bool has_error() const
+    {
+        return !m_ok;
+    }
+    void value() const
+    {
+        if (!m_ok) {
+            throw Error(MGO_ERR_INVALID_VALUE, m_status.formatted_message());
+        }
+    }
+    const IOStatus& error() const
+    {
+        return m_status;
+    }
+
+private:
+    IOStatus m_status;
+    bool m_ok = true;
+};
+
+namespace json
+{
+
+// JSON document value; objects keep their members in insertion order.
+class Value
+{
+public:
+    enum Kind
+    {
+        Null,
+        Bool,
+        Int,
+        Real,
+        String,
+        Array,
+        Object
+    };
+    Value() = default;
+    static Value integer(long long v)
+    {
+        Value r;
+        r.m_kind = Int;
+        r.m_int  = v;
+        r.m_real = double(v);
+        return r;
+    }
+    static Value real(double v)
+    {
+        Value r;
+        r.m_kind = Real;
+        r.m_real = v;
+        return r;
+    }
+    static Value string(std::string s)
+    {
+        Value r;
+        r.m_kind = String;
+        r.m_str  = std::move(s);
+        return r;
+    }
+    static Value array()
+    {
+        Value r;
+        r.m_kind = Array;
+        return r;
+    }
+    static Value object()
+    {
+        Value r;
+        r.m_kind = Object;
+        return r;
+    }
+    static Value boolean(bool b)
+    {
+        Value r;
+        r.m_kind = Bool;
+        r.m_int  = b;
+        return r;
+    }
+    Kind kind() const
+    {
+        return m_kind;
+    }
+    bool is_number() const
+    {
+        return m_kind == Int || m_kind == Real;
+    }
+    bool is_int() const
+    {
+        return m_kind == Int;
+    }
+    double as_double() const
+    {
+        return m_real;
+    }
+    long long as_int() const
+    {
+        return m_int;
+    }
+    const std::string& as_string() const
+    {
+        return m_str;
+    }
+    // arrays
+    void append(Value v)
+    {
+        m_items.push_back(std::move(v));
+    }
+    size_t size() const
+    {
+        return m_kind == Array ? m_items.size() : m_members.size();
+    }
+    const Value& operator[](size_t i) const
+    {
+        return m_items.at(i);
+    }
+    const std::vector<Value>& items() const
+    {
+        return m_items;
+    }
+    // objects
+    Value& set(const std::string& key, Value v)
+    {
+        for (auto& m : m_members) {
+            if (m.first == key) {
+                m.second = std::move(v);
+                return m.second;
+            }
+        }
+        m_members.emplace_back(key, std::move(v));
+        return m_members.back().second;
+    }
+    const Value* find(const std::string& key) const
+    {
+        for (auto& m : m_members) {
+            if (m.first == key) {
+                return &m.second;
+            }
+        }
+        return nullptr;
+    }
+    const std::vector<std::pair<std::string, Value>>& members() const
+    {
+        return m_members;
+    }
+
+private:
+    Kind m_kind = Null;
+    long long m_int = 0;
+    double m_real   = 0.0;
+    std::string m_str;
+    std::vector<Value> m_items;
+    std::vector<std::pair<std::string, Value>> m_members;
+};
+
+// strict recursive-descent reader (RFC 8259 grammar); throws std::runtime_error with the offset
+class Reader
+{
+public:
+    explicit Reader(const std::string& text)
+        : m_s(text)
+    {
+    }
+    Value parse()
+    {
+        Value v = value(0);
+        ws();
+        if (m_p != m_s.size()) {
+            fail("trailing characters");
+        }
+        return v;
+    }
+
+private:
+    [[noreturn]] void fail(const char* what) const
+    {
+        throw std::runtime_error(std::string("JSON: ") + what + " at offset " + std::to_string(m_p));
+    }
+    void ws()
+    {
+        while (m_p < m_s.size() && (m_s[m_p] == ' ' || m_s[m_p] == '\n' || m_s[m_p] == '\r' || m_s[m_p] == '\t')) {
+            ++m_p;
+        }
+    }
+    bool eat(char c)
+    {
+        ws();
+        if (m_p < m_s.size() && m_s[m_p] == c) {
+            ++m_p;
+            return true;
+        }
+        return false;
+    }
+    void expect(char c)
+    {
+        if (!eat(c)) {
+            fail("unexpected character");
+        }
+    }
+    bool literal(const char* w)
+    {
+        const size_t n = std::char_traits<char>::length(w);
+        if (m_s.compare(m_p, n, w) == 0) {
+            m_p += n;
+            return true;
+        }
+        return false;
+    }
+    static void utf8(std::string& out, unsigned cp)
+    {
+        if (cp < 0x80) {
+            out += char(cp);
+        }
+        else if (cp < 0x800) {
+            out += char(0xC0 | (cp >> 6));
+            out += char(0x80 | (cp & 0x3F));
+        }
+        else if (cp < 0x10000) {
+            out += char(0xE0 | (cp >> 12));
+            out += char(0x80 | ((cp >> 6) & 0x3F));
+            out += char(0x80 | (cp & 0x3F));
+        }
+        else {
+            out += char(0xF0 | (cp >> 18));
+            out += char(0x80 | ((cp >> 12) & 0x3F));
+            out += char(0x80 | ((cp >> 6) & 0x3F));
+            out += char(0x80 | (cp & 0x3F));
+        }
+    }
+    unsigned hex4()
+    {
+        if (m_p + 4 > m_s.size()) {
+            fail("short \\u escape");
+        }
+        unsigned v = 0;
+        for (int k = 0; k < 4; ++k) {
+            const char c = m_s[m_p++];
+            v <<= 4;
+            if (c >= '0' && c <= '9') {
+                v |= unsigned(c - '0');
+            }
+            else if (c >= 'a' && c <= 'f') {
+                v |= unsigned(c - 'a' + 10);
+            }
+            else if (c >= 'A' && c <= 'F') {
+                v |= unsigned(c - 'A' + 10);
+            }
+            else {
+                fail("bad \\u escape");
+            }
+        }
+        return v;
+    }
+    std::string str()
+    {
+        expect('"');
+        std::string out;
+        while (true) {
+            if (m_p >= m_s.size()) {
+                fail("unterminated string");
+            }
+            const char c = m_s[m_p++];
+            if (c == '"') {
+                return out;
+            }
+            if (static_cast<unsigned char>(c) < 0x20) {
+                fail("control character in string");
+            }
+            if (c != '\\') {
+                out += c;
+                continue;
+            }
+            if (m_p >= m_s.size()) {
+                fail("unterminated escape");
+            }
+            const char e = m_s[m_p++];
+            switch (e) {
+            case '"':
+                out += '"';
+                break;
+            case '\\':
+                out += '\\';
+                break;
+            case '/':
+                out += '/';
+                break;
+            case 'b':
+                out += '\b';
+                break;
+            case 'f':
+                out += '\f';
+                break;
+            case 'n':
+                out += '\n';
+                break;
+            case 'r':
+                out += '\r';
+                break;
+            case 't':
+                out += '\t';
+                break;
+            case 'u': {
+                unsigned cp = hex4();
+                if (cp >= 0xD800 && cp < 0xDC00) {
+                    if (!literal("\\u")) {
+                        fail("lone surrogate");
+                    }
+                    const unsigned lo = hex4();
+                    if (lo < 0xDC00 || lo >= 0xE000) {
+                        fail("bad surrogate pair");
+                    }
+                    cp = 0x10000 + ((cp - 0xD800) << 10) + (lo - 0xDC00);
+                }
+                utf8(out, cp);
+                break;
+            }
+            default:
+                fail("bad escape");
+            }
+        }
+    }
+    Value number()
+    {
+        const size_t b = m_p;
+        bool real      = false;
+        if (m_s[m_p] == '-') {
+            ++m_p;
+        }
+        auto digits = [&] {
+            const size_t d = m_p;
+            while (m_p < m_s.size() && m_s[m_p] >= '0' && m_s[m_p] <= '9') {
+                ++m_p;
+            }
+            if (m_p == d) {
+                fail("digit expected");
+            }
+        };
+        if (m_p < m_s.size() && m_s[m_p] == '0') {
+            ++m_p;
+        }
+        else {
+            digits();
+        }
+        if (m_p < m_s.size() && m_s[m_p] == '.') {
+            real = true;
+            ++m_p;
+            digits();
+        }
+        if (m_p < m_s.size() && (m_s[m_p] == 'e' || m_s[m_p] == 'E')) {
+            real = true;
+            ++m_p;
+            if (m_p < m_s.size() && (m_s[m_p] == '+' || m_s[m_p] == '-')) {
+                ++m_p;
+            }
+            digits();
+        }
+        const std::string tok = m_s.substr(b, m_p - b);
+        if (!real) {
+            errno             = 0;
+            const long long v = std::strtoll(tok.c_str(), nullptr, 10);
+            if (errno == 0) {
+                return Value::integer(v);
+            }
+        }
+        return Value::real(std::strtod(tok.c_str(), nullptr));
+    }
+    Value value(int depth)
+    {
+        if (depth > 256) {
+            fail("nesting too deep");
+        }
+        ws();
+        if (m_p >= m_s.size()) {
+            fail("value expected");
+        }
+        const char c = m_s[m_p];
+        if (c == '{') {
+            ++m_p;
+            Value o = Value::object();
+            if (eat('}')) {
+                return o;
+            }
+            do {
+                ws();
+                std::string k = str();
+                expect(':');
+                o.set(k, value(depth + 1));
+            } while (eat(','));
+            expect('}');
+            return o;
+        }
+        if (c == '[') {
+            ++m_p;
+            Value a = Value::array();
+            if (eat(']')) {
+                return a;
+            }
+            do {
+                a.append(value(depth + 1));
+            } while (eat(','));
+            expect(']');
+            return a;
+        }
+        if (c == '"') {
+            return Value::string(str());
+        }
+        if (c == '-' || (c >= '0' && c <= '9')) {
+            return number();
+        }
+        if (literal("true")) {
+            return Value::boolean(true);
+        }
+        if (literal("false")) {
+            return Value::boolean(false);
+        }
+        if (literal("null")) {
+            return Value();
+        }
+        fail("unexpected character");
+    }
+
+    const std::string& m_s;
+    size_t m_p = 0;
+};
+
+inline void write_string(std::string& out, const std::string& s)
+{
+    out += '"';
+    for (const char c : s) {
+        switch (c) {
+        case '"':
+            out += "\\\"";
+            break;
+        case '\\':
+            out += "\\\\";
+            break;
+        case '\n':
+            out += "\\n";
+            break;
+        case '\r':
+            out += "\\r";
+            break;
+        case '\t':
+            out += "\\t";
+            break;
+        default:
+            if (static_cast<unsigned char>(c) < 0x20) {
+                char buf[8];
+                std::snprintf(buf, sizeof buf, "\\u%04x", unsigned(c));
+                out += buf;
+            }
+            else {
+                out += c;
+            }
+        }
+    }
+    out += '"';
+}
+
+// doubles with 17 significant digits (exact round trip) and always a '.' or exponent
+inline void write_real(std::string& out, double v)
+{
+    if (!std::isfinite(v)) {
+        throw std::runtime_error("JSON: non-finite number");
+    }
+    char buf[40];
+    std::snprintf(buf, sizeof buf, "%.17g", v);
+    out += buf;
+    if (!std::strpbrk(buf, ".eE")) {
+        out += ".0";
+    }
+}
+
+inline void write(std::string& out, const Value& v, int indent, int level)
+{
+    auto nl = [&](int l) {
+        if (indent > 0) {
+            out += '\n';
+            out.append(size_t(indent * l), ' ');
+        }
+    };
+    switch (v.kind()) {
+    case Value::Null:
+        out += "null";
+        break;
+    case Value::Bool:
+        out += v.as_int() ? "true" : "false";
+        break;
+    case Value::Int:
+        out += std::to_string(v.as_int());
+        break;
+    case Value::Real:
+        write_real(out, v.as_double());
+        break;
+    case Value::String:
+        write_string(out, v.as_string());
+        break;
+    case Value::Array:
+        out += '[';
+        for (size_t i = 0; i < v.items().size(); ++i) {
+            if (i) {
+                out += ',';
+            }
+            nl(level + 1);
+            write(out, v.items()[i], indent, level + 1);
+        }
+        if (!v.items().empty()) {
+            nl(level);
+        }
+        out += ']';
+        break;
+    case Value::Object:
+        out += '{';
+        for (size_t i = 0; i < v.members().size(); ++i) {
+            if (i) {
+                out += ',';
+            }
+            nl(level + 1);
+            write_string(out, v.members()[i].first);
+            out += indent > 0 ? ": " : ":";
+            write(out, v.members()[i].second, indent, level + 1);
+        }
+        if (!v.members().empty()) {
+            nl(level);
+        }
+        out += '}';
+        break;
+    }
+}
+
+inline std::string to_string(const Value& v, int indent = 1)
+{
+    std::string out;
+    write(out, v, indent, 0);
+    return out;
+}
+
+} // namespace json
+
+namespace detail
+{
+namespace jio
+{
+using json::Value;
+
+// thrown inside the readers, turned into an IOStatus at the file boundary
+struct Fail {
+    StatusCode code;
+    std::string msg;
+};
+
+inline const Value& get(const Value& o, const char* key, const std::string& where)
+{
+    if (o.kind() != Value::Object) {
+        throw Fail{StatusCode::InvalidType, where + ": object expected"};
+    }
+    const Value* v = o.find(key);
+    if (!v) {
+        throw Fail{StatusCode::KeyNotFound, where + ": element " + key + " missing"};
+    }
+    return *v;
+}
+inline double num(const Value& v, const std::string& where)
+{
+    if (!v.is_number()) {
+        throw Fail{StatusCode::InvalidType, where + ": number expected"};
+    }
+    return v.as_double();
+}
+inline long long integer(const Value& v, const std::string& where)
+{
+    if (!v.is_int()) {
+        throw Fail{StatusCode::InvalidType, where + ": integer expected"};
+    }
+    return v.as_int();
+}
+inline const Value& list(const Value& v, const std::string& where)
+{
+    if (v.kind() != Value::Array) {
+        throw Fail{StatusCode::InvalidType, where + ": list expected"};
+    }
+    return v;
+}
+
+inline Value uv(double v, int flags)
+{
+    Value o = Value::object();
+    if (!(flags & IOF_OmitValues)) {
+        o.set("Value", Value::real(v));
+    }
+    return o;
+}
+inline double uv_read(const Value& o, int flags, const std::string& where)
+{
+    if (o.kind() != Value::Object) {
+        throw Fail{StatusCode::InvalidType, where + ": uncertain value expected"};
+    }
+    return (flags & IOF_OmitValues) ? 0.0 : num(get(o, "Value", where), where);
+}
+
+inline Value matrix(size_t rows, size_t cols, const double* rowmajor)
+{
+    Value o = Value::object();
+    o.set("Rows", Value::integer((long long)rows));
+    o.set("Columns", Value::integer((long long)cols));
+    Value el = Value::array();
+    for (size_t i = 0; i < rows * cols; ++i) {
+        el.append(Value::real(rowmajor[i]));
+    }
+    o.set("Elements", std::move(el));
+    return o;
+}
+inline Matrix matrix_read(const Value& o, const std::string& where)
+{
+    const long long r = integer(get(o, "Rows", where), where), c = integer(get(o, "Columns", where), where);
+    const Value& el   = list(get(o, "Elements", where), where);
+    if (r < 0 || c < 0 || el.size() != size_t(r * c)) {
+        throw Fail{StatusCode::OutOfRange, where + ": matrix size does not match its elements"};
+    }
+    Matrix m{size_t(r), size_t(c)};
+    for (long long i = 0; i < r; ++i) {
+        for (long long j = 0; j < c; ++j) {
+            m(size_t(i), size_t(j)) = num(el[size_t(i * c + j)], where);
+        }
+    }
+    return m;
+}
+inline Vector vector_read(const Value& o, const std::string& where)
+{
+    const Matrix m = matrix_read(o, where);
+    if (m.cols() != 1) {
+        throw Fail{StatusCode::InvalidValue, where + ": column vector expected"};
+    }
+    Vector v(m.rows());
+    for (size_t i = 0; i < m.rows(); ++i) {
+        v[i] = m(i, 0);
+    }
+    return v;
+}
+
+inline Value damping(double t, int level, int type, Value coeffs)
+{
+    Value o = Value::object();
+    o.set("Time", Value::real(t));
+    o.set("Type", Value::integer(type));
+    o.set("Level", Value::integer(level));
+    o.set("Coeffs", std::move(coeffs));
+    return o;
+}
+
+template <class FP>
+Value damping_sampling(const DampingSampling<FP>& d, int flags)
+{
+    Value o = Value::object();
+    o.set("Time", Value::real(d.get_time().get()));
+    o.set("Type", Value::integer(d.get_type().get()));
+    o.set("Level", Value::integer(d.get_level().get()));
+    o.set("Value", uv(d.get_value(), flags));
+    Value mi = Value::array();
+    for (size_t k : d.get_matrix_indices()) {
+        mi.append(Value::integer((long long)k));
+    }
+    o.set("MatrixIndices", std::move(mi));
+    o.set("GroupWeights", matrix(d.get_group_weights().size(), 1, d.get_group_weights().data()));
+    return o;
+}
+template <class FP>
+DampingSampling<FP> damping_sampling_read(const Value& o, int flags, const std::string& where)
+{
+    std::vector<size_t> mats;
+    for (const Value& k : list(get(o, "MatrixIndices", where), where).items()) {
+        const long long i = integer(k, where);
+        if (i < 0) {
+            throw Fail{StatusCode::OutOfRange, where + ": negative matrix index"};
+        }
+        mats.push_back(size_t(i));
+    }
+    return DampingSampling<FP>(uv_read(get(o, "Value", where), flags, where),
+                               DampingLevel(int(integer(get(o, "Level", where), where))),
+                               DampingType(int(integer(get(o, "Type", where), where))),
+                               SimulationTime(num(get(o, "Time", where), where)), std::move(mats),
+                               vector_read(get(o, "GroupWeights", where), where));
+}
+
+template <class FP>
+Value dynamic_npis(const DynamicNPIs<FP>& n, int flags)
+{
+    Value o   = Value::object();
+    Value ths = Value::array();
+    for (const auto& th : n.get_thresholds()) {
+        Value p = Value::object();
+        p.set("Element0", Value::real(th.first));
+        Value ds = Value::array();
+        for (const auto& d : th.second) {
+            ds.append(damping_sampling(d, flags));
+        }
+        p.set("Element1", std::move(ds));
+        ths.append(std::move(p));
+    }
+    o.set("Thresholds", std::move(ths));
+    o.set("Duration", Value::real(n.get_duration().get()));
+    o.set("Delay", Value::real(n.get_implementation_delay().get()));
+    o.set("BaseValue", Value::real(n.get_base_value()));
+    o.set("DirectiveBegin", Value::real(n.get_directive_begin().get()));
+    o.set("DirectiveEnd", Value::real(n.get_directive_end().get()));
+    return o;
+}
+template <class FP>
+DynamicNPIs<FP> dynamic_npis_read(const Value& o, int flags, const std::string& where)
+{
+    DynamicNPIs<FP> n;
+    n.set_duration(SimulationTime(num(get(o, "Duration", where), where)));
+    n.set_implementation_delay(SimulationTime(num(get(o, "Delay", where), where)));
+    n.set_base_value(num(get(o, "BaseValue", where), where));
+    for (const Value& p : list(get(o, "Thresholds", where), where).items()) {
+        std::vector<DampingSampling<FP>> ds;
+        for (const Value& d : list(get(p, "Element1", where), where).items()) {
+            ds.push_back(damping_sampling_read<FP>(d, flags, where));
+        }
+        n.set_threshold(num(get(p, "Element0", where), where), ds);
+    }
+    n.set_directive_begin(SimulationTime(num(get(o, "DirectiveBegin", where), where)));
+    n.set_directive_end(SimulationTime(num(get(o, "DirectiveEnd", where), where)));
+    return n;
+}
+
+inline Value contact_matrix(const ContactMatrix& cm)
+{
+    Value o = Value::object();
+    const Matrix& b = cm.get_baseline();
+    const Matrix& m = cm.get_minimum();
+    o.set("Baseline", matrix(b.rows(), b.cols(), b.data()));
+    o.set("Minimum", matrix(m.rows(), m.cols(), m.data()));
+    Value ds = Value::array();
+    for (const auto& d : cm.get_dampings()) {
+        ds.append(damping(d.time, d.level, d.type, matrix(d.coeffs.rows(), d.coeffs.cols(), d.coeffs.data())));
+    }
+    o.set("Dampings", std::move(ds));
+    return o;
+}
+inline ContactMatrix contact_matrix_read(const Value& o, const std::string& where)
+{
+    ContactMatrix cm(matrix_read(get(o, "Baseline", where), where), matrix_read(get(o, "Minimum", where), where));
+    for (const Value& d : list(get(o, "Dampings", where), where).items()) {
+        cm.add_damping(matrix_read(get(d, "Coeffs", where), where),
+                       DampingLevel(int(integer(get(d, "Level", where), where))),
+                       DampingType(int(integer(get(d, "Type", where), where))),
+                       SimulationTime(num(get(d, "Time", where), where)));
+    }
+    return cm;
+}
+
+template <class FP>
+Value model(const osecir::Model<FP>& m, int flags)
+{
+    const auto& p = m.parameters;
+    const size_t G = m.get_num_groups();
+    Value par      = Value::object();
+    par.set("StartDay", Value::real(p.scalars()[MGO_P_START_DAY]));
+    const std::pair<const char*, int> scalars[] = {{"Seasonality", MGO_P_SEASONALITY},
+                                                   {"ICUCapacity", MGO_P_ICU_CAPACITY},
+                                                   {"TestAndTraceCapacity", MGO_P_TEST_AND_TRACE_CAPACITY},
+                                                   {"TestAndTraceCapacityMaxRisk", MGO_P_TEST_AND_TRACE_MAX_RISK}};
+    for (const auto& s : scalars) {
+        par.set(s.first, uv(p.scalars()[size_t(s.second)], flags));
+    }
+    // ContactPatterns: the group, then (unless distributions are omitted) the sampling inputs at their
+    // defaults (uncertain_matrix.h:48-56, 209-218)
+    Value cp   = Value::object();
+    Value mats = Value::array();
+    const auto& grp = p.template get<osecir::ContactPatterns<FP>>();
+    for (size_t k = 0; k < grp.get_num_matrices(); ++k) {
+        mats.append(contact_matrix(grp[k]));
+    }
+    Value cmg = Value::object();
+    cmg.set("Matrices", std::move(mats));
+    cp.set("ContactMatrix", std::move(cmg));
+    if (!(flags & IOF_OmitDistributions)) {
+        cp.set("SchoolHolidayDamping",
+               damping_sampling(DampingSampling<FP>(0.0, DampingLevel(0), DampingType(0), SimulationTime(0.0), {},
+                                                    Vector(G, 0.0)),
+                                flags));
+        cp.set("SchoolHolidays", Value::array());
+        cp.set("Dampings", Value::array());
+    }
+    par.set("ContactPatterns", std::move(cp));
+    par.set("DynamicNPIsInfectedSymptoms",
+            dynamic_npis(p.template get<osecir::DynamicNPIsInfectedSymptoms<FP>>(), flags));
+    const std::pair<const char*, int> ages[] = {
+        {"TimeExposed", MGO_A_TIME_EXPOSED},
+        {"TimeInfectedNoSymptoms", MGO_A_TIME_INFECTED_NO_SYMPTOMS},
+        {"TimeInfectedSymptoms", MGO_A_TIME_INFECTED_SYMPTOMS},
+        {"TimeInfectedSevere", MGO_A_TIME_INFECTED_SEVERE},
+        {"TimeInfectedCritical", MGO_A_TIME_INFECTED_CRITICAL},
+        {"TransmissionProbabilityOnContact", MGO_A_TRANSMISSION_PROBABILITY_ON_CONTACT},
+        {"RelativeTransmissionNoSymptoms", MGO_A_RELATIVE_TRANSMISSION_NO_SYMPTOMS},
+        {"RecoveredPerInfectedNoSymptoms", MGO_A_RECOVERED_PER_INFECTED_NO_SYMPTOMS},
+        {"RiskOfInfectionFromSymptomatic", MGO_A_RISK_OF_INFECTION_FROM_SYMPTOMATIC},
+        {"MaxRiskOfInfectionFromSymptomatic", MGO_A_MAX_RISK_OF_INFECTION_FROM_SYMPTOMATIC},
+        {"SeverePerInfectedSymptoms", MGO_A_SEVERE_PER_INFECTED_SYMPTOMS},
+        {"CriticalPerSevere", MGO_A_CRITICAL_PER_SEVERE},
+        {"DeathsPerSevere", MGO_A_DEATHS_PER_SEVERE},
+        {"DeathsPerCritical", MGO_A_DEATHS_PER_CRITICAL}};
+    for (const auto& a : ages) {
+        Value arr = Value::object();
+        arr.set("Dimensions", Value::integer((long long)G));
+        Value el = Value::array();
+        for (size_t g = 0; g < G; ++g) {
+            el.append(uv(p.age_param(a.second)[AgeGroup(g)], flags));
+        }
+        arr.set("Elements", std::move(el));
+        par.set(a.first, std::move(arr));
+    }
+    Value pops = Value::object();
+    Value dims = Value::object();
+    Value idx  = Value::object();
+    idx.set("Element0", Value::integer((long long)G));
+    idx.set("Element1", Value::integer(MGO_NUM_COMPARTMENTS));
+    dims.set("Indices", std::move(idx));
+    pops.set("Dimensions", std::move(dims));
+    Value el = Value::array();
+    for (size_t i = 0; i < m.populations.numel(); ++i) {
+        el.append(uv(m.populations.data()[i], flags));
+    }
+    pops.set("Elements", std::move(el));
+    Value o = Value::object();
+    o.set("Parameters", std::move(par));
+    o.set("Populations", std::move(pops));
+    return o;
+}
+
+template <class FP>
+osecir::Model<FP> model_read(const Value& o, int flags, const std::string& where)
+{
+    const Value& pops = get(o, "Populations", where);
+    const Value& idx  = get(get(pops, "Dimensions", where), "Indices", where);
+    const long long G = integer(get(idx, "Element0", where), where);
+    const long long C = integer(get(idx, "Element1", where), where);
+    if (G < 1 || C != MGO_NUM_COMPARTMENTS) {
+        throw Fail{StatusCode::InvalidValue, where + ": osecir populations need 10 compartments per age group"};
+    }
+    const Value& pel = list(get(pops, "Elements", where), where);
+    if (pel.size() != size_t(G * C)) {
+        throw Fail{StatusCode::OutOfRange, where + ": Dimensions of Array don't match the number of elements."};
+    }
+    osecir::Model<FP> m{int(G)};
+    for (long long g = 0; g < G; ++g) {
+        for (long long c = 0; c < C; ++c) {
+            m.populations[{AgeGroup(size_t(g)), osecir::InfectionState(c)}] =
+                uv_read(pel[size_t(g * C + c)], flags, where);
+        }
+    }
+    const Value& par = get(o, "Parameters", where);
+    auto& p          = m.parameters;
+    p.template get<osecir::StartDay<FP>>()    = num(get(par, "StartDay", where), where);
+    p.template get<osecir::Seasonality<FP>>() = uv_read(get(par, "Seasonality", where), flags, where);
+    p.template get<osecir::ICUCapacity<FP>>() = uv_read(get(par, "ICUCapacity", where), flags, where);
+    p.template get<osecir::TestAndTraceCapacity<FP>>() =
+        uv_read(get(par, "TestAndTraceCapacity", where), flags, where);
+    p.template get<osecir::TestAndTraceCapacityMaxRisk<FP>>() =
+        uv_read(get(par, "TestAndTraceCapacityMaxRisk", where), flags, where);
+    auto age = [&](const char* name, osecir::AgeArray& dst) {
+        const Value& a  = get(par, name, where);
+        const Value& el = list(get(a, "Elements", where), where);
+        if (integer(get(a, "Dimensions", where), where) != G || el.size() != size_t(G)) {
+            throw Fail{StatusCode::OutOfRange, where + ": " + name + " does not have one value per age group"};
+        }
+        for (long long g = 0; g < G; ++g) {
+            dst[AgeGroup(size_t(g))] = uv_read(el[size_t(g)], flags, where);
+        }
+    };
+    age("TimeExposed", p.template get<osecir::TimeExposed<FP>>());
+    age("TimeInfectedNoSymptoms", p.template get<osecir::TimeInfectedNoSymptoms<FP>>());
+    age("TimeInfectedSymptoms", p.template get<osecir::TimeInfectedSymptoms<FP>>());
+    age("TimeInfectedSevere", p.template get<osecir::TimeInfectedSevere<FP>>());
+    age("TimeInfectedCritical", p.template get<osecir::TimeInfectedCritical<FP>>());
+    age("TransmissionProbabilityOnContact", p.template get<osecir::TransmissionProbabilityOnContact<FP>>());
+    age("RelativeTransmissionNoSymptoms", p.template get<osecir::RelativeTransmissionNoSymptoms<FP>>());
+    age("RecoveredPerInfectedNoSymptoms", p.template get<osecir::RecoveredPerInfectedNoSymptoms<FP>>());
+    age("RiskOfInfectionFromSymptomatic", p.template get<osecir::RiskOfInfectionFromSymptomatic<FP>>());
+    age("MaxRiskOfInfectionFromSymptomatic", p.template get<osecir::MaxRiskOfInfectionFromSymptomatic<FP>>());
+    age("SeverePerInfectedSymptoms", p.template get<osecir::SeverePerInfectedSymptoms<FP>>());
+    age("CriticalPerSevere", p.template get<osecir::CriticalPerSevere<FP>>());
+    age("DeathsPerSevere", p.template get<osecir::DeathsPerSevere<FP>>());
+    age("DeathsPerCritical", p.template get<osecir::DeathsPerCritical<FP>>());
+    const Value& cp    = get(par, "ContactPatterns", where);
+    const Value& mats  = list(get(get(cp, "ContactMatrix", where), "Matrices", where), where);
+    if (mats.size() == 0) {
+        throw Fail{StatusCode::InvalidValue, where + ": no contact matrix"};
+    }
+    ContactMatrixGroup grp(mats.size(), size_t(G));
+    for (size_t k = 0; k < mats.size(); ++k) {
+        grp[k] = contact_matrix_read(mats[k], where);
+        if (grp[k].get_baseline().rows() != size_t(G) || grp[k].get_baseline().cols() != size_t(G)) {
+            throw Fail{StatusCode::OutOfRange, where + ": contact matrix size differs from the age groups"};
+        }
+    }
+    if (!(flags & IOF_OmitDistributions)) {
+        // the sampling inputs must be present, as for the reference's reader; their values are not kept
+        damping_sampling_read<FP>(get(cp, "SchoolHolidayDamping", where), flags, where);
+        list(get(cp, "SchoolHolidays", where), where);
+        list(get(cp, "Dampings", where), where);
+    }
+    p.template get<osecir::ContactPatterns<FP>>() = grp;
+    p.template get<osecir::DynamicNPIsInfectedSymptoms<FP>>() =
+        dynamic_npis_read<FP>(get(par, "DynamicNPIsInfectedSymptoms", where), flags, where);
+    return m;
+}
+
+template <class FP>
+Value mobility_parameters(const MobilityParameters<FP>& mp, int flags)
+{
+    Value mats        = Value::array();
+    const auto& group = mp.get_coefficients();
+    for (size_t k = 0; k < group.get_num_matrices(); ++k) {
+        const auto& c = group[k];
+        Value o       = Value::object();
+        o.set("Baseline", matrix(c.get_baseline().size(), 1, c.get_baseline().data()));
+        o.set("Minimum", matrix(c.get_minimum().size(), 1, c.get_minimum().data()));
+        Value ds = Value::array();
+        for (const auto& d : c.get_dampings()) {
+            ds.append(damping(d.time, d.level, d.type, matrix(d.coeffs.size(), 1, d.coeffs.data())));
+        }
+        o.set("Dampings", std::move(ds));
+        mats.append(std::move(o));
+    }
+    Value co = Value::object();
+    co.set("Matrices", std::move(mats));
+    Value o = Value::object();
+    o.set("Coefficients", std::move(co));
+    o.set("DynamicNPIs", dynamic_npis(mp.get_dynamic_npis_infected(), flags));
+    return o;
+}
+
+template <class FP>
+MobilityParameters<FP> mobility_parameters_read(const Value& o, int flags, const std::string& where)
+{
+    const Value& mats = list(get(get(o, "Coefficients", where), "Matrices", where), where);
+    if (mats.size() == 0) {
+        throw Fail{StatusCode::InvalidValue, where + ": no coefficient vector"};
+    }
+    MobilityCoefficientGroup grp(mats.size(), 0);
+    for (size_t k = 0; k < mats.size(); ++k) {
+        MobilityCoefficients c(vector_read(get(mats[k], "Baseline", where), where),
+                               vector_read(get(mats[k], "Minimum", where), where));
+        for (const Value& d : list(get(mats[k], "Dampings", where), where).items()) {
+            c.add_damping(vector_read(get(d, "Coeffs", where), where),
+                          DampingLevel(int(integer(get(d, "Level", where), where))),
+                          DampingType(int(integer(get(d, "Type", where), where))),
+                          SimulationTime(num(get(d, "Time", where), where)));
+        }
+        grp[k] = c;
+    }
+    MobilityParameters<FP> mp(grp);
+    mp.set_dynamic_npis_infected(dynamic_npis_read<FP>(get(o, "DynamicNPIs", where), flags, where));
+    return mp;
+}
+
+inline IOResult<json::Value> read_json(const std::string& path)
+{
+    std::ifstream f(path);
+    if (!f) {
+        return IOStatus(StatusCode::FileNotFound, path);
+    }
+    std::stringstream ss;
+    ss << f.rdbuf();
+    try {
+        return json::Reader(ss.str()).parse();
+    }
+    catch (const std::runtime_error& e) {
+        return IOStatus(StatusCode::InvalidFileFormat, path + ": " + e.what());
+    }
+}
+
+inline IOResult<void> write_json(const std::string& path, const json::Value& v)
+{
+    std::ofstream f(path);
+    if (!f) {
+        return IOStatus(StatusCode::UnknownError, "cannot write " + path);
+    }
+    f << json::to_string(v) << '\n';
+    if (!f) {
+        return IOStatus(StatusCode::UnknownError, "cannot write " + path);
+    }
+    return IOResult<void>();
+}
+
+} // namespace jio
+} // namespace detail
+
+// mio::write_graph (mobility_io.h:70-122): GraphNode<i>.json for every node and GraphEdges_node<i>.json for
+// every node with out-edges; the directory is created if missing.
+template <class FP, class Model>
+IOResult<void> write_graph(const Graph<Model, MobilityParameters<FP>>& graph, const std::string& directory,
+                           int ioflags = IOF_None)
+{
+    using detail::jio::Value;
+    if (graph.nodes().empty()) {
+        return IOStatus(StatusCode::InvalidValue, "Graph Nodes are empty");
+    }
+    std::error_code ec;
+    std::filesystem::create_directories(directory, ec);
+    if (ec) {
+        return IOStatus(StatusCode::UnknownError, "cannot create " + directory + ": " + ec.message());
+    }
+    try {
+        for (size_t i = 0; i < graph.nodes().size(); ++i) {
+            const auto& node = graph.nodes()[i];
+            Value js         = Value::object();
+            js.set("NodeId", Value::integer(node.id));
+            js.set("Model", detail::jio::model(node.property, ioflags));
+            auto st = detail::jio::write_json(
+                (std::filesystem::path(directory) / ("GraphNode" + std::to_string(i) + ".json")).string(), js);
+            if (!st) {
+                return st;
+            }
+            Value edges = Value::array();
+            for (const auto& e : graph.edges()) {
+                if (e.start_node_idx != i) {
+                    continue;
+                }
+                Value je = Value::object();
+                je.set("StartNodeIndex", Value::integer((long long)e.start_node_idx));
+                je.set("EndNodeIndex", Value::integer((long long)e.end_node_idx));
+                je.set("Parameters", detail::jio::mobility_parameters(e.property, ioflags));
+                edges.append(std::move(je));
+            }
+            if (edges.size()) {
+                st = detail::jio::write_json(
+                    (std::filesystem::path(directory) / ("GraphEdges_node" + std::to_string(i) + ".json")).string(),
+                    edges);
+                if (!st) {
+                    return st;
+                }
+            }
+        }
+    }
+    catch (const std::runtime_error& e) {
+        return IOStatus(StatusCode::InvalidValue, e.what());
+    }
+    return IOResult<void>();
+}
+
+// mio::read_graph (mobility_io.h:124-192): nodes GraphNode0.json, GraphNode1.json, ... up to the first
+// missing index; with read_edges, GraphEdges_node<i>.json of every node (a missing one is an error, as in the
+// reference).
+template <class FP, class Model>
+IOResult<Graph<Model, MobilityParameters<FP>>> read_graph(const std::string& directory, int ioflags = IOF_None,
+                                                          bool read_edges = true)
+{
+    using detail::jio::Fail;
+    if (!std::filesystem::is_directory(directory)) {
+        return IOStatus(StatusCode::FileNotFound, "Directory " + directory + " does not exist.");
+    }
+    Graph<Model, MobilityParameters<FP>> graph;
+    try {
+        for (size_t i = 0;; ++i) {
+            const std::string path =
+                (std::filesystem::path(directory) / ("GraphNode" + std::to_string(i) + ".json")).string();
+            if (!std::filesystem::exists(path)) {
+                break;
+            }
+            auto js = detail::jio::read_json(path);
+            if (!js) {
+                return js.error();
+            }
+            const auto* id = js.value().kind() == json::Value::Object ? js.value().find("NodeId") : nullptr;
+            if (!id || !id->is_int()) {
+                return IOStatus(StatusCode::InvalidType, path + ", NodeId must be an integer.");
+            }
+            graph.add_node(int(id->as_int()),
+                           detail::jio::model_read<FP>(detail::jio::get(js.value(), "Model", path), ioflags, path));
+        }
+        if (read_edges) {
+            for (size_t i = 0; i < graph.nodes().size(); ++i) {
+                const std::string path =
+                    (std::filesystem::path(directory) / ("GraphEdges_node" + std::to_string(i) + ".json")).string();
+                auto js = detail::jio::read_json(path);
+                if (!js) {
+                    return js.error();
+                }
+                for (const auto& e : detail::jio::list(js.value(), path).items()) {
+                    const auto* end = e.kind() == json::Value::Object ? e.find("EndNodeIndex") : nullptr;
+                    if (!end || !end->is_int() || end->as_int() < 0) {
+                        return IOStatus(StatusCode::InvalidType, path + ", EndNodeIndex must be an integer.");
+                    }
+                    if (size_t(end->as_int()) >= graph.nodes().size()) {
+                        return IOStatus(StatusCode::OutOfRange,
+                                        path + ", EndNodeIndex not in range of number of graph nodes.");
+                    }
+                    graph.add_edge(i, size_t(end->as_int()),
+                                   detail::jio::mobility_parameters_read<FP>(detail::jio::get(e, "Parameters", path),
+                                                                             ioflags, path));
+                }
+            }
+        }
+    }
+    catch (const Fail& f) {
+        return IOStatus(f.code, f.msg);
+    }
+    return graph;
+}
+
+} // namespace mgo_mio
+
+#endif

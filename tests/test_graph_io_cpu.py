@@ -323,3 +323,59 @@ def test_model_deepcopy_keeps_dynamic_npis():
     g = osecir.ModelGraph()
     g.add_node(0, m)
     assert [t for t, _ in g.get_node(0).property.parameters.DynamicNPIsInfectedSymptoms.threshold] == [10.0]
+
+
+# ------------------------------------------------------------------------------ the C++ layer's files
+CPP_IO = os.path.join(os.path.dirname(__file__), "cpp", "test_mio_io")
+
+
+def _cpp(*args):
+    import subprocess
+    assert os.path.exists(CPP_IO), "tests/cpp/test_mio_io is built by __graft_entry__.build()"
+    r = subprocess.run([CPP_IO, *args], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and "ALL PASSED" in r.stdout, r.stdout + r.stderr
+
+
+def test_cpp_writes_python_reads(tmp_path):
+    """mio::write_graph of the C++ layer (the reference's ode_secir_read_graph.cpp set-up, plus dampings, a
+    dynamic NPI and a damped edge with minimum) read by osecir.read_graph."""
+    _cpp("example", str(tmp_path))
+    g = osecir.read_graph(str(tmp_path))
+    assert g.num_nodes == 3 and g.num_edges == 9
+    p = g.get_node(1).property.parameters
+    A1 = mio.AgeGroup(1)
+    assert p.Seasonality.value == 0.1 and p.TimeExposed[A1].value == 3.2 and p.DeathsPerCritical[A1].value == 0.3
+    cm = p.ContactPatterns.cont_freq_mat[0]
+    assert cm.baseline.tolist() == [[5.0, 5.0], [5.0, 5.0]] and cm.minimum.tolist() == [[0.1, 0.1], [0.1, 0.1]]
+    assert [(d.time, d.level, d.type) for d in cm.dampings] == [(12.5, 1, 2)]
+    (thr, ds), = p.DynamicNPIsInfectedSymptoms.threshold
+    assert thr == 200.0 and ds[0].value.value == 0.4
+    assert g.get_node(1).property.populations[mio.AgeGroup(0), osecir.InfectionState.Exposed].value == 50.0
+    e = g.get_edge(1).property.coefficients[0]
+    assert e.minimum.tolist() == [0.001] * 20 and [d.time for d in e.dampings] == [4.0]
+    assert g.get_edge(1).start_node_idx == 0 and g.get_edge(1).end_node_idx == 1
+
+
+def test_python_writes_cpp_round_trip(tmp_path):
+    """osecir.write_graph -> mio::read_graph + mio::write_graph -> osecir.read_graph: the same objects in the
+    files and the same device descriptor."""
+    graph = _feature_graph()
+    a, b = tmp_path / "py", tmp_path / "cpp"
+    osecir.write_graph(graph, str(a))
+    _cpp("roundtrip", str(a), str(b))
+    for name in sorted(os.listdir(a)):
+        assert json.loads((a / name).read_text()) == json.loads((b / name).read_text()), name
+    back = osecir.read_graph(str(b))
+    spec_a = osecir._spec_from([graph.get_node(v).property for v in range(4)],
+                               [(e.start_node_idx, e.end_node_idx, e.property) for e in graph._edges])
+    spec_b = osecir._spec_from([back.get_node(v).property for v in range(4)],
+                               [(e.start_node_idx, e.end_node_idx, e.property) for e in back._edges])
+    da, db = _descriptor_arrays(spec_a), _descriptor_arrays(spec_b)
+    for k in da:
+        assert (da[k] is None and db[k] is None) or np.array_equal(np.asarray(da[k]), np.asarray(db[k])), k
+
+
+def test_cpp_error_statuses(tmp_path):
+    """read_graph's statuses in the C++ layer: FileNotFound (directory, edge file), OutOfRange (EndNodeIndex),
+    KeyNotFound, InvalidType (NodeId), InvalidFileFormat (truncated JSON)."""
+    _cpp("errors", str(tmp_path))
