@@ -1913,9 +1913,14 @@ static int autotune_tile(mgo_context* ctx, const std::vector<RunSpec>& runs)
         trial.push_back(t);
         tmax.push_back(t.tmax);
     }
-    int best_tile    = 1;
-    double best_ms   = INFINITY;
-    const int cands[] = {1, 2, 4, 8, 16, 64};
+    int best_tile  = 1;
+    double best_ms = INFINITY;
+    // powers of two, and the wave-aligned tiles: 64 / G runs put exactly one graph node in each wavefront
+    // of the edge kernels (all lanes walk the same edge list), twice that pairs two such waves per tile
+    const int npw = WAVE / std::max(1, r0.g->num_groups);
+    std::vector<int> cands = {1, 2, 4, 8, 16, 64, npw, 2 * npw};
+    std::sort(cands.begin(), cands.end());
+    cands.erase(std::unique(cands.begin(), cands.end()), cands.end());
     for (int c : cands) {
         if (c > bucket) {
             break;
