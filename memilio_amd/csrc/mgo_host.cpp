@@ -1591,6 +1591,7 @@ struct mgo_context {
     int flags          = 0;
     std::recursive_mutex mu;
     std::string last_error;
+    uint64_t last_error_seq = 0; // order of this error among the context's and its serving loop's errors
     int64_t next_id = 1;
     std::unordered_map<int64_t, std::unique_ptr<Batch>> graphs;
     std::unordered_map<int64_t, std::unique_ptr<Batch>> batches;
@@ -1599,6 +1600,9 @@ struct mgo_context {
     int tile = 0;                                    // 0 = autotune
     std::map<std::tuple<int, int, int, int, int>, int> tile_cache; // shape -> autotuned tile
 };
+
+// numbers every recorded error so that mgo_last_error can return the newest one
+static std::atomic<uint64_t> g_error_seq{0};
 
 namespace
 {
@@ -1618,11 +1622,13 @@ mgo_status guarded(mgo_context* ctx, F&& f)
         return MGO_OK;
     }
     catch (const Error& e) {
-        ctx->last_error = e.msg;
+        ctx->last_error     = e.msg;
+        ctx->last_error_seq = ++g_error_seq;
         return e.code;
     }
     catch (const std::bad_alloc&) {
-        ctx->last_error = "host allocation failed";
+        ctx->last_error     = "host allocation failed";
+        ctx->last_error_seq = ++g_error_seq;
         return MGO_ERR_OUT_OF_MEMORY;
     }
 }
@@ -1726,6 +1732,13 @@ const char* mgo_last_error(const mgo_context* ctx)
         return tl_error.c_str();
     }
     tl_error = ctx->last_error;
+    if (ctx->serve) {
+        // the serving loop records its errors (worker, preparation, submissions) on its own; the newer wins
+        std::lock_guard<std::mutex> lk(ctx->serve->mu);
+        if (ctx->serve->error_seq > ctx->last_error_seq) {
+            tl_error = ctx->serve->error;
+        }
+    }
     return tl_error.c_str();
 }
 

@@ -158,6 +158,12 @@ def test_serve_errors():
         with pytest.raises(MgoError) as e:
             srv.update(tk + 1000, 0.5)
         assert e.value.status == capi.ERR_NOT_FOUND
+        bad = synthetic.single_model(seed=2, G=2)
+        bad.age_params[0, capi.A_TIME_EXPOSED, 0] = -1.0
+        with pytest.raises(MgoError) as e:
+            srv.submit(bad, synthetic.ck_cfg(1e-8), 5.0)
+        assert e.value.status == capi.ERR_INVALID_VALUE
+        assert "stay times must be > 0" in str(e.value), str(e.value)   # the serving loop's message
         srv.wait([tk])
         out = srv.take(tk)
         assert out.shape == (6, 1, 20)
