@@ -31,7 +31,7 @@
 
 namespace mgo
 {
-hipError_t launch_clock(InstDev* inst, int n, hipStream_t s);
+hipError_t launch_clock(InstDev* inst, int n, int32_t* cyc, hipStream_t s);
 hipError_t launch_scatter_slots(InstDev* table, const InstDev* src, const int32_t* ids, int n, hipStream_t s);
 hipError_t launch_set_segments(InstDev* table, const int32_t* ids, int n, hipStream_t s);
 hipError_t launch_node(const LaunchDev& c, InstDev* inst, hipStream_t s);
@@ -743,6 +743,7 @@ struct Batch {
     std::deque<ClassHost> classes;
     std::vector<InstHost> inst;
     DevBuf inst_dev;
+    DevBuf cyc; // [CYC_FLAGS] cycle flags (clock kernel, fixup_check), shared by the classes' launches
     int64_t cycles_run = 0;
     double last_ms     = 0;
     hipGraphExec_t graph_exec = nullptr;
@@ -812,6 +813,8 @@ struct Batch {
         for (auto& C : classes) {
             place_nodes(C, runs);
         }
+        const std::vector<int32_t> cyc_init(CYC_FLAGS, 0); // lives until the synchronisation below
+        cyc.upload(cyc_init, stream);
         std::vector<InstDev> idev(runs.size());
         for (size_t ci = 0; ci < classes.size(); ++ci) {
             build_class(ci, runs, idev);
@@ -1272,6 +1275,7 @@ struct Batch {
         ld.G           = G;
         ld.kind        = C.kind;
         ld.npi         = (C.npi.nodes0.empty() && C.oute.nodes0.empty()) ? 0 : 1;
+        ld.cyc         = cyc.as<int32_t>();
     }
 
     // launch tables of a cycle: one per class. (Batches keep per-G kernels so that a run's arithmetic is
@@ -1299,7 +1303,7 @@ struct Batch {
         std::vector<const LaunchDev*> node_ld, edge_ld;
         launch_tables(node_ld, edge_ld);
         InstDev* id = inst_dev.as<InstDev>();
-        hip_check(launch_clock(id, int(inst.size()), stream), "clock kernel");
+        hip_check(launch_clock(id, int(inst.size()), cyc.as<int32_t>(), stream), "clock kernel");
         if (ev) {
             hip_check(hipEventRecord(ev[0], stream), "event");
         }

@@ -184,6 +184,14 @@ constexpr int FX_SLOTS      = 1024; // pool slots per instance and return phase
 constexpr int FX_MAX_PASSES = 8;    // passes before falling back to the sequential replay
 constexpr int FX_EPOCH_STEP = 16;   // > FX_MAX_PASSES
 
+// Cycle flags, written by the clock kernel (cleared, then set by the instances) and by fixup_check: which
+// phases of this half-day cycle have work anywhere in the batch, so that the launches of idle phases (the
+// departure kernel in a return cycle, fix-up passes nobody needs) exit at once.
+constexpr int CYC_DEPART = 0;                    // some instance departs
+constexpr int CYC_RETURN = 1;                    // some instance returns
+constexpr int CYC_PASS0  = 2;                    // CYC_PASS0 + p: some instance runs fix-up pass p
+constexpr int CYC_FLAGS  = CYC_PASS0 + FX_MAX_PASSES;
+
 // CSR edge structure of one graph (node indices local to the instance). In-edges of node v are sorted
 // by source; in_split[v] = number of in-edges whose source < v, so walking
 //   in[0, split) , out[all] , in[split, end)
@@ -260,6 +268,7 @@ struct LaunchDev {
     int32_t kind;
     int32_t npi;                // some node has dynamic NPIs (per-G kernels only)
     int32_t pad;
+    int32_t* cyc;               // [CYC_FLAGS] cycle flags of the batch (or NULL: every phase runs its checks)
 };
 
 } // namespace mgo

@@ -433,13 +433,8 @@ __device__ __forceinline__ double mobility_factor(const Lane<G>& L, const double
 // ------------------------------------------------------------------------------- clock
 // One GraphSimulation::advance loop iteration per instance (graph_simulation.h:101-120) and the
 // departure/return alternation of MobilityEdge::apply_mobility (metapopulation_mobility_instant.h:544-623).
-__global__ void clock_kernel(InstDev* inst, int n)
+__device__ __forceinline__ void clock_one(InstDev& I)
 {
-    const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
-    if (i >= n) {
-        return;
-    }
-    InstDev& I  = inst[i];
     I.c_active  = 0;
     I.c_action  = 0;
     I.c_out_day = -1;
@@ -494,6 +489,28 @@ __global__ void clock_kernel(InstDev* inst, int n)
         const int d = int(dn) - I.day0;
         if (d >= 0 && d < I.ndays) {
             I.c_out_day = d;
+        }
+    }
+}
+
+// One block: the first threads clear the cycle flags, then every instance advances its clock and flags the
+// phases it needs (every writer stores 1).
+__global__ void __launch_bounds__(1024) clock_kernel(InstDev* inst, int n, int32_t* cyc)
+{
+    if (cyc) {
+        if (threadIdx.x < CYC_FLAGS) {
+            cyc[threadIdx.x] = 0;
+        }
+        __syncthreads();
+    }
+    for (int i = int(threadIdx.x); i < n; i += int(blockDim.x)) {
+        InstDev& I = inst[i];
+        clock_one(I);
+        if (cyc && I.c_action == 1) {
+            cyc[CYC_DEPART] = 1;
+        }
+        if (cyc && I.c_action == 2) {
+            cyc[CYC_RETURN] = 1;
         }
     }
 }
@@ -1083,6 +1100,9 @@ __device__ __forceinline__ bool fixup_negative(const double (&y)[NC], double (&m
 template <int G>
 __device__ __forceinline__ void depart_body(const LaunchDev& ld, InstDev* __restrict__ inst)
 {
+    if (ld.cyc && !ld.cyc[CYC_DEPART]) {
+        return; // nobody departs in this cycle
+    }
     int node, age, base;
     const SegDev* cp = lane_map<G>(ld, node, age, base);
     if (!cp) {
@@ -1186,8 +1206,11 @@ __device__ __forceinline__ int32_t record_slot(const FxRec* r, int32_t lo, int32
 // reproduces the sequential result exactly. A node whose out-edge records did not change in pass p-1
 // would repeat its walk of pass p-1 unchanged, so pass p only re-walks the nodes marked dirty then.
 template <int G, bool FIRST>
-__device__ __forceinline__ void return_body(const LaunchDev& ld, InstDev* __restrict__ inst)
+__device__ __forceinline__ void return_body(const LaunchDev& ld, InstDev* __restrict__ inst, int launch_pass)
 {
+    if (ld.cyc && !ld.cyc[FIRST ? CYC_RETURN : CYC_PASS0 + launch_pass]) {
+        return; // nobody returns in this cycle, or no instance needs this fix-up pass
+    }
     int node, age, base;
     const SegDev* cp = lane_map<G>(ld, node, age, base);
     if (!cp) {
@@ -1351,6 +1374,9 @@ __global__ void fixup_check(LaunchDev ld, InstDev* inst)
     I.fx_pass    = I.fx_pass + 1;
     I.fx_changed = 0;
     I.fx_passes += 1;
+    if (ld.cyc) {
+        ld.cyc[CYC_PASS0 + I.fx_pass] = 1; // fx_pass < FX_MAX_PASSES here
+    }
 }
 
 // ------------------------------------------------------------------------------- exact replay
@@ -1476,15 +1502,16 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
 
 template <int G, bool FIRST>
 __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) return_phase(LaunchDev ld,
-                                                                                       InstDev* __restrict__ inst)
+                                                                                       InstDev* __restrict__ inst,
+                                                                                       int pass)
 {
-    return_body<G, FIRST>(ld, inst);
+    return_body<G, FIRST>(ld, inst, pass);
 }
 
 template <bool FIRST>
-__global__ void __launch_bounds__(BLOCK) return_phase_mixed(LaunchDev ld, InstDev* __restrict__ inst)
+__global__ void __launch_bounds__(BLOCK) return_phase_mixed(LaunchDev ld, InstDev* __restrict__ inst, int pass)
 {
-    MGO_DISPATCH_G(block_groups(ld), (return_body<GG, FIRST>(ld, inst)))
+    MGO_DISPATCH_G(block_groups(ld), (return_body<GG, FIRST>(ld, inst, pass)))
 }
 
 template <int G>
@@ -1538,12 +1565,12 @@ static hipError_t launch_edge_g(const LaunchDev& c, InstDev* inst, hipStream_t s
 {
     const int nb = c.num_blocks;
     hipLaunchKernelGGL((depart_phase<G>), dim3(nb), dim3(BLOCK), 0, s, c, inst);
-    hipLaunchKernelGGL((return_phase<G, true>), dim3(nb), dim3(BLOCK), 0, s, c, inst);
+    hipLaunchKernelGGL((return_phase<G, true>), dim3(nb), dim3(BLOCK), 0, s, c, inst, 0);
     hipError_t err = hipGetLastError();
     const int ni   = (c.num_inst + 255) / 256;
     for (int p = 1; p < FX_MAX_PASSES && err == hipSuccess; ++p) {
         hipLaunchKernelGGL(fixup_check, dim3(ni), dim3(256), 0, s, c, inst);
-        hipLaunchKernelGGL((return_phase<G, false>), dim3(nb), dim3(BLOCK), 0, s, c, inst);
+        hipLaunchKernelGGL((return_phase<G, false>), dim3(nb), dim3(BLOCK), 0, s, c, inst, p);
         err = hipGetLastError();
     }
     if (err != hipSuccess) {
@@ -1556,9 +1583,9 @@ static hipError_t launch_edge_g(const LaunchDev& c, InstDev* inst, hipStream_t s
     return hipGetLastError();
 }
 
-hipError_t launch_clock(InstDev* inst, int n, hipStream_t s)
+hipError_t launch_clock(InstDev* inst, int n, int32_t* cyc, hipStream_t s)
 {
-    hipLaunchKernelGGL(clock_kernel, dim3((n + 255) / 256), dim3(256), 0, s, inst, n);
+    hipLaunchKernelGGL(clock_kernel, dim3(1), dim3(1024), 0, s, inst, n, cyc);
     return hipGetLastError();
 }
 
@@ -1631,12 +1658,12 @@ static hipError_t launch_edge_mixed(const LaunchDev& c, InstDev* inst, hipStream
 {
     const int nb = c.num_blocks;
     hipLaunchKernelGGL(depart_phase_mixed, dim3(nb), dim3(BLOCK), 0, s, c, inst);
-    hipLaunchKernelGGL((return_phase_mixed<true>), dim3(nb), dim3(BLOCK), 0, s, c, inst);
+    hipLaunchKernelGGL((return_phase_mixed<true>), dim3(nb), dim3(BLOCK), 0, s, c, inst, 0);
     hipError_t err = hipGetLastError();
     const int ni   = (c.num_inst + 255) / 256;
     for (int p = 1; p < FX_MAX_PASSES && err == hipSuccess; ++p) {
         hipLaunchKernelGGL(fixup_check, dim3(ni), dim3(256), 0, s, c, inst);
-        hipLaunchKernelGGL((return_phase_mixed<false>), dim3(nb), dim3(BLOCK), 0, s, c, inst);
+        hipLaunchKernelGGL((return_phase_mixed<false>), dim3(nb), dim3(BLOCK), 0, s, c, inst, p);
         err = hipGetLastError();
     }
     if (err != hipSuccess) {
