@@ -813,13 +813,22 @@ struct Batch {
         for (auto& C : classes) {
             place_nodes(C, runs);
         }
-        const std::vector<int32_t> cyc_init(CYC_FLAGS, 0); // lives until the synchronisation below
-        cyc.upload(cyc_init, stream);
+        // sources of the asynchronous uploads below; they outlive the handler
+        const std::vector<int32_t> cyc_init(CYC_FLAGS, 0);
         std::vector<InstDev> idev(runs.size());
-        for (size_t ci = 0; ci < classes.size(); ++ci) {
-            build_class(ci, runs, idev);
+        try {
+            cyc.upload(cyc_init, stream);
+            for (size_t ci = 0; ci < classes.size(); ++ci) {
+                build_class(ci, runs, idev);
+            }
+            inst_dev.upload(idev, stream);
         }
-        inst_dev.upload(idev, stream);
+        catch (...) {
+            // an allocation failed part-way (a full serving arena): the uploads already queued must land
+            // before the caller frees this batch, or they would write into memory handed to someone else
+            (void)hipStreamSynchronize(stream);
+            throw;
+        }
         device_bytes += inst_dev.bytes;
         hip_check(hipStreamSynchronize(stream), "batch upload");
         staging.clear();

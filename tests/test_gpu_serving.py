@@ -120,6 +120,29 @@ def test_serve_queueing_and_budget(oracle):
         assert rel_err(out, _oracle_daily(oracle, spec, cfg, tmax)) < tol
 
 
+def test_serve_full_arena_churn(oracle):
+    """A 12 MB serving arena that holds only a few requests at a time: admissions keep failing for memory and
+    waiting for retirements while others run (a failed preparation waits for its queued uploads before its
+    memory goes back to the arena). Every request finishes, equal to the oracle."""
+    rng = np.random.default_rng(5)
+    reqs = []
+    for k in range(24):
+        n, G = int(rng.integers(60, 220)), int(rng.integers(2, 7))
+        reqs.append((synthetic.graph(num_nodes=n, G=G, seed=300 + k, mean_degree=6.0), synthetic.ck_cfg(1e-8),
+                     float(rng.integers(10, 25))))
+    with Context() as ctx:
+        srv = Server(ctx, hbm_budget_bytes=12 << 20)
+        tickets = [srv.submit(*r) for r in reqs]
+        infos = srv.wait(tickets, timeout_s=120.0)
+        outs = [srv.take(tk) for tk in tickets]
+        st = srv.stats()
+        srv.stop()
+    assert all(infos[tk].status == capi.OK for tk in tickets)
+    assert st.max_resident_seen < len(reqs)          # the arena, not the slots, limited admission
+    for (spec, cfg, tmax), out in zip(reqs, outs):
+        assert rel_err(out, _oracle_daily(oracle, spec, cfg, tmax)) < 1e-6
+
+
 def test_serve_damping_update(oracle):
     """Two scenario updates queued before admission (the request waits behind a long one with
     max_resident = 1) equal the oracle with the same dampings on every node from the start."""
