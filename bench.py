@@ -272,7 +272,7 @@ def main():
                     help="launch every kernel directly everywhere (no hipGraph capture; implies --no-graph)")
     ap.add_argument("--uneven", type=int, default=4096, help="graphs in the uneven batch (0 = skip)")
     ap.add_argument("--uneven-steps", type=int, default=2)
-    ap.add_argument("--serve-rates", type=float, nargs="*", default=[50.0, 200.0, 800.0])
+    ap.add_argument("--serve-rates", type=float, nargs="*", default=[50.0, 200.0, 800.0, 1600.0])
     ap.add_argument("--serve-seconds", type=float, default=4.0)
     ap.add_argument("--serve-pool", type=int, default=384)
     ap.add_argument("--serve-batch-kernels", action="store_true",

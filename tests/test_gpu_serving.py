@@ -138,7 +138,7 @@ def test_serve_full_arena_churn(oracle):
         st = srv.stats()
         srv.stop()
     assert all(infos[tk].status == capi.OK for tk in tickets)
-    assert st.max_resident_seen < len(reqs)          # the arena, not the slots, limited admission
+    assert st["max_resident_seen"] < len(reqs)       # the arena, not the slots, limited admission
     for (spec, cfg, tmax), out in zip(reqs, outs):
         assert rel_err(out, _oracle_daily(oracle, spec, cfg, tmax)) < 1e-6
 
