@@ -748,6 +748,7 @@ struct Batch {
     double last_ms     = 0;
     hipGraphExec_t graph_exec = nullptr;
     bool use_graph            = true;
+    bool concurrent_classes   = true; // classes of a cycle on their own streams (off: one stream)
     bool profile              = false;
     bool sequential_fixup     = false;
     size_t device_bytes       = 0;
@@ -757,10 +758,40 @@ struct Batch {
     double node_ms = 0, edge_ms = 0;
     int node_launches = 0, edge_launches = 0;
 
+    // classes of a cycle run concurrently on these streams (one per class, default priority), forked from
+    // and joined back into `stream` (also inside the captured cycle graph)
+    std::vector<hipStream_t> side;
+    std::vector<hipEvent_t> side_done;
+    hipEvent_t fork_ev = nullptr;
+
     ~Batch()
     {
         if (graph_exec) {
             (void)hipGraphExecDestroy(graph_exec);
+        }
+        for (auto e : side_done) {
+            (void)hipEventDestroy(e);
+        }
+        for (auto st : side) {
+            (void)hipStreamDestroy(st);
+        }
+        if (fork_ev) {
+            (void)hipEventDestroy(fork_ev);
+        }
+    }
+
+    void make_side_streams(size_t n)
+    {
+        if (!fork_ev) {
+            hip_check(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming), "fork event");
+        }
+        while (side.size() < n) {
+            hipStream_t st = nullptr;
+            hip_check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "class stream");
+            side.push_back(st);
+            hipEvent_t e = nullptr;
+            hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "class event");
+            side_done.push_back(e);
         }
     }
 
@@ -1313,6 +1344,21 @@ struct Batch {
         launch_tables(node_ld, edge_ld);
         InstDev* id = inst_dev.as<InstDev>();
         hip_check(launch_clock(id, int(inst.size()), cyc.as<int32_t>(), stream), "clock kernel");
+        if (!ev && concurrent_classes && node_ld.size() > 1) {
+            // classes share no data: each runs its node and edge kernels on its own stream
+            make_side_streams(node_ld.size());
+            hip_check(hipEventRecord(fork_ev, stream), "fork");
+            for (size_t k = 0; k < node_ld.size(); ++k) {
+                hip_check(hipStreamWaitEvent(side[k], fork_ev, 0), "fork wait");
+                hip_check(launch_node(*node_ld[k], id, side[k]), "node kernel");
+                if (std::find(edge_ld.begin(), edge_ld.end(), node_ld[k]) != edge_ld.end()) {
+                    hip_check(launch_edge(*node_ld[k], id, side[k]), "edge kernel");
+                }
+                hip_check(hipEventRecord(side_done[k], side[k]), "join");
+                hip_check(hipStreamWaitEvent(stream, side_done[k], 0), "join wait");
+            }
+            return;
+        }
         if (ev) {
             hip_check(hipEventRecord(ev[0], stream), "event");
         }
