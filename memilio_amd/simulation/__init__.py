@@ -11,6 +11,7 @@ A script written against the reference switches with
     import memilio_amd.simulation as mio
     import memilio_amd.simulation.osecir as osecir
 """
+import copy
 import enum
 
 import numpy as np
@@ -31,9 +32,19 @@ def set_log_level(level):
     return None
 
 
-def seed_random_number_generator(*args):
-    """Accepted for call compatibility; parameter sampling stays with the caller."""
-    return None
+# random numbers of the parameter distributions (draw_sample); numpy's PCG64, not the reference's Random123
+# counter streams: samples have the reference's distributions, not its values
+_rng = np.random.default_rng()
+
+
+def seed_random_number_generator(*seeds):
+    """Seeds the generator that parameter distributions draw from (mio::thread_local_rng().seed)."""
+    global _rng
+    _rng = np.random.default_rng([int(x) for x in seeds] if seeds else None)
+
+
+def rng():
+    return _rng
 
 
 class AgeGroup:
@@ -58,11 +69,184 @@ class AgeGroup:
         return f"AgeGroup({self.i})"
 
 
-class UncertainValue:
-    """A parameter value (mio::UncertainValue without a distribution); `.value` reads and writes it."""
+class ParameterDistribution:
+    """mio::ParameterDistribution (utils/parameter_distributions.h:72-130): predefined samples are returned
+    first, in order; then random samples."""
 
-    def __init__(self, getter, setter):
+    type_name = None
+
+    def __init__(self):
+        self._predefined = []
+
+    def add_predefined_sample(self, v):
+        self._predefined.append(float(v))
+
+    def remove_predefined_samples(self):
+        self._predefined = []
+
+    def get_predefined_samples(self):
+        return list(self._predefined)
+
+    def get_sample(self):
+        if self._predefined:
+            return self._predefined.pop(0)
+        return self.get_rand_sample(_rng)
+
+    def _json_fields(self):
+        raise NotImplementedError
+
+    def to_json(self):
+        """{"Type": ..., fields, "PredefinedSamples": [...]} (parameter_distributions.h:363-432, 505-559, 867-917)."""
+        js = {"Type": self.type_name}
+        js.update(self._json_fields())
+        js["PredefinedSamples"] = list(self._predefined)
+        return js
+
+
+class ParameterDistributionNormal(ParameterDistribution):
+    """mio::ParameterDistributionNormal (parameter_distributions.h:159-427), constructed as the binding allows
+    (utils/parameter_distributions.cpp:40-52): (lb, ub, mean, std_dev, quantile), (lb, ub, mean, quantile)
+    or (mean, std_dev) with bounds [DBL_MIN, DBL_MAX]. A sample is redrawn up to 10 times while outside the
+    bounds, then set to the bound it missed (to the lower one when the tenth redraw is inside: the
+    reference's loop does so); before drawing, a mean outside the bounds becomes (ub - lb) / 2 and the
+    standard deviation shrinks until mean +- quantile * std fits the bounds."""
+
+    type_name = "Normal"
+
+    def __init__(self, *args, **kw):
+        super().__init__()
+        if kw:
+            # keyword forms of the binding: (lb, ub, mean, std_dev, quantile), (lb, ub, mean, quantile),
+            # (mean, std_dev)
+            order = {5: ("lb", "ub", "mean", "std_dev", "quantile"), 4: ("lb", "ub", "mean", "quantile"),
+                     2: ("mean", "std_dev")}
+            n = len(args) + len(kw)
+            if n not in order:
+                raise TypeError("ParameterDistributionNormal takes 2, 4 or 5 arguments")
+            vals = dict(zip(order[n], args))
+            vals.update(kw)
+            args = tuple(vals[k] for k in order[n])
+        self.lower_bound, self.upper_bound = np.finfo(np.float64).tiny, np.finfo(np.float64).max
+        self.quantile = 2.5758
+        if len(args) == 2:
+            self.mean, self.standard_dev = float(args[0]), float(args[1])
+        elif len(args) == 4:
+            self.lower_bound, self.upper_bound, self.mean, self.quantile = map(float, args)
+            self.standard_dev = self.upper_bound      # too high, then adapted
+            self._adapt_standard_dev()
+        elif len(args) == 5:
+            self.lower_bound, self.upper_bound, self.mean, self.standard_dev, self.quantile = map(float, args)
+            self._check_quantiles()
+        else:
+            raise TypeError("ParameterDistributionNormal(lb, ub, mean, std_dev, quantile), (lb, ub, mean, "
+                            "quantile) or (mean, std_dev)")
+
+    def _adapt_standard_dev(self):
+        changed = False
+        if self.mean + self.standard_dev * self.quantile > self.upper_bound:
+            self.standard_dev = (self.upper_bound - self.mean) / self.quantile
+            changed = True
+        if self.mean - self.standard_dev * self.quantile < self.lower_bound:
+            self.standard_dev = (self.mean - self.lower_bound) / self.quantile
+            changed = True
+        return changed
+
+    def _check_quantiles(self):
+        changed = False
+        if self.mean < self.lower_bound or self.mean > self.upper_bound:
+            self.mean = 0.5 * (self.upper_bound - self.lower_bound)
+            changed = True
+        return self._adapt_standard_dev() or changed
+
+    def get_rand_sample(self, gen):
+        if self.upper_bound == self.lower_bound:
+            return self.lower_bound
+        self._check_quantiles()
+        r = gen.normal(self.mean, self.standard_dev)
+        i = 0
+        while (r > self.upper_bound or r < self.lower_bound) and i < 10:
+            r = gen.normal(self.mean, self.standard_dev)
+            i += 1
+            if i == 10:
+                r = self.upper_bound if r > self.upper_bound else self.lower_bound
+        return float(r)
+
+    def _json_fields(self):
+        return {"Mean": self.mean, "StandardDev": self.standard_dev, "LowerBound": self.lower_bound,
+                "UpperBound": self.upper_bound, "Quantile": self.quantile}
+
+
+class ParameterDistributionUniform(ParameterDistribution):
+    """mio::ParameterDistributionUniform (parameter_distributions.h:440-560)."""
+
+    type_name = "Uniform"
+
+    def __init__(self, lb, ub):
+        super().__init__()
+        self.lower_bound, self.upper_bound = float(lb), float(ub)
+
+    def get_rand_sample(self, gen):
+        return float(gen.uniform(self.lower_bound, self.upper_bound))
+
+    def _json_fields(self):
+        return {"LowerBound": self.lower_bound, "UpperBound": self.upper_bound}
+
+
+class ParameterDistributionConstant(ParameterDistribution):
+    """mio::ParameterDistributionConstant (parameter_distributions.h:810-920)."""
+
+    type_name = "Constant"
+
+    def __init__(self, constant):
+        super().__init__()
+        self.constant = float(constant)
+
+    def get_rand_sample(self, gen):
+        return self.constant
+
+    def _json_fields(self):
+        return {"Constant": self.constant}
+
+
+def distribution_from_json(js):
+    """A distribution object from its JSON, or the JSON itself for the types not sampled here (LogNormal,
+    Exponential): kept so that files round-trip."""
+    if not isinstance(js, dict):
+        return js
+    t = js.get("Type")
+    if t == "Normal":
+        d = ParameterDistributionNormal.__new__(ParameterDistributionNormal)
+        ParameterDistribution.__init__(d)
+        d.mean, d.standard_dev = float(js["Mean"]), float(js["StandardDev"])
+        d.lower_bound, d.upper_bound = float(js["LowerBound"]), float(js["UpperBound"])
+        d.quantile = float(js["Quantile"])
+    elif t == "Uniform":
+        d = ParameterDistributionUniform(js["LowerBound"], js["UpperBound"])
+    elif t == "Constant":
+        d = ParameterDistributionConstant(js["Constant"])
+    else:
+        return js
+    for v in js.get("PredefinedSamples", []):
+        d.add_predefined_sample(v)
+    return d
+
+
+def distribution_to_json(d):
+    return d.to_json() if isinstance(d, ParameterDistribution) else d
+
+
+class UncertainValue:
+    """mio::UncertainValue: `.value` reads and writes the value; set_distribution / get_distribution /
+    draw_sample as the reference (utils/uncertain_value.h:150-188; the distribution is copied). Built on a
+    value (UncertainValue(0.5)) or as a view of a value held elsewhere (a parameter array, populations)."""
+
+    def __init__(self, getter=0.0, setter=None, dist_get=None, dist_set=None):
+        if not callable(getter):
+            box = [float(getter), None]
+            getter, setter = (lambda: box[0]), (lambda v: box.__setitem__(0, float(v)))
+            dist_get, dist_set = (lambda: box[1]), (lambda d: box.__setitem__(1, d))
         self._get, self._set = getter, setter
+        self._dist_get, self._dist_set = dist_get, dist_set
 
     @property
     def value(self):
@@ -72,11 +256,32 @@ class UncertainValue:
     def value(self, v):
         self._set(float(v))
 
+    def set_distribution(self, dist):
+        if self._dist_set is None:
+            raise TypeError("this value takes no distribution")
+        self._dist_set(copy.deepcopy(dist))
+
+    def get_distribution(self):
+        return self._dist_get() if self._dist_get is not None else None
+
+    def draw_sample(self):
+        d = self.get_distribution()
+        if isinstance(d, ParameterDistribution):
+            self._set(d.get_sample())
+        return self._get()
+
     def __float__(self):
         return float(self._get())
 
     def __repr__(self):
         return f"UncertainValue({self._get()})"
+
+    def __deepcopy__(self, memo):
+        u = UncertainValue(self._get())
+        d = self.get_distribution()
+        if d is not None:
+            u.set_distribution(d)
+        return u
 
 
 class Damping:
@@ -155,7 +360,7 @@ class ContactMatrixGroup:
 class UncertainContactMatrix:
     """mio::UncertainContactMatrix (epidemiology/uncertain_matrix.h:40-60): `cont_freq_mat` is the
     ContactMatrixGroup; `dampings`, `school_holiday_damping` and `school_holidays` are the parameter-study
-    sampling inputs the reference keeps beside it (kept and written to files, not sampled here)."""
+    sampling inputs kept beside it (draw_sample, make_matrix)."""
 
     def __init__(self, group):
         self.cont_freq_mat = group
@@ -164,27 +369,73 @@ class UncertainContactMatrix:
         self.school_holiday_damping = DampingSampling(0.0, 0, 0, 0.0, [], np.zeros(G))
         self.school_holidays = []
 
+    def draw_sample_dampings(self):
+        for d in self.dampings:
+            d.draw_sample()
+        self.school_holiday_damping.draw_sample()
+
+    def make_matrix(self, accum=False):
+        """The contact matrices with the sampled dampings (uncertain_matrix.h:172-198): without accum the
+        matrices' dampings are replaced; each school holiday starts the holiday damping at its first day
+        and sets it to 0 at its last."""
+        if not accum:
+            self.cont_freq_mat.clear_dampings()
+        apply_dampings(self.cont_freq_mat, self.dampings, make_contact_damping_matrix)
+        for a, b in self.school_holidays:
+            d = copy.deepcopy(self.school_holiday_damping)
+            d.time = float(a)
+            apply_dampings(self.cont_freq_mat, [d], make_contact_damping_matrix)
+            d.value.value = 0.0
+            d.time = float(b)
+            apply_dampings(self.cont_freq_mat, [d], make_contact_damping_matrix)
+        return self.cont_freq_mat
+
+    def draw_sample(self, accum=False):
+        self.draw_sample_dampings()
+        return self.make_matrix(accum)
+
 
 class DampingSampling:
     """mio::DampingSampling (epidemiology/damping_sampling.h:40-230; binding damping_sampling.cpp:33-79): a
-    damping of value * group_weights that a dynamic NPI adds to the listed contact matrices."""
+    damping of value * group_weights that a dynamic NPI (or a parameter study) adds to the listed contact
+    matrices; the value may carry a distribution (draw_sample)."""
 
     def __init__(self, value, level, type, time, matrix_indices, group_weights):
-        self.value = value if isinstance(value, UncertainValue) else _FixedValue(float(value))
+        self.value = copy.deepcopy(value) if isinstance(value, UncertainValue) else UncertainValue(float(value))
         self.level = int(level)
         self.type = int(type)
         self.time = float(time)
         self.matrix_indices = [int(k) for k in matrix_indices]
         self.group_weights = np.asarray(group_weights, dtype=np.float64).ravel().copy()
 
+    def draw_sample(self):
+        self.value.draw_sample()
 
-class _FixedValue(UncertainValue):
-    def __init__(self, v):
-        box = [v]
-        super().__init__(lambda: box[0], lambda x: box.__setitem__(0, float(x)))
 
-    def __deepcopy__(self, memo):
-        return _FixedValue(self.value)
+def make_contact_damping_matrix(groups):
+    """d_ij = 1 - sqrt((1 - g_i)(1 - g_j)) (epidemiology/damping_sampling.h:268-290)."""
+    g = np.asarray(groups, dtype=np.float64).ravel()
+    return 1.0 - np.sqrt(np.outer(1.0 - g, 1.0 - g))
+
+
+def make_mobility_damping_vector(size, groups):
+    """Each group's value on every compartment of the group (damping_sampling.h:298-306)."""
+    g = np.asarray(groups, dtype=np.float64).ravel()
+    per = int(size) // g.size
+    return np.repeat(g, per)
+
+
+def apply_dampings(group, dampings, make_matrix):
+    """Every damping sampling adds make_matrix(value * weights) to each matrix it lists
+    (damping_sampling.h:255-266); `group` is a ContactMatrixGroup or a MobilityCoefficientGroup."""
+    for d in dampings:
+        for i in d.matrix_indices:
+            m = make_matrix(d.value.value * d.group_weights)
+            target = group[i]
+            if isinstance(target, ContactMatrix):
+                target.add_damping(Damping(m, t=d.time, level=d.level, type=d.type))
+            else:
+                target.add_damping(MobilityDamping(m, t=d.time, level=d.level, type=d.type))
 
 
 class DynamicNPIs:
@@ -212,6 +463,12 @@ class DynamicNPIs:
     @property
     def threshold(self):
         return list(self._thresholds)
+
+    def draw_sample(self):
+        """Samples the value of every damping of every threshold (dynamic_npis.h:192-199)."""
+        for _, ds in self._thresholds:
+            for d in ds:
+                d.draw_sample()
 
 
 class MobilityDamping:

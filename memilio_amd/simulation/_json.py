@@ -23,8 +23,9 @@ reference's serializers, JSON flavour (cpp/memilio/io/json_serializer.h, io.h:58
                                                                       models/ode_secir/model.h:219-225
   MobilityParameters    {"Coefficients": group, "DynamicNPIs": npis}  metapopulation_mobility_instant.h:259-264
 
-Distributions of uncertain values are not sampled here; they are kept as the JSON they were read from and
-written back unchanged. Reading is as strict as the reference's: a missing element is an error.
+Distributions of uncertain values become ParameterDistributionNormal / Uniform / Constant objects (the types
+draw_sample uses); other types (LogNormal, Exponential) are kept as the JSON they were read from and written
+back unchanged. Reading is as strict as the reference's: a missing element is an error.
 """
 import json
 import os
@@ -62,22 +63,24 @@ def _integer(v, where):
 
 
 def uv_to_json(value, flags=IOF_None, distribution=None):
+    from . import distribution_to_json
     js = {}
     if not flags & IOF_OmitValues:
         js["Value"] = float(value)
     if not flags & IOF_OmitDistributions and distribution is not None:
-        js["Distribution"] = distribution
+        js["Distribution"] = distribution_to_json(distribution)
     return js
 
 
 def uv_from_json(js, flags=IOF_None, where="UncertainValue"):
-    """(value, distribution JSON or None) (uncertain_value.h:211-250)."""
+    """(value, distribution or None) (uncertain_value.h:211-250)."""
     if not isinstance(js, dict):
         raise JsonFormatError(f"{where}: object expected")
+    from . import distribution_from_json
     v = 0.0 if flags & IOF_OmitValues else _number(_expect(js, "Value", where), where)
     dist = None
-    if not flags & (IOF_OmitValues | IOF_OmitDistributions):
-        dist = js.get("Distribution")
+    if not flags & (IOF_OmitValues | IOF_OmitDistributions) and js.get("Distribution") is not None:
+        dist = distribution_from_json(js["Distribution"])
     return v, dist
 
 
@@ -164,7 +167,7 @@ def damping_from_json(js, where="Damping"):
 
 def damping_sampling_to_json(ds, flags=IOF_None):
     return {"Time": float(ds.time), "Type": int(ds.type), "Level": int(ds.level),
-            "Value": uv_to_json(ds.value.value, flags, getattr(ds, "value_distribution", None)),
+            "Value": uv_to_json(ds.value.value, flags, ds.value.get_distribution()),
             "MatrixIndices": [int(k) for k in ds.matrix_indices],
             "GroupWeights": matrix_to_json(np.asarray(ds.group_weights, dtype=np.float64).ravel())}
 
@@ -179,7 +182,7 @@ def damping_sampling_from_json(js, flags=IOF_None, where="DampingSampling"):
                          _number(_expect(js, "Time", where), where), [_integer(k, where) for k in idx],
                          vector_from_json(_expect(js, "GroupWeights", where), where))
     if dist is not None:
-        ds.value_distribution = dist
+        ds.value._dist_set(dist)
     return ds
 
 

@@ -21,12 +21,14 @@ import numpy as np
 from .. import capi
 from ..engine import Context
 from ..graphspec import GraphSpec, NC
-from . import (AgeGroup, ContactMatrix, ContactMatrixGroup, DynamicNPIs, MobilityParameters, TimeSeries,
-               UncertainContactMatrix, UncertainValue, interpolate_simulation_result)
+from . import (AgeGroup, ContactMatrix, ContactMatrixGroup, DampingSampling, DynamicNPIs, MobilityParameters,
+               ParameterDistributionConstant, ParameterDistributionNormal, TimeSeries, UncertainContactMatrix,
+               UncertainValue, apply_dampings, interpolate_simulation_result, make_mobility_damping_vector)
 
 __all__ = ["InfectionState", "Model", "ModelGraph", "MobilityGraph", "MobilitySimulation", "GraphParameterStudy",
-           "Simulation", "simulate", "interpolate_simulation_result", "ControlledStepperWrapper",
-           "RKIntegratorCore", "ExplicitStepperWrapper", "EulerIntegratorCore", "write_graph", "read_graph"]
+           "ParameterStudy", "Simulation", "simulate", "interpolate_simulation_result", "ControlledStepperWrapper",
+           "RKIntegratorCore", "ExplicitStepperWrapper", "EulerIntegratorCore", "write_graph", "read_graph",
+           "draw_sample", "set_params_distributions_normal"]
 
 
 class InfectionState(enum.IntEnum):
@@ -110,14 +112,17 @@ def _context():
 
 # ----------------------------------------------------------------------------------------- model
 class _AgeParameter:
-    """Per-age-group values of one parameter: p[AgeGroup(i)] = v, p[AgeGroup(i)].value."""
+    """Per-age-group values of one parameter: p[AgeGroup(i)] = v, p[AgeGroup(i)].value, and the groups'
+    distributions (p[AgeGroup(i)].set_distribution(...))."""
 
-    def __init__(self, arr):
+    def __init__(self, arr, dists):
         self._a = arr
+        self._d = dists   # {group: distribution}
 
     def __getitem__(self, g):
         i = int(g.get() if isinstance(g, AgeGroup) else g)
-        return UncertainValue(lambda: float(self._a[i]), lambda v: self._a.__setitem__(i, v))
+        return UncertainValue(lambda: float(self._a[i]), lambda v: self._a.__setitem__(i, v),
+                              lambda: self._d.get(i), lambda d: self._d.__setitem__(i, d))
 
     def __setitem__(self, g, v):
         i = int(g.get() if isinstance(g, AgeGroup) else g)
@@ -141,14 +146,17 @@ class Parameters:
         object.__setattr__(self, "ContactPatterns",
                            UncertainContactMatrix(ContactMatrixGroup(1, self._G)))
         object.__setattr__(self, "DynamicNPIsInfectedSymptoms", DynamicNPIs())
-        object.__setattr__(self, "_dist", {})   # distributions read from files, written back unchanged
+        object.__setattr__(self, "_dist", {})   # distributions: name -> dist (scalars), name -> {group: dist}
 
     def __getattr__(self, name):
         if name in _SCALARS:
             k = _SCALARS[name]
-            return UncertainValue(lambda: float(self._sc[k]), lambda v: self._sc.__setitem__(k, v))
+            if name == "StartDay":   # a plain number in the reference (parameters.h:47-62)
+                return UncertainValue(lambda: float(self._sc[k]), lambda v: self._sc.__setitem__(k, v))
+            return UncertainValue(lambda: float(self._sc[k]), lambda v: self._sc.__setitem__(k, v),
+                                  lambda: self._dist.get(name), lambda d: self._dist.__setitem__(name, d))
         if name in _AGES:
-            return _AgeParameter(self._age[_AGES[name]])
+            return _AgeParameter(self._age[_AGES[name]], self._dist.setdefault(name, {}))
         raise AttributeError(f"osecir parameter {name!r} is not on the device path")
 
     def __setattr__(self, name, v):
@@ -239,7 +247,7 @@ class Populations:
     def __init__(self, num_groups):
         self._G = int(num_groups)
         self._v = np.zeros((self._G, NC))
-        self._dist = {}   # distributions read from files, by flat index
+        self._dist = {}   # distributions of the compartments (draw_sample, graph files), by flat index
 
     @staticmethod
     def _idx(key):
@@ -248,7 +256,9 @@ class Populations:
 
     def __getitem__(self, key):
         g, s = self._idx(key)
-        return UncertainValue(lambda: float(self._v[g, s]), lambda v: self._v.__setitem__((g, s), v))
+        f = g * NC + s
+        return UncertainValue(lambda: float(self._v[g, s]), lambda v: self._v.__setitem__((g, s), v),
+                              lambda: self._dist.get(f), lambda d: self._dist.__setitem__(f, d))
 
     def __setitem__(self, key, v):
         g, s = self._idx(key)
@@ -292,6 +302,12 @@ class Model:
 
     def apply_constraints(self):
         return self.parameters.apply_constraints()
+
+    def __deepcopy__(self, memo):
+        m = Model.__new__(Model)
+        m.parameters = copy.deepcopy(self.parameters, memo)
+        m.populations = copy.deepcopy(self.populations, memo)
+        return m
 
     def check_constraints(self):
         test = copy.deepcopy(self)
@@ -565,10 +581,171 @@ def simulate(t0, tmax, dt, model, integrator=None):
     return Simulation(model, t0, dt, integrator).advance(tmax)
 
 
+# ----------------------------------------------------------------------------------------- sampling
+_S = InfectionState
+
+
+def _copy_uv(dst, src):
+    """UncertainValue assignment: value and (a copy of the) distribution."""
+    dst.value = src.value
+    d = src.get_distribution()
+    dst._dist_set(copy.deepcopy(d) if d is not None else None)
+
+
+def draw_sample_infection(model):
+    """osecir::draw_sample_infection (models/ode_secir/parameter_space.h:153-195): seasonality and the
+    parameters shared by all age groups are drawn once (group 0) and copied to every group; the age-specific
+    ones are drawn per group."""
+    p = model.parameters
+    p.Seasonality.draw_sample()
+    shared = ("TimeExposed", "TimeInfectedNoSymptoms", "RelativeTransmissionNoSymptoms",
+              "RiskOfInfectionFromSymptomatic", "MaxRiskOfInfectionFromSymptomatic")
+    for name in ("TimeExposed", "TimeInfectedNoSymptoms", "TimeInfectedSymptoms",
+                 "RelativeTransmissionNoSymptoms", "RiskOfInfectionFromSymptomatic",
+                 "MaxRiskOfInfectionFromSymptomatic"):
+        getattr(p, name)[0].draw_sample()
+    for i in range(p.get_num_groups()):
+        for name in shared:
+            arr = getattr(p, name)
+            _copy_uv(arr[i], arr[0])
+        for name in ("TimeInfectedSymptoms", "TimeInfectedSevere", "TimeInfectedCritical",
+                     "TransmissionProbabilityOnContact", "RecoveredPerInfectedNoSymptoms", "DeathsPerCritical",
+                     "SeverePerInfectedSymptoms", "CriticalPerSevere"):
+            getattr(p, name)[i].draw_sample()
+
+
+def draw_sample_demographics(model):
+    """osecir::draw_sample_demographics (parameter_space.h:120-151): the capacities and the initial
+    compartments E, INS, ISy, ISev, ICr, R are drawn; S absorbs the difference so each group keeps its total."""
+    p, pop = model.parameters, model.populations
+    p.ICUCapacity.draw_sample()
+    p.TestAndTraceCapacity.draw_sample()
+    for i in range(p.get_num_groups()):
+        total = pop.get_group_total_AgeGroup(i)
+        for st in (_S.Exposed, _S.InfectedNoSymptoms, _S.InfectedSymptoms, _S.InfectedSevere,
+                   _S.InfectedCritical, _S.Recovered):
+            pop[i, st].draw_sample()
+        pop.set_difference_from_group_total_AgeGroup((i, _S.Susceptible), total)
+        pop.set_difference_from_group_total_AgeGroup((i, _S.Susceptible), pop.get_group_total_AgeGroup(i))
+
+
+def draw_sample(model_or_graph):
+    """osecir::draw_sample (parameter_space.h:197-251). For a Model: infection and demographic parameters,
+    then the contact matrices from the sampled dampings (`ContactPatterns.draw_sample`), then
+    apply_constraints. For a ModelGraph: returns the sampled graph: node 0's infection parameters, contact
+    dampings and dynamic NPIs are drawn once and given to every node (each keeps its own demographics, ICU and
+    test-and-trace capacities and school holidays; its contact matrices are rebuilt from the sampled
+    dampings, replacing the dampings they had); every edge gets the sampled contact dampings as mobility
+    dampings and the node-0 dynamic NPIs. The graph passed in is changed too (sampled in place)."""
+    if isinstance(model_or_graph, Model):
+        m = model_or_graph
+        draw_sample_infection(m)
+        draw_sample_demographics(m)
+        m.parameters.ContactPatterns.draw_sample()
+        m.apply_constraints()
+        return None
+    graph = model_or_graph
+    shared = graph.get_node(0).property
+    draw_sample_infection(shared)
+    shared_contacts = shared.parameters.ContactPatterns
+    shared_contacts.draw_sample_dampings()
+    shared_npis = shared.parameters.DynamicNPIsInfectedSymptoms
+    shared_npis.draw_sample()
+    sampled = ModelGraph()
+    for v in range(graph.num_nodes):
+        node = graph.get_node(v)
+        m = node.property
+        draw_sample_demographics(m)
+        icu, tnt = UncertainValue(), UncertainValue()
+        _copy_uv(icu, m.parameters.ICUCapacity)
+        _copy_uv(tnt, m.parameters.TestAndTraceCapacity)
+        holidays = list(m.parameters.ContactPatterns.school_holidays)
+        m.parameters = copy.deepcopy(shared.parameters)
+        _copy_uv(m.parameters.ICUCapacity, icu)
+        _copy_uv(m.parameters.TestAndTraceCapacity, tnt)
+        m.parameters.ContactPatterns.school_holidays = holidays
+        m.parameters.ContactPatterns.make_matrix()
+        m.apply_constraints()
+        sampled.add_node(node.id, m)
+    for e in graph._edges:
+        ep = copy.deepcopy(e.property)
+        n = ep.coefficients[0].baseline.size
+        apply_dampings(ep.coefficients, shared_contacts.dampings, lambda g: make_mobility_damping_vector(n, g))
+        ep.dynamic_npis_infected = copy.deepcopy(shared_npis)
+        sampled.add_edge(e.start_node_idx, e.end_node_idx, ep)
+    return sampled
+
+
+def set_params_distributions_normal(model, t0, tmax, dev_rel):
+    """osecir::set_params_distributions_normal (parameter_space.h:46-118): a truncated normal distribution
+    around every sampled value within (1 -+ 2.6 dev_rel) * value, and a contact damping of value 0.5 (with
+    such a distribution) at the middle of [t0, tmax] on every contact matrix. The reference passes
+    dev_rel * value in the place of the quantile of ParameterDistributionNormal(lb, ub, mean, quantile) (its
+    four-argument form); that is kept, so the distributions are the ones the reference samples."""
+    big = np.finfo(np.float64).max
+    dev_rel = float(dev_rel)
+
+    def set_distribution(uv, min_val=0.001):
+        v = uv.value
+        lb = min(max(min_val, (1 - dev_rel * 2.6) * v), 0.1 * big)
+        ub = min(max(min_val, (1 + dev_rel * 2.6) * v), 0.5 * big)
+        if abs(lb - ub) <= 1e-12:
+            uv.set_distribution(ParameterDistributionConstant(min(max(min_val, v), 0.3 * big)))
+        else:
+            uv.set_distribution(ParameterDistributionNormal(lb, ub, min(max(min_val, v), 0.3 * big),
+                                                            min(max(min_val, dev_rel * v), big)))
+
+    p = model.parameters
+    set_distribution(p.Seasonality, 0.0)
+    set_distribution(p.ICUCapacity)
+    set_distribution(p.TestAndTraceCapacity)
+    for i in range(p.get_num_groups()):
+        for st in _S:
+            if st in (_S.Susceptible, _S.Dead, _S.Count):
+                continue
+            set_distribution(model.populations[i, st], 0.0)
+    for i in range(p.get_num_groups()):
+        for name in ("TimeExposed", "TimeInfectedNoSymptoms", "TimeInfectedSymptoms", "TimeInfectedSevere",
+                     "TimeInfectedCritical", "TransmissionProbabilityOnContact", "RelativeTransmissionNoSymptoms",
+                     "RecoveredPerInfectedNoSymptoms", "RiskOfInfectionFromSymptomatic",
+                     "MaxRiskOfInfectionFromSymptomatic", "DeathsPerCritical", "SeverePerInfectedSymptoms",
+                     "CriticalPerSevere"):
+            set_distribution(getattr(p, name)[i])
+    cp = p.ContactPatterns
+    cp.dampings.append(DampingSampling(0.5, 0, 0, float(t0) + (float(tmax) - float(t0)) / 2,
+                                       list(range(cp.cont_freq_mat.num_matrices)), np.ones(p.get_num_groups())))
+    set_distribution(cp.dampings[-1].value, 0.0)
+
+
+class ParameterStudy:
+    """ParameterStudy of one model (bindings compartments/parameter_studies.h:55-88): every run draws a
+    sample of a copy of the model (draw_sample) and simulates it from t0 to tmax with step dt."""
+
+    def __init__(self, model, t0, tmax, dt, num_runs):
+        self.parameters = model
+        self.t0, self.tmax, self.dt = float(t0), float(tmax), float(dt)
+        self.num_runs = int(num_runs)
+
+    def run(self, handle_result_func=None, integrator=None):
+        sims = []
+        for run in range(self.num_runs):
+            m = copy.deepcopy(self.parameters)
+            draw_sample(m)
+            sim = Simulation(m, self.t0, self.dt, integrator)
+            sim.advance(self.tmax)
+            if handle_result_func is not None:
+                handle_result_func(sim, run)
+            else:
+                sims.append(sim)
+        return None if handle_result_func is not None else sims
+
+
 class GraphParameterStudy:
-    """ParameterStudy over a ModelGraph (parameter_studies.h:156-217, bindings :92-124): every run builds
-    the simulation graph with node step = graph step = dt (make_sampled_graph_simulation), after the
-    optional sample(graph, run_idx) callback; all runs execute as one device batch (mgo_ensemble_run)."""
+    """ParameterStudy over a ModelGraph (parameter_studies.h:156-217, bindings :92-124): every run draws a
+    sample of a copy of the graph (draw_sample: node 0's infection parameters, contact dampings and dynamic
+    NPIs for all nodes, per-node demographics) and builds the simulation graph with node step = graph step =
+    dt (make_sampled_graph_simulation). An optional sample(graph, run_idx) callback then edits the sampled
+    graph; all runs execute as one device batch (mgo_ensemble_run)."""
 
     def __init__(self, graph, t0, tmax, dt, num_runs):
         self.parameters = graph
@@ -578,7 +755,7 @@ class GraphParameterStudy:
     def run(self, handle_result_func=None, sample=None, integrator=None):
         graphs, specs, cfgs = [], [], []
         for run in range(self.num_runs):
-            g = copy.deepcopy(self.parameters)
+            g = draw_sample(copy.deepcopy(self.parameters))
             if sample is not None:
                 sample(g, run)
             sg = MobilityGraph()

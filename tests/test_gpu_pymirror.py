@@ -1,5 +1,7 @@
 """The Python mirror of memilio.simulation.osecir on the device: the reference's Python mobility example,
 a GraphParameterStudy batch and osecir.simulate, each against the oracle on the same descriptor."""
+import copy
+
 import numpy as np
 import pytest
 
@@ -40,16 +42,24 @@ def test_mobility_simulation_example(oracle):
 
 
 def test_graph_parameter_study(oracle):
+    """Every run simulates its own sample of the graph (draw_sample over the normal distributions of
+    set_params_distributions_normal, then the sample callback); each run's device result is checked against
+    the oracle on the graph that run sampled."""
     graph = ps.study_graph(n=5, G=2)
+    for v in range(graph.num_nodes):
+        osecir.set_params_distributions_normal(graph.get_node(v).property, 1.0, 30.0, 0.2)
     study = osecir.GraphParameterStudy(graph, t0=1, tmax=30, dt=0.5, num_runs=6)
-    seen = []
+    sampled_graphs, seen = {}, []
+
+    def sample(g, run_idx):
+        ps.scale_contacts(g, run_idx)
+        sampled_graphs[run_idx] = copy.deepcopy(g)
 
     def handle(g, run_idx):
         seen.append(run_idx)
         assert g.get_node(0).property.result.get_time(0) == 1
         assert g.get_node(0).property.result.get_last_time() == 30
-        sampled = ps.study_graph(n=5, G=2)
-        ps.scale_contacts(sampled, run_idx)
+        sampled = sampled_graphs[run_idx]
         sg = osecir.MobilityGraph()
         for v in range(sampled.num_nodes):
             sg.add_node(v, sampled.get_node(v).property, 1.0, 0.5)
@@ -59,8 +69,14 @@ def test_graph_parameter_study(oracle):
         ref, _ = oracle.graph_run(_spec(sg), cfg, 30.0)
         assert rel_err(_results(g), ref) < 1e-6
 
-    study.run(handle, sample=ps.scale_contacts)
+    mio.seed_random_number_generator(17)
+    study.run(handle, sample=sample)
     assert seen == list(range(6))
+    # the runs are samples: their shared infection parameters and initial compartments differ
+    t_exp = {sampled_graphs[r].get_node(0).property.parameters.TimeExposed[0].value for r in range(6)}
+    e0 = {sampled_graphs[r].get_node(3).property.populations[0, osecir.InfectionState.Exposed].value
+          for r in range(6)}
+    assert len(t_exp) == 6 and len(e0) == 6
 
 
 def test_simulate_single_model(oracle):

@@ -27,7 +27,8 @@ def test_uncertain_value_known_answers():
     assert mj.uv_to_json(2.0, distribution=normal) == {"Value": 2.0, "Distribution": normal}
     assert mj.uv_to_json(2.0, mj.IOF_OmitDistributions, normal) == {"Value": 2.0}
     assert mj.uv_from_json({"Value": 2.0}) == (2.0, None)
-    assert mj.uv_from_json({"Value": 2.0, "Distribution": normal}) == (2.0, normal)
+    v, d = mj.uv_from_json({"Value": 2.0, "Distribution": normal})
+    assert v == 2.0 and isinstance(d, mio.ParameterDistributionNormal) and d.to_json() == normal
     assert mj.uv_from_json({"Value": 2.0, "Distribution": normal}, mj.IOF_OmitDistributions) == (2.0, None)
 
 
