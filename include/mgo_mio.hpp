@@ -23,6 +23,9 @@
 //   osecir::simulate                     cpp/models/ode_secir/model.h:385-390
 //   ParameterStudy::run / run_serial     cpp/memilio/compartments/parameter_studies.h:47-130,156-217,288-345
 //   interpolate_simulation_result        cpp/memilio/data/analyze_result.h:186-265
+//   parameter sampling                   cpp/memilio/utils/uncertain_value.h, parameter_distributions.h,
+//                                        epidemiology/uncertain_matrix.h, damping_sampling.h:255-306,
+//                                        models/ode_secir/parameter_space.h:46-251
 //
 // Differences a caller sees:
 //   * Results are daily: a node's get_result() holds the values at the integer days reached (what
@@ -33,7 +36,11 @@
 //   * All nodes of a graph share one integrator core configuration and initial step (the device runs a
 //     graph with one configuration); differing nodes raise Error(MGO_ERR_UNSUPPORTED).
 //   * ParameterStudy::run executes every simulation create_simulation returns in one device batch, then
-//     calls process_result in run order. Parameter sampling stays with create_simulation.
+//     calls process_result in run order; create_simulation samples as in the reference (draw_sample,
+//     make_sampled_graph_simulation).
+//   * Parameter sampling (UncertainValue distributions, set_params_distributions_normal, draw_sample) follows
+//     the reference's rules with std::mt19937_64 in place of its Random123 generator: samples follow the
+//     same distributions, not the same values for a seed.
 //   * Errors are thrown as mgo_mio::Error (status + message) instead of the reference's asserts.
 //   * Edge result tracking (MobilityEdge save_indices / get_mobility_results) is not provided; passing save
 //     indices is accepted and ignored.
@@ -52,6 +59,7 @@
 #include <cstdint>
 #include <limits>
 #include <memory>
+#include <random>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -680,12 +688,430 @@ public:
 // ---------------------------------------------------------------------------------------------- osecir
 // DampingSampling (cpp/memilio/epidemiology/damping_sampling.h:40-230): one damping a dynamic NPI adds,
 // value * group weights on the listed matrices of the contact matrix group.
+// ---------------------------------------------------------------------------------------------- sampling
+// Random numbers for parameter sampling (mio::thread_local_rng, utils/random_number_generator.h). The
+// reference draws from a Random123 counter-based generator; this layer uses std::mt19937_64 with the
+// standard library's distributions, so samples follow the reference's distributions, not its values.
+class RandomNumberGenerator
+{
+public:
+    using result_type = std::mt19937_64::result_type;
+    RandomNumberGenerator()
+    {
+        std::random_device rd;
+        seed({rd(), rd(), rd(), rd()});
+    }
+    static constexpr result_type min()
+    {
+        return std::mt19937_64::min();
+    }
+    static constexpr result_type max()
+    {
+        return std::mt19937_64::max();
+    }
+    result_type operator()()
+    {
+        return m_gen();
+    }
+    void seed(const std::vector<uint32_t>& seeds)
+    {
+        std::seed_seq sq(seeds.begin(), seeds.end());
+        m_gen.seed(sq);
+    }
+
+private:
+    std::mt19937_64 m_gen;
+};
+
+inline RandomNumberGenerator& thread_local_rng()
+{
+    thread_local RandomNumberGenerator rng;
+    return rng;
+}
+
+// ParameterDistribution (utils/parameter_distributions.h:72-130): predefined samples are returned first, in
+// the order added, each once; then random samples.
+class ParameterDistribution
+{
+public:
+    virtual ~ParameterDistribution() = default;
+    void add_predefined_sample(double v)
+    {
+        m_predefined.push_back(v);
+    }
+    void remove_predefined_samples()
+    {
+        m_predefined.clear();
+    }
+    const std::vector<double>& get_predefined_samples() const
+    {
+        return m_predefined;
+    }
+    double get_sample(RandomNumberGenerator& rng)
+    {
+        if (!m_predefined.empty()) {
+            const double v = m_predefined.front();
+            m_predefined.erase(m_predefined.begin());
+            return v;
+        }
+        return get_rand_sample(rng);
+    }
+    virtual double get_rand_sample(RandomNumberGenerator& rng) = 0;
+    virtual std::unique_ptr<ParameterDistribution> clone() const = 0;
+
+protected:
+    std::vector<double> m_predefined;
+};
+
+// ParameterDistributionNormal (parameter_distributions.h:159-427): a normal distribution truncated to
+// [lb, ub] (default [DBL_MIN, DBL_MAX]). Before drawing, a mean outside the bounds becomes (ub - lb) / 2 and
+// the standard deviation shrinks until mean +- quantile * std fits; a draw outside the bounds is redrawn up
+// to 10 times, then set to ub if the last draw is above it, else to lb.
+class ParameterDistributionNormal : public ParameterDistribution
+{
+public:
+    ParameterDistributionNormal() = default;
+    ParameterDistributionNormal(double mean, double standard_dev)
+        : m_mean(mean)
+        , m_std(standard_dev)
+    {
+    }
+    // standard deviation: the widest for which mean +- quantile * std fits in [lb, ub]
+    ParameterDistributionNormal(double lower_bound, double upper_bound, double mean, double quantile)
+        : m_mean(mean)
+        , m_std(upper_bound)
+        , m_lb(lower_bound)
+        , m_ub(upper_bound)
+        , m_quantile(quantile)
+    {
+        adapt_standard_dev();
+    }
+    ParameterDistributionNormal(double lower_bound, double upper_bound, double mean, double standard_dev,
+                                double quantile)
+        : m_mean(mean)
+        , m_std(standard_dev)
+        , m_lb(lower_bound)
+        , m_ub(upper_bound)
+        , m_quantile(quantile)
+    {
+        check_quantiles();
+    }
+    double get_mean() const
+    {
+        return m_mean;
+    }
+    void set_mean(double v)
+    {
+        m_mean = v;
+    }
+    double get_standard_dev() const
+    {
+        return m_std;
+    }
+    void set_standard_dev(double v)
+    {
+        m_std = v;
+    }
+    double get_lower_bound() const
+    {
+        return m_lb;
+    }
+    void set_lower_bound(double v)
+    {
+        m_lb = v;
+    }
+    double get_upper_bound() const
+    {
+        return m_ub;
+    }
+    void set_upper_bound(double v)
+    {
+        m_ub = v;
+    }
+    double get_quantile() const
+    {
+        return m_quantile;
+    }
+    bool check_quantiles()
+    {
+        bool changed = false;
+        if (m_mean < m_lb || m_mean > m_ub) {
+            m_mean  = 0.5 * (m_ub - m_lb);
+            changed = true;
+        }
+        return adapt_standard_dev() || changed;
+    }
+    double get_rand_sample(RandomNumberGenerator& rng) override
+    {
+        if (m_ub == m_lb) {
+            return m_lb;
+        }
+        check_quantiles();
+        std::normal_distribution<double> nd(m_mean, m_std);
+        double r = nd(rng);
+        for (int i = 0; (r > m_ub || r < m_lb) && i < 10;) {
+            r = nd(rng);
+            if (++i == 10) {
+                r = r > m_ub ? m_ub : m_lb;
+            }
+        }
+        return r;
+    }
+    std::unique_ptr<ParameterDistribution> clone() const override
+    {
+        return std::make_unique<ParameterDistributionNormal>(*this);
+    }
+
+private:
+    bool adapt_standard_dev()
+    {
+        bool changed = false;
+        if (m_mean + m_std * m_quantile > m_ub) {
+            m_std   = (m_ub - m_mean) / m_quantile;
+            changed = true;
+        }
+        if (m_mean - m_std * m_quantile < m_lb) {
+            m_std   = (m_mean - m_lb) / m_quantile;
+            changed = true;
+        }
+        return changed;
+    }
+    double m_mean = 0.0;
+    double m_std  = 1.0;
+    double m_lb   = DBL_MIN;
+    double m_ub   = DBL_MAX;
+    double m_quantile = 2.5758;
+};
+
+// ParameterDistributionUniform (parameter_distributions.h:439-560)
+class ParameterDistributionUniform : public ParameterDistribution
+{
+public:
+    ParameterDistributionUniform(double lower_bound = 0.0, double upper_bound = 1.0)
+        : m_lb(lower_bound)
+        , m_ub(upper_bound)
+    {
+    }
+    double get_lower_bound() const
+    {
+        return m_lb;
+    }
+    double get_upper_bound() const
+    {
+        return m_ub;
+    }
+    void set_lower_bound(double v)
+    {
+        m_lb = v;
+    }
+    void set_upper_bound(double v)
+    {
+        m_ub = v;
+    }
+    double get_rand_sample(RandomNumberGenerator& rng) override
+    {
+        return std::uniform_real_distribution<double>(m_lb, m_ub)(rng);
+    }
+    std::unique_ptr<ParameterDistribution> clone() const override
+    {
+        return std::make_unique<ParameterDistributionUniform>(*this);
+    }
+
+private:
+    double m_lb, m_ub;
+};
+
+// ParameterDistributionLogNormal (parameter_distributions.h:566-690): exp of a normal (log_mean, log_stddev)
+class ParameterDistributionLogNormal : public ParameterDistribution
+{
+public:
+    ParameterDistributionLogNormal(double log_mean = 0.0, double log_stddev = 1.0)
+        : m_mean(log_mean)
+        , m_std(log_stddev)
+    {
+    }
+    double get_log_mean() const
+    {
+        return m_mean;
+    }
+    double get_log_stddev() const
+    {
+        return m_std;
+    }
+    double get_rand_sample(RandomNumberGenerator& rng) override
+    {
+        return std::lognormal_distribution<double>(m_mean, m_std)(rng);
+    }
+    std::unique_ptr<ParameterDistribution> clone() const override
+    {
+        return std::make_unique<ParameterDistributionLogNormal>(*this);
+    }
+
+private:
+    double m_mean, m_std;
+};
+
+// ParameterDistributionExponential (parameter_distributions.h:693-808)
+class ParameterDistributionExponential : public ParameterDistribution
+{
+public:
+    explicit ParameterDistributionExponential(double rate = 1.0)
+        : m_rate(rate)
+    {
+    }
+    double get_rate() const
+    {
+        return m_rate;
+    }
+    double get_rand_sample(RandomNumberGenerator& rng) override
+    {
+        return std::exponential_distribution<double>(m_rate)(rng);
+    }
+    std::unique_ptr<ParameterDistribution> clone() const override
+    {
+        return std::make_unique<ParameterDistributionExponential>(*this);
+    }
+
+private:
+    double m_rate;
+};
+
+// ParameterDistributionConstant (parameter_distributions.h:812-920)
+class ParameterDistributionConstant : public ParameterDistribution
+{
+public:
+    explicit ParameterDistributionConstant(double constant = 0.0)
+        : m_c(constant)
+    {
+    }
+    double get_constant() const
+    {
+        return m_c;
+    }
+    double get_rand_sample(RandomNumberGenerator&) override
+    {
+        return m_c;
+    }
+    std::unique_ptr<ParameterDistribution> clone() const override
+    {
+        return std::make_unique<ParameterDistributionConstant>(*this);
+    }
+
+private:
+    double m_c;
+};
+
+// non-owning pointer to a value's distribution (the reference's observer_ptr)
+template <class T>
+class observer_ptr
+{
+public:
+    explicit observer_ptr(T* p = nullptr)
+        : m_p(p)
+    {
+    }
+    T* get() const
+    {
+        return m_p;
+    }
+    T* operator->() const
+    {
+        return m_p;
+    }
+    T& operator*() const
+    {
+        return *m_p;
+    }
+    explicit operator bool() const
+    {
+        return m_p != nullptr;
+    }
+
+private:
+    T* m_p;
+};
+
+// UncertainValue (utils/uncertain_value.h:44-190): a value and an optional distribution it can be drawn
+// from. Converts to the value (FP&), so it is read and assigned like a number; assigning a number keeps
+// the distribution, copying copies it.
+template <class FP = double>
+class UncertainValue
+{
+public:
+    UncertainValue(FP v = FP(0))
+        : m_value(v)
+    {
+    }
+    UncertainValue(const UncertainValue& o)
+        : m_value(o.m_value)
+        , m_dist(o.m_dist ? o.m_dist->clone() : nullptr)
+    {
+    }
+    UncertainValue(UncertainValue&&) noexcept = default;
+    UncertainValue& operator=(const UncertainValue& o)
+    {
+        if (this != &o) {
+            m_value = o.m_value;
+            m_dist  = o.m_dist ? o.m_dist->clone() : nullptr;
+        }
+        return *this;
+    }
+    UncertainValue& operator=(UncertainValue&&) noexcept = default;
+    UncertainValue& operator=(FP v)
+    {
+        m_value = v;
+        return *this;
+    }
+    operator FP&()
+    {
+        return m_value;
+    }
+    operator const FP&() const
+    {
+        return m_value;
+    }
+    FP& value()
+    {
+        return m_value;
+    }
+    const FP& value() const
+    {
+        return m_value;
+    }
+    void set_distribution(const ParameterDistribution& d)
+    {
+        m_dist = d.clone();
+    }
+    observer_ptr<ParameterDistribution> get_distribution()
+    {
+        return observer_ptr<ParameterDistribution>(m_dist.get());
+    }
+    observer_ptr<const ParameterDistribution> get_distribution() const
+    {
+        return observer_ptr<const ParameterDistribution>(m_dist.get());
+    }
+    // value = a sample of the distribution (unchanged without one)
+    FP draw_sample()
+    {
+        if (m_dist) {
+            m_value = m_dist->get_sample(thread_local_rng());
+        }
+        return m_value;
+    }
+
+private:
+    FP m_value;
+    std::unique_ptr<ParameterDistribution> m_dist;
+};
+
+// DampingSampling (epidemiology/damping_sampling.h:40-230): a damping of value * group weights that a
+// dynamic NPI or a parameter study adds to the listed contact matrices; the value may carry a distribution.
 template <class FP = double>
 class DampingSampling
 {
 public:
     template <class V>
-    DampingSampling(double value, DampingLevel level, DampingType type, SimulationTime time,
+    DampingSampling(const UncertainValue<FP>& value, DampingLevel level, DampingType type, SimulationTime time,
                     std::vector<size_t> matrix_indices, const V& group_weights)
         : m_value(value)
         , m_level(level)
@@ -695,7 +1121,11 @@ public:
         , m_weights(detail::to_vector(group_weights))
     {
     }
-    double get_value() const
+    const UncertainValue<FP>& get_value() const
+    {
+        return m_value;
+    }
+    UncertainValue<FP>& get_value()
     {
         return m_value;
     }
@@ -703,13 +1133,25 @@ public:
     {
         return m_level;
     }
+    void set_level(DampingLevel l)
+    {
+        m_level = l;
+    }
     DampingType get_type() const
     {
         return m_type;
     }
+    void set_type(DampingType t)
+    {
+        m_type = t;
+    }
     SimulationTime get_time() const
     {
         return m_time;
+    }
+    void set_time(SimulationTime t)
+    {
+        m_time = t;
     }
     const std::vector<size_t>& get_matrix_indices() const
     {
@@ -719,15 +1161,58 @@ public:
     {
         return m_weights;
     }
+    void draw_sample()
+    {
+        m_value.draw_sample();
+    }
 
 private:
-    double m_value;
+    UncertainValue<FP> m_value;
     DampingLevel m_level;
     DampingType m_type;
     SimulationTime m_time;
     std::vector<size_t> m_mats;
     Vector m_weights;
 };
+
+// make_contact_damping_matrix (damping_sampling.h:268-290): d_ij = 1 - sqrt((1 - g_i)(1 - g_j))
+inline Matrix make_contact_damping_matrix(const Vector& g)
+{
+    Matrix m(g.size(), g.size());
+    for (size_t i = 0; i < g.size(); ++i) {
+        for (size_t j = 0; j < g.size(); ++j) {
+            m(i, j) = 1.0 - std::sqrt((1.0 - g[i]) * (1.0 - g[j]));
+        }
+    }
+    return m;
+}
+
+// make_mobility_damping_vector (damping_sampling.h:298-306): each group's value on all its compartments
+inline Vector make_mobility_damping_vector(size_t size, const Vector& g)
+{
+    Vector v(size);
+    const size_t per = g.size() ? size / g.size() : 0;
+    for (size_t i = 0; i < size; ++i) {
+        v[i] = per ? g[i / per] : 0.0;
+    }
+    return v;
+}
+
+// apply_dampings (damping_sampling.h:255-266): every damping sampling adds make_matrix(value * weights) to
+// each matrix it lists
+template <class Group, class DampingRange, class F>
+void apply_dampings(Group& group, const DampingRange& dampings, F make_matrix)
+{
+    for (const auto& d : dampings) {
+        Vector w = d.get_group_weights();
+        for (size_t i = 0; i < w.size(); ++i) {
+            w[i] *= double(d.get_value());
+        }
+        for (size_t k : d.get_matrix_indices()) {
+            group[k].add_damping(make_matrix(w), d.get_level(), d.get_type(), d.get_time());
+        }
+    }
+}
 
 // DynamicNPIs (cpp/memilio/epidemiology/dynamic_npis.h:36-242): thresholds (cases per base value, sorted
 // descending, an equal threshold replaced) and the NPIs they trigger; duration, implementation delay and
@@ -793,6 +1278,15 @@ public:
     {
         m_end = v;
     }
+    // samples the value of every damping of every threshold (dynamic_npis.h:192-199)
+    void draw_sample()
+    {
+        for (auto& t : m_thresholds) {
+            for (auto& d : t.second) {
+                d.draw_sample();
+            }
+        }
+    }
 
 private:
     std::vector<std::pair<double, std::vector<DampingSampling<FP>>>> m_thresholds;
@@ -822,24 +1316,24 @@ enum class InfectionState
 };
 
 // Per-age-group parameter values (CustomIndexArray<UncertainValue, AgeGroup>); assigning a scalar sets
-// every group.
+// every group (and, as assigning an UncertainValue without distribution, drops their distributions).
 class AgeArray
 {
 public:
     explicit AgeArray(size_t num_groups = 0, double v = 0.0)
-        : m_v(num_groups, v)
+        : m_v(num_groups, UncertainValue<double>(v))
     {
     }
     AgeArray& operator=(double v)
     {
-        std::fill(m_v.begin(), m_v.end(), v);
+        std::fill(m_v.begin(), m_v.end(), UncertainValue<double>(v));
         return *this;
     }
-    double& operator[](AgeGroup g)
+    UncertainValue<double>& operator[](AgeGroup g)
     {
         return m_v.at(g.get());
     }
-    double operator[](AgeGroup g) const
+    const UncertainValue<double>& operator[](AgeGroup g) const
     {
         return m_v.at(g.get());
     }
@@ -849,7 +1343,7 @@ public:
     }
 
 private:
-    std::vector<double> m_v;
+    std::vector<UncertainValue<double>> m_v;
 };
 
 // parameter tags (parameters.h:46-394): scalar ones index MGO_P_*, per-age ones MGO_A_*
@@ -901,6 +1395,116 @@ struct ContactPatterns {
     }
 };
 
+} // namespace osecir
+
+// UncertainContactMatrix (epidemiology/uncertain_matrix.h:40-230): the contact matrix group and the
+// parameter-study inputs beside it: contact dampings to sample, the school-holiday damping and the
+// holidays. Converts to the ContactMatrixGroup.
+template <class FP = double>
+class UncertainContactMatrix
+{
+public:
+    UncertainContactMatrix(size_t num_matrices = 1, size_t num_groups = 1)
+        : UncertainContactMatrix(ContactMatrixGroup(num_matrices, num_groups))
+    {
+    }
+    UncertainContactMatrix(const ContactMatrixGroup& cont_freq)
+        : m_cont_freq(cont_freq)
+        , m_holiday_damping(UncertainValue<FP>(0.0), DampingLevel(0), DampingType(0), SimulationTime(0.0), {},
+                            Vector(cont_freq.get_num_matrices() ? cont_freq[0].get_baseline().rows() : 0, 0.0))
+    {
+    }
+    // replaces the matrices; the sampling inputs stay
+    UncertainContactMatrix& operator=(const ContactMatrixGroup& cont_freq)
+    {
+        m_cont_freq = cont_freq;
+        return *this;
+    }
+    operator ContactMatrixGroup&()
+    {
+        return m_cont_freq;
+    }
+    operator const ContactMatrixGroup&() const
+    {
+        return m_cont_freq;
+    }
+    ContactMatrixGroup& get_cont_freq_mat()
+    {
+        return m_cont_freq;
+    }
+    const ContactMatrixGroup& get_cont_freq_mat() const
+    {
+        return m_cont_freq;
+    }
+    std::vector<DampingSampling<FP>>& get_dampings()
+    {
+        return m_dampings;
+    }
+    const std::vector<DampingSampling<FP>>& get_dampings() const
+    {
+        return m_dampings;
+    }
+    DampingSampling<FP>& get_school_holiday_damping()
+    {
+        return m_holiday_damping;
+    }
+    const DampingSampling<FP>& get_school_holiday_damping() const
+    {
+        return m_holiday_damping;
+    }
+    std::vector<std::pair<SimulationTime, SimulationTime>>& get_school_holidays()
+    {
+        return m_holidays;
+    }
+    const std::vector<std::pair<SimulationTime, SimulationTime>>& get_school_holidays() const
+    {
+        return m_holidays;
+    }
+    void draw_sample_dampings()
+    {
+        for (auto& d : m_dampings) {
+            d.draw_sample();
+        }
+        m_holiday_damping.draw_sample();
+    }
+    // the matrices with the sampled dampings (uncertain_matrix.h:172-198): without accum the matrices'
+    // dampings are replaced; each holiday switches the holiday damping on at its first day and to 0 at its
+    // last
+    ContactMatrixGroup make_matrix(bool accum = false)
+    {
+        if (!accum) {
+            m_cont_freq.clear_dampings();
+        }
+        auto mk = [](const Vector& v) {
+            return make_contact_damping_matrix(v);
+        };
+        apply_dampings(m_cont_freq, m_dampings, mk);
+        for (const auto& h : m_holidays) {
+            DampingSampling<FP> d = m_holiday_damping;
+            d.set_time(h.first);
+            apply_dampings(m_cont_freq, std::vector<DampingSampling<FP>>{d}, mk);
+            d.get_value() = 0.0;
+            d.set_time(h.second);
+            apply_dampings(m_cont_freq, std::vector<DampingSampling<FP>>{d}, mk);
+        }
+        return m_cont_freq;
+    }
+    ContactMatrixGroup draw_sample(bool accum = false)
+    {
+        draw_sample_dampings();
+        return make_matrix(accum);
+    }
+
+private:
+    ContactMatrixGroup m_cont_freq;
+    std::vector<DampingSampling<FP>> m_dampings;
+    DampingSampling<FP> m_holiday_damping;
+    std::vector<std::pair<SimulationTime, SimulationTime>> m_holidays;
+};
+
+namespace osecir
+{
+
 // DynamicNPIsInfectedSymptoms (parameters.h:342-355)
 template <class FP = double>
 struct DynamicNPIsInfectedSymptoms {
@@ -918,7 +1522,7 @@ public:
         : m_G(num_groups.get())
         , m_contacts(1, num_groups.get())
     {
-        m_scalars.fill(0.0);
+        m_scalars.fill(UncertainValue<double>(0.0));
         m_scalars[MGO_P_ICU_CAPACITY]            = std::numeric_limits<double>::max();
         m_scalars[MGO_P_TEST_AND_TRACE_CAPACITY] = std::numeric_limits<double>::max();
         m_scalars[MGO_P_TEST_AND_TRACE_MAX_RISK] = 5.0;
@@ -997,9 +1601,59 @@ public:
     {
         return m_scalars[MGO_P_END_COMMUTER_DETECTION];
     }
-    const std::array<double, MGO_NUM_SCALARS>& scalars() const
+    // Clamps parameters to admissible values (parameters.h:465-612): seasonality outside [0, 0.5] -> 0,
+    // negative capacities and TestAndTraceCapacityMaxRisk -> 0, stay times below 0.1 -> 0.1,
+    // RelativeTransmissionNoSymptoms < 0 -> 0, the other probabilities outside [0, 1] -> 0 (not
+    // MaxRiskOfInfectionFromSymptomatic), DeathsPerSevere -> 0 where CriticalPerSevere + DeathsPerSevere > 1.
+    // Returns true if anything was corrected.
+    bool apply_constraints()
     {
-        return m_scalars;
+        bool corrected = false;
+        auto fix       = [&](UncertainValue<double>& v, bool bad, double to) {
+            if (bad) {
+                v         = to;
+                corrected = true;
+            }
+        };
+        auto& sea = m_scalars[MGO_P_SEASONALITY];
+        fix(sea, sea < 0.0 || sea > 0.5, 0.0);
+        for (int k : {MGO_P_ICU_CAPACITY, MGO_P_TEST_AND_TRACE_CAPACITY, MGO_P_TEST_AND_TRACE_MAX_RISK}) {
+            fix(m_scalars[size_t(k)], m_scalars[size_t(k)] < 0.0, 0.0);
+        }
+        for (size_t g = 0; g < m_G; ++g) {
+            const AgeGroup a(g);
+            for (int k : {MGO_A_TIME_EXPOSED, MGO_A_TIME_INFECTED_NO_SYMPTOMS, MGO_A_TIME_INFECTED_SYMPTOMS,
+                          MGO_A_TIME_INFECTED_SEVERE, MGO_A_TIME_INFECTED_CRITICAL}) {
+                auto& v = m_ages[size_t(k)][a];
+                fix(v, v < 0.1, 0.1);
+            }
+            auto& rel = m_ages[MGO_A_RELATIVE_TRANSMISSION_NO_SYMPTOMS][a];
+            fix(rel, rel < 0.0, 0.0);
+            for (int k : {MGO_A_TRANSMISSION_PROBABILITY_ON_CONTACT, MGO_A_RECOVERED_PER_INFECTED_NO_SYMPTOMS,
+                          MGO_A_RISK_OF_INFECTION_FROM_SYMPTOMATIC, MGO_A_SEVERE_PER_INFECTED_SYMPTOMS,
+                          MGO_A_CRITICAL_PER_SEVERE, MGO_A_DEATHS_PER_SEVERE}) {
+                auto& v = m_ages[size_t(k)][a];
+                fix(v, v < 0.0 || v > 1.0, 0.0);
+            }
+            auto& dps = m_ages[MGO_A_DEATHS_PER_SEVERE][a];
+            fix(dps, m_ages[MGO_A_CRITICAL_PER_SEVERE][a] + dps > 1.0, 0.0);
+            auto& dpc = m_ages[MGO_A_DEATHS_PER_CRITICAL][a];
+            fix(dpc, dpc < 0.0 || dpc > 1.0, 0.0);
+        }
+        return corrected;
+    }
+    const UncertainValue<double>& scalar_value(int k) const
+    {
+        return m_scalars.at(size_t(k));
+    }
+    // the scalar parameters' values in MGO_P_* order
+    std::array<double, MGO_NUM_SCALARS> scalars() const
+    {
+        std::array<double, MGO_NUM_SCALARS> v{};
+        for (size_t k = 0; k < v.size(); ++k) {
+            v[k] = m_scalars[k];
+        }
+        return v;
     }
     const AgeArray& age_param(int k) const
     {
@@ -1008,13 +1662,14 @@ public:
 
 private:
     size_t m_G;
-    std::array<double, MGO_NUM_SCALARS> m_scalars;
+    std::array<UncertainValue<double>, MGO_NUM_SCALARS> m_scalars;
     std::vector<AgeArray> m_ages;
-    ContactMatrixGroup m_contacts;
+    UncertainContactMatrix<FP> m_contacts;
     mgo_mio::DynamicNPIs<FP> m_npis;
 };
 
-// Compartment values indexed by {AgeGroup, InfectionState}; flat index g*10 + state.
+// Compartment values indexed by {AgeGroup, InfectionState}; flat index g*10 + state. Each compartment is
+// an UncertainValue (populations.h over CustomIndexArray<UncertainValue>).
 class Populations
 {
 public:
@@ -1024,18 +1679,18 @@ public:
     };
     explicit Populations(size_t num_groups = 0)
         : m_G(num_groups)
-        , m_v(num_groups * MGO_NUM_COMPARTMENTS, 0.0)
+        , m_v(num_groups * MGO_NUM_COMPARTMENTS, UncertainValue<double>(0.0))
     {
     }
     size_t get_flat_index(const Index& i) const
     {
         return i.group.get() * MGO_NUM_COMPARTMENTS + size_t(i.state);
     }
-    double& operator[](const Index& i)
+    UncertainValue<double>& operator[](const Index& i)
     {
         return m_v.at(get_flat_index(i));
     }
-    double operator[](const Index& i) const
+    const UncertainValue<double>& operator[](const Index& i) const
     {
         return m_v.at(get_flat_index(i));
     }
@@ -1046,8 +1701,8 @@ public:
     double get_total() const
     {
         double s = 0.0;
-        for (double x : m_v) {
-            s += x;
+        for (const auto& x : m_v) {
+            s += double(x);
         }
         return s;
     }
@@ -1055,7 +1710,7 @@ public:
     {
         double s = 0.0;
         for (size_t c = 0; c < MGO_NUM_COMPARTMENTS; ++c) {
-            s += m_v[g.get() * MGO_NUM_COMPARTMENTS + c];
+            s += double(m_v[g.get() * MGO_NUM_COMPARTMENTS + c]);
         }
         return s;
     }
@@ -1072,14 +1727,15 @@ public:
         (*this)[i] = 0.0;
         (*this)[i] = total - get_group_total(i.group);
     }
-    const double* data() const
+    // the compartment values in flat-index order (Populations::get_compartments)
+    std::vector<double> get_compartments() const
     {
-        return m_v.data();
+        return std::vector<double>(m_v.begin(), m_v.end());
     }
 
 private:
     size_t m_G;
-    std::vector<double> m_v;
+    std::vector<UncertainValue<double>> m_v;
 };
 
 template <class FP = double>
@@ -1094,6 +1750,10 @@ public:
     size_t get_num_groups() const
     {
         return parameters.get_num_groups().get();
+    }
+    bool apply_constraints()
+    {
+        return parameters.apply_constraints();
     }
     Populations populations;
     Parameters<FP> parameters;
@@ -1356,6 +2016,183 @@ private:
 
 namespace osecir
 {
+// ---------------------------------------------------------------------------------------------- sampling
+// set_params_distributions_normal (models/ode_secir/parameter_space.h:46-118): a truncated normal
+// distribution around every sampled value with bounds (1 -+ 2.6 dev_rel) * value (at least min_val: 0.001,
+// 0 for seasonality and compartments), a constant one where the bounds meet, and a contact damping of value
+// 0.5 at the middle of [t0, tmax] on every contact matrix. As in the reference,
+// dev_rel * value is passed in the quantile's place of the (lb, ub, mean, quantile) constructor.
+template <class FP>
+void set_params_distributions_normal(Model<FP>& model, double t0, double tmax, double dev_rel)
+{
+    const double big = std::numeric_limits<double>::max();
+    auto set_dist    = [dev_rel, big](UncertainValue<double>& v, double min_val = 0.001) {
+        const double x  = v;
+        const double lb = std::min(std::max(min_val, (1 - dev_rel * 2.6) * x), 0.1 * big);
+        const double ub = std::min(std::max(min_val, (1 + dev_rel * 2.6) * x), 0.5 * big);
+        const double diff = std::abs(lb - ub);
+        if (diff <= 1e-12 || diff <= std::max(std::abs(lb), std::abs(ub)) * DBL_MIN) {
+            v.set_distribution(ParameterDistributionConstant(std::min(std::max(min_val, x), 0.3 * big)));
+        }
+        else {
+            v.set_distribution(ParameterDistributionNormal(lb, ub, std::min(std::max(min_val, x), 0.3 * big),
+                                                           std::min(std::max(min_val, dev_rel * x), big)));
+        }
+    };
+    auto& p = model.parameters;
+    set_dist(p.template get<Seasonality<FP>>(), 0.0);
+    set_dist(p.template get<ICUCapacity<FP>>());
+    set_dist(p.template get<TestAndTraceCapacity<FP>>());
+    const size_t G = model.get_num_groups();
+    for (size_t g = 0; g < G; ++g) {
+        for (auto st : {InfectionState::Exposed, InfectionState::InfectedNoSymptoms,
+                        InfectionState::InfectedNoSymptomsConfirmed, InfectionState::InfectedSymptoms,
+                        InfectionState::InfectedSymptomsConfirmed, InfectionState::InfectedSevere,
+                        InfectionState::InfectedCritical, InfectionState::Recovered}) {
+            set_dist(model.populations[{AgeGroup(g), st}], 0.0);
+        }
+    }
+    for (size_t g = 0; g < G; ++g) {
+        const AgeGroup a(g);
+        set_dist(p.template get<TimeExposed<FP>>()[a]);
+        set_dist(p.template get<TimeInfectedNoSymptoms<FP>>()[a]);
+        set_dist(p.template get<TimeInfectedSymptoms<FP>>()[a]);
+        set_dist(p.template get<TimeInfectedSevere<FP>>()[a]);
+        set_dist(p.template get<TimeInfectedCritical<FP>>()[a]);
+        set_dist(p.template get<TransmissionProbabilityOnContact<FP>>()[a]);
+        set_dist(p.template get<RelativeTransmissionNoSymptoms<FP>>()[a]);
+        set_dist(p.template get<RecoveredPerInfectedNoSymptoms<FP>>()[a]);
+        set_dist(p.template get<RiskOfInfectionFromSymptomatic<FP>>()[a]);
+        set_dist(p.template get<MaxRiskOfInfectionFromSymptomatic<FP>>()[a]);
+        set_dist(p.template get<DeathsPerCritical<FP>>()[a]);
+        set_dist(p.template get<SeverePerInfectedSymptoms<FP>>()[a]);
+        set_dist(p.template get<CriticalPerSevere<FP>>()[a]);
+    }
+    auto& cp = p.template get<ContactPatterns<FP>>();
+    std::vector<size_t> mats(cp.get_cont_freq_mat().get_num_matrices());
+    for (size_t k = 0; k < mats.size(); ++k) {
+        mats[k] = k;
+    }
+    cp.get_dampings().emplace_back(UncertainValue<FP>(0.5), DampingLevel(0), DampingType(0),
+                                   SimulationTime(t0 + (tmax - t0) / 2), mats, Vector(G, 1.0));
+    // the reference gives the distribution to the first contact damping sampling (the new one only if
+    // there was none before)
+    set_dist(cp.get_dampings()[0].get_value(), 0.0);
+}
+
+// draw_sample_demographics (parameter_space.h:120-151): capacities and the initial compartments E, INS, ISy,
+// ISev, ICr, R are drawn; S takes the difference so that each age group keeps its total.
+template <class FP>
+void draw_sample_demographics(Model<FP>& model)
+{
+    auto& p = model.parameters;
+    p.template get<ICUCapacity<FP>>().draw_sample();
+    p.template get<TestAndTraceCapacity<FP>>().draw_sample();
+    for (size_t g = 0; g < model.get_num_groups(); ++g) {
+        const AgeGroup a(g);
+        const double total = model.populations.get_group_total(a);
+        for (auto st : {InfectionState::Exposed, InfectionState::InfectedNoSymptoms, InfectionState::InfectedSymptoms,
+                        InfectionState::InfectedSevere, InfectionState::InfectedCritical, InfectionState::Recovered}) {
+            model.populations[{a, st}].draw_sample();
+        }
+        model.populations.set_difference_from_group_total({a, InfectionState::Susceptible}, total);
+        model.populations.set_difference_from_group_total({a, InfectionState::Susceptible},
+                                                          model.populations.get_group_total(a));
+    }
+}
+
+// draw_sample_infection (parameter_space.h:153-195): seasonality and the parameters shared by all age
+// groups are drawn once (group 0) and copied to every group; the age-specific ones are drawn per group.
+template <class FP>
+void draw_sample_infection(Model<FP>& model)
+{
+    auto& p = model.parameters;
+    const AgeGroup a0(0);
+    p.template get<Seasonality<FP>>().draw_sample();
+    p.template get<TimeExposed<FP>>()[a0].draw_sample();
+    p.template get<TimeInfectedNoSymptoms<FP>>()[a0].draw_sample();
+    p.template get<TimeInfectedSymptoms<FP>>()[a0].draw_sample();
+    p.template get<RelativeTransmissionNoSymptoms<FP>>()[a0].draw_sample();
+    p.template get<RiskOfInfectionFromSymptomatic<FP>>()[a0].draw_sample();
+    p.template get<MaxRiskOfInfectionFromSymptomatic<FP>>()[a0].draw_sample();
+    for (size_t g = 0; g < model.get_num_groups(); ++g) {
+        const AgeGroup a(g);
+        p.template get<TimeExposed<FP>>()[a]            = p.template get<TimeExposed<FP>>()[a0];
+        p.template get<TimeInfectedNoSymptoms<FP>>()[a] = p.template get<TimeInfectedNoSymptoms<FP>>()[a0];
+        p.template get<RelativeTransmissionNoSymptoms<FP>>()[a] =
+            p.template get<RelativeTransmissionNoSymptoms<FP>>()[a0];
+        p.template get<RiskOfInfectionFromSymptomatic<FP>>()[a] =
+            p.template get<RiskOfInfectionFromSymptomatic<FP>>()[a0];
+        p.template get<MaxRiskOfInfectionFromSymptomatic<FP>>()[a] =
+            p.template get<MaxRiskOfInfectionFromSymptomatic<FP>>()[a0];
+        p.template get<TimeInfectedSymptoms<FP>>()[a].draw_sample();
+        p.template get<TimeInfectedSevere<FP>>()[a].draw_sample();
+        p.template get<TimeInfectedCritical<FP>>()[a].draw_sample();
+        p.template get<TransmissionProbabilityOnContact<FP>>()[a].draw_sample();
+        p.template get<RecoveredPerInfectedNoSymptoms<FP>>()[a].draw_sample();
+        p.template get<DeathsPerCritical<FP>>()[a].draw_sample();
+        p.template get<SeverePerInfectedSymptoms<FP>>()[a].draw_sample();
+        p.template get<CriticalPerSevere<FP>>()[a].draw_sample();
+    }
+}
+
+// draw_sample of a model (parameter_space.h:197-209): infection and demographic parameters, the contact
+// matrices from the sampled dampings, then apply_constraints.
+template <class FP>
+void draw_sample(Model<FP>& model)
+{
+    draw_sample_infection(model);
+    draw_sample_demographics(model);
+    model.parameters.template get<ContactPatterns<FP>>().draw_sample();
+    model.apply_constraints();
+}
+
+// draw_sample of a parameter graph (parameter_space.h:211-251): node 0's infection parameters, contact
+// dampings and dynamic NPIs are drawn once and given to every node; each node keeps its own demographics,
+// ICU and test-and-trace capacities and school holidays, and its contact matrices are rebuilt from the
+// sampled dampings; every edge gets the sampled contact dampings as mobility dampings and node 0's dynamic
+// NPIs. The graph passed in is sampled in place too (its models), as in the reference.
+template <class FP>
+Graph<Model<FP>, MobilityParameters<FP>> draw_sample(Graph<Model<FP>, MobilityParameters<FP>>& graph)
+{
+    Graph<Model<FP>, MobilityParameters<FP>> sampled;
+    if (graph.nodes().empty()) {
+        return sampled;
+    }
+    auto& shared = graph.nodes()[0].property;
+    draw_sample_infection(shared);
+    auto& shared_contacts = shared.parameters.template get<ContactPatterns<FP>>();
+    shared_contacts.draw_sample_dampings();
+    auto& shared_npis = shared.parameters.template get<DynamicNPIsInfectedSymptoms<FP>>();
+    shared_npis.draw_sample();
+    for (auto& node : graph.nodes()) {
+        auto& m = node.property;
+        draw_sample_demographics(m);
+        const UncertainValue<double> icu = m.parameters.template get<ICUCapacity<FP>>();
+        const UncertainValue<double> tnt = m.parameters.template get<TestAndTraceCapacity<FP>>();
+        const auto holidays = m.parameters.template get<ContactPatterns<FP>>().get_school_holidays();
+        if (&m != &shared) {
+            m.parameters = shared.parameters;
+        }
+        m.parameters.template get<ICUCapacity<FP>>()                           = icu;
+        m.parameters.template get<TestAndTraceCapacity<FP>>()                  = tnt;
+        m.parameters.template get<ContactPatterns<FP>>().get_school_holidays() = holidays;
+        m.parameters.template get<ContactPatterns<FP>>().make_matrix();
+        m.apply_constraints();
+        sampled.add_node(node.id, m);
+    }
+    for (auto& e : graph.edges()) {
+        MobilityParameters<FP> ep = e.property;
+        const size_t n            = ep.get_coefficients()[0].get_baseline().size();
+        apply_dampings(ep.get_coefficients(), shared_contacts.get_dampings(), [n](const Vector& v) {
+            return make_mobility_damping_vector(n, v);
+        });
+        ep.set_dynamic_npis_infected(shared_npis);
+        sampled.add_edge(e.start_node_idx, e.end_node_idx, ep);
+    }
+    return sampled;
+}
+
 // Simulation of one model; results are daily values (see the header comment).
 template <class FP>
 class Simulation
@@ -1368,7 +2205,7 @@ public:
         , m_t0(t0)
         , m_dt(dt)
     {
-        m_result.add_time_point(t0, model.populations.data());
+        m_result.add_time_point(t0, model.populations.get_compartments());
     }
     Simulation(const Simulation& o)
         : m_model(o.m_model)
@@ -1517,7 +2354,8 @@ struct GraphArrays {
             throw Error(MGO_ERR_UNSUPPORTED, "number of age groups outside 1.." + std::to_string(MGO_MAX_GROUPS));
         }
         const auto& p = m.parameters;
-        scalars.insert(scalars.end(), p.scalars().begin(), p.scalars().end());
+        const auto sc = p.scalars();
+        scalars.insert(scalars.end(), sc.begin(), sc.end());
         for (int k = 0; k < MGO_NUM_AGE_PARAMS; ++k) {
             for (int a = 0; a < G; ++a) {
                 ages.push_back(p.age_param(k)[AgeGroup(size_t(a))]);
@@ -1577,7 +2415,8 @@ struct GraphArrays {
             }
         }
         doffs.push_back(int32_t(dtimes.size()));
-        pops.insert(pops.end(), m.populations.data(), m.populations.data() + m.populations.numel());
+        const auto comp = m.populations.get_compartments();
+        pops.insert(pops.end(), comp.begin(), comp.end());
         ++n;
     }
 
@@ -2003,6 +2842,24 @@ template <class FP>
 GraphSimulation<FP, OsecirGraph<FP>> make_mobility_sim(FP t0, FP dt, const OsecirGraph<FP>& g)
 {
     return GraphSimulation<FP, OsecirGraph<FP>>(t0, dt, g);
+}
+
+// make_sampled_graph_simulation (compartments/parameter_studies.h:382-398): the simulation graph of a sampled
+// parameter graph, every node starting at t0 with step dt_node_sim, exchanged every dt_graph_sim
+template <class FP, class Sim = osecir::Simulation<FP>>
+GraphSimulation<FP, OsecirGraph<FP>>
+make_sampled_graph_simulation(const Graph<osecir::Model<FP>, MobilityParameters<FP>>& sampled_graph, FP t0,
+                              FP dt_node_sim, FP dt_graph_sim)
+{
+    static_assert(std::is_same_v<Sim, osecir::Simulation<FP>>, "the device path runs osecir::Simulation");
+    OsecirGraph<FP> g;
+    for (const auto& node : sampled_graph.nodes()) {
+        g.add_node(node.id, node.property, t0, dt_node_sim);
+    }
+    for (const auto& e : sampled_graph.edges()) {
+        g.add_edge(e.start_node_idx, e.end_node_idx, e.property);
+    }
+    return make_mobility_sim<FP>(t0, dt_graph_sim, std::move(g));
 }
 
 namespace osecir

@@ -15,10 +15,9 @@
 //                                       _json.py for the same layouts with their reference lines)
 //
 // The reference builds these files with jsoncpp; this header has its own small JSON reader and writer so
-// that it needs nothing beyond the standard library. Differences a caller sees: distributions of uncertain
-// values are not kept (the mio:: layer holds plain values; reading skips them), and the parameter-study
-// inputs of the contact patterns (school holidays and their damping, contact dampings to sample) are
-// written with their default values and skipped when read.
+// that it needs nothing beyond the standard library. Distributions of uncertain values and the
+// parameter-study inputs of the contact patterns (contact dampings to sample, school holidays and their
+// damping) are read and written as the reference does.
 #ifndef MGO_MIO_IO_HPP
 #define MGO_MIO_IO_HPP
 
@@ -734,20 +733,113 @@ inline const Value& list(const Value& v, const std::string& where)
     return v;
 }
 
-inline Value uv(double v, int flags)
+// distributions (utils/parameter_distributions.h:363-432, 505-559, 630-690, 750-808, 865-920, 925-958):
+// {"Type", the type's fields, "PredefinedSamples"}
+inline Value distribution(const ParameterDistribution& d)
+{
+    Value o = Value::object();
+    if (auto* n = dynamic_cast<const ParameterDistributionNormal*>(&d)) {
+        o.set("Type", Value::string("Normal"));
+        o.set("Mean", Value::real(n->get_mean()));
+        o.set("StandardDev", Value::real(n->get_standard_dev()));
+        o.set("LowerBound", Value::real(n->get_lower_bound()));
+        o.set("UpperBound", Value::real(n->get_upper_bound()));
+        o.set("Quantile", Value::real(n->get_quantile()));
+    }
+    else if (auto* u = dynamic_cast<const ParameterDistributionUniform*>(&d)) {
+        o.set("Type", Value::string("Uniform"));
+        o.set("LowerBound", Value::real(u->get_lower_bound()));
+        o.set("UpperBound", Value::real(u->get_upper_bound()));
+    }
+    else if (auto* l = dynamic_cast<const ParameterDistributionLogNormal*>(&d)) {
+        o.set("Type", Value::string("LogNormal"));
+        o.set("LogMean", Value::real(l->get_log_mean()));
+        o.set("LogStddev", Value::real(l->get_log_stddev()));
+    }
+    else if (auto* e = dynamic_cast<const ParameterDistributionExponential*>(&d)) {
+        o.set("Type", Value::string("Exponential"));
+        o.set("Rate", Value::real(e->get_rate()));
+    }
+    else if (auto* c = dynamic_cast<const ParameterDistributionConstant*>(&d)) {
+        o.set("Type", Value::string("Constant"));
+        o.set("Constant", Value::real(c->get_constant()));
+    }
+    else {
+        throw Fail{StatusCode::InvalidType, "distribution of an unknown type"};
+    }
+    Value pre = Value::array();
+    for (double v : d.get_predefined_samples()) {
+        pre.append(Value::real(v));
+    }
+    o.set("PredefinedSamples", std::move(pre));
+    return o;
+}
+inline std::unique_ptr<ParameterDistribution> distribution_read(const Value& o, const std::string& where)
+{
+    const Value& t = get(o, "Type", where);
+    if (t.kind() != Value::String) {
+        throw Fail{StatusCode::InvalidType, where + ": distribution Type must be a string"};
+    }
+    auto f = [&](const char* k) {
+        return num(get(o, k, where), where);
+    };
+    std::unique_ptr<ParameterDistribution> d;
+    const std::string& type = t.as_string();
+    if (type == "Normal") {
+        // the five-argument constructor, as the reference's reader (it adapts mean and spread to the bounds)
+        d = std::make_unique<ParameterDistributionNormal>(f("LowerBound"), f("UpperBound"), f("Mean"),
+                                                          f("StandardDev"), f("Quantile"));
+    }
+    else if (type == "Uniform") {
+        d = std::make_unique<ParameterDistributionUniform>(f("LowerBound"), f("UpperBound"));
+    }
+    else if (type == "LogNormal") {
+        d = std::make_unique<ParameterDistributionLogNormal>(f("LogMean"), f("LogStddev"));
+    }
+    else if (type == "Exponential") {
+        d = std::make_unique<ParameterDistributionExponential>(f("Rate"));
+    }
+    else if (type == "Constant") {
+        d = std::make_unique<ParameterDistributionConstant>(f("Constant"));
+    }
+    else {
+        throw Fail{StatusCode::InvalidValue, where + ": Type of ParameterDistribution " + type + " not valid."};
+    }
+    for (const Value& v : list(get(o, "PredefinedSamples", where), where).items()) {
+        d->add_predefined_sample(num(v, where));
+    }
+    return d;
+}
+
+// uncertain values (utils/uncertain_value.h:195-250): {"Value"[, "Distribution"]}
+inline Value uv(const UncertainValue<double>& v, int flags)
 {
     Value o = Value::object();
     if (!(flags & IOF_OmitValues)) {
-        o.set("Value", Value::real(v));
+        o.set("Value", Value::real(v.value()));
+    }
+    if (!(flags & IOF_OmitDistributions) && v.get_distribution()) {
+        o.set("Distribution", distribution(*v.get_distribution()));
     }
     return o;
 }
-inline double uv_read(const Value& o, int flags, const std::string& where)
+inline void uv_read(UncertainValue<double>& dst, const Value& o, int flags, const std::string& where)
 {
     if (o.kind() != Value::Object) {
         throw Fail{StatusCode::InvalidType, where + ": uncertain value expected"};
     }
-    return (flags & IOF_OmitValues) ? 0.0 : num(get(o, "Value", where), where);
+    dst = UncertainValue<double>((flags & IOF_OmitValues) ? 0.0 : num(get(o, "Value", where), where));
+    if (!(flags & (IOF_OmitValues | IOF_OmitDistributions))) {
+        if (const Value* d = o.find("Distribution")) {
+            dst.set_distribution(*distribution_read(*d, where));
+        }
+    }
+}
+inline UncertainValue<double> uv_read(const Value& o, int flags, const std::string& where)
+{
+    UncertainValue<double> v;
+    uv_read(v, o, flags, where);
+    return v;
 }
 
 inline Value matrix(size_t rows, size_t cols, const double* rowmajor)
@@ -827,7 +919,7 @@ DampingSampling<FP> damping_sampling_read(const Value& o, int flags, const std::
         }
         mats.push_back(size_t(i));
     }
-    return DampingSampling<FP>(uv_read(get(o, "Value", where), flags, where),
+    return DampingSampling<FP>(UncertainValue<FP>(uv_read(get(o, "Value", where), flags, where)),
                                DampingLevel(int(integer(get(o, "Level", where), where))),
                                DampingType(int(integer(get(o, "Type", where), where))),
                                SimulationTime(num(get(o, "Time", where), where)), std::move(mats),
@@ -908,19 +1000,20 @@ Value model(const osecir::Model<FP>& m, int flags)
     const auto& p = m.parameters;
     const size_t G = m.get_num_groups();
     Value par      = Value::object();
-    par.set("StartDay", Value::real(p.scalars()[MGO_P_START_DAY]));
+    par.set("StartDay", Value::real(p.template get<osecir::StartDay<FP>>()));
     const std::pair<const char*, int> scalars[] = {{"Seasonality", MGO_P_SEASONALITY},
                                                    {"ICUCapacity", MGO_P_ICU_CAPACITY},
                                                    {"TestAndTraceCapacity", MGO_P_TEST_AND_TRACE_CAPACITY},
                                                    {"TestAndTraceCapacityMaxRisk", MGO_P_TEST_AND_TRACE_MAX_RISK}};
     for (const auto& s : scalars) {
-        par.set(s.first, uv(p.scalars()[size_t(s.second)], flags));
+        par.set(s.first, uv(p.scalar_value(s.second), flags));
     }
-    // ContactPatterns: the group, then (unless distributions are omitted) the sampling inputs at their
-    // defaults (uncertain_matrix.h:48-56, 209-218)
-    Value cp   = Value::object();
-    Value mats = Value::array();
-    const auto& grp = p.template get<osecir::ContactPatterns<FP>>();
+    // ContactPatterns: the group, then (unless distributions are omitted) the sampling inputs
+    // (uncertain_matrix.h:209-218)
+    Value cp        = Value::object();
+    Value mats      = Value::array();
+    const auto& ucm = p.template get<osecir::ContactPatterns<FP>>();
+    const auto& grp = ucm.get_cont_freq_mat();
     for (size_t k = 0; k < grp.get_num_matrices(); ++k) {
         mats.append(contact_matrix(grp[k]));
     }
@@ -928,12 +1021,20 @@ Value model(const osecir::Model<FP>& m, int flags)
     cmg.set("Matrices", std::move(mats));
     cp.set("ContactMatrix", std::move(cmg));
     if (!(flags & IOF_OmitDistributions)) {
-        cp.set("SchoolHolidayDamping",
-               damping_sampling(DampingSampling<FP>(0.0, DampingLevel(0), DampingType(0), SimulationTime(0.0), {},
-                                                    Vector(G, 0.0)),
-                                flags));
-        cp.set("SchoolHolidays", Value::array());
-        cp.set("Dampings", Value::array());
+        cp.set("SchoolHolidayDamping", damping_sampling(ucm.get_school_holiday_damping(), flags));
+        Value hol = Value::array();
+        for (const auto& h : ucm.get_school_holidays()) {
+            Value pr = Value::object();
+            pr.set("Element0", Value::real(h.first.get()));
+            pr.set("Element1", Value::real(h.second.get()));
+            hol.append(std::move(pr));
+        }
+        cp.set("SchoolHolidays", std::move(hol));
+        Value ds = Value::array();
+        for (const auto& d : ucm.get_dampings()) {
+            ds.append(damping_sampling(d, flags));
+        }
+        cp.set("Dampings", std::move(ds));
     }
     par.set("ContactPatterns", std::move(cp));
     par.set("DynamicNPIsInfectedSymptoms",
@@ -971,8 +1072,10 @@ Value model(const osecir::Model<FP>& m, int flags)
     dims.set("Indices", std::move(idx));
     pops.set("Dimensions", std::move(dims));
     Value el = Value::array();
-    for (size_t i = 0; i < m.populations.numel(); ++i) {
-        el.append(uv(m.populations.data()[i], flags));
+    for (size_t g = 0; g < G; ++g) {
+        for (int c = 0; c < MGO_NUM_COMPARTMENTS; ++c) {
+            el.append(uv(m.populations[{AgeGroup(g), osecir::InfectionState(c)}], flags));
+        }
     }
     pops.set("Elements", std::move(el));
     Value o = Value::object();
@@ -998,19 +1101,18 @@ osecir::Model<FP> model_read(const Value& o, int flags, const std::string& where
     osecir::Model<FP> m{int(G)};
     for (long long g = 0; g < G; ++g) {
         for (long long c = 0; c < C; ++c) {
-            m.populations[{AgeGroup(size_t(g)), osecir::InfectionState(c)}] =
-                uv_read(pel[size_t(g * C + c)], flags, where);
+            uv_read(m.populations[{AgeGroup(size_t(g)), osecir::InfectionState(c)}], pel[size_t(g * C + c)], flags,
+                    where);
         }
     }
     const Value& par = get(o, "Parameters", where);
     auto& p          = m.parameters;
     p.template get<osecir::StartDay<FP>>()    = num(get(par, "StartDay", where), where);
-    p.template get<osecir::Seasonality<FP>>() = uv_read(get(par, "Seasonality", where), flags, where);
-    p.template get<osecir::ICUCapacity<FP>>() = uv_read(get(par, "ICUCapacity", where), flags, where);
-    p.template get<osecir::TestAndTraceCapacity<FP>>() =
-        uv_read(get(par, "TestAndTraceCapacity", where), flags, where);
-    p.template get<osecir::TestAndTraceCapacityMaxRisk<FP>>() =
-        uv_read(get(par, "TestAndTraceCapacityMaxRisk", where), flags, where);
+    uv_read(p.template get<osecir::Seasonality<FP>>(), get(par, "Seasonality", where), flags, where);
+    uv_read(p.template get<osecir::ICUCapacity<FP>>(), get(par, "ICUCapacity", where), flags, where);
+    uv_read(p.template get<osecir::TestAndTraceCapacity<FP>>(), get(par, "TestAndTraceCapacity", where), flags, where);
+    uv_read(p.template get<osecir::TestAndTraceCapacityMaxRisk<FP>>(), get(par, "TestAndTraceCapacityMaxRisk", where),
+            flags, where);
     auto age = [&](const char* name, osecir::AgeArray& dst) {
         const Value& a  = get(par, name, where);
         const Value& el = list(get(a, "Elements", where), where);
@@ -1018,7 +1120,7 @@ osecir::Model<FP> model_read(const Value& o, int flags, const std::string& where
             throw Fail{StatusCode::OutOfRange, where + ": " + name + " does not have one value per age group"};
         }
         for (long long g = 0; g < G; ++g) {
-            dst[AgeGroup(size_t(g))] = uv_read(el[size_t(g)], flags, where);
+            uv_read(dst[AgeGroup(size_t(g))], el[size_t(g)], flags, where);
         }
     };
     age("TimeExposed", p.template get<osecir::TimeExposed<FP>>());
@@ -1047,13 +1149,18 @@ osecir::Model<FP> model_read(const Value& o, int flags, const std::string& where
             throw Fail{StatusCode::OutOfRange, where + ": contact matrix size differs from the age groups"};
         }
     }
+    auto& ucm = p.template get<osecir::ContactPatterns<FP>>();
+    ucm       = UncertainContactMatrix<FP>(grp);
     if (!(flags & IOF_OmitDistributions)) {
-        // the sampling inputs must be present, as for the reference's reader; their values are not kept
-        damping_sampling_read<FP>(get(cp, "SchoolHolidayDamping", where), flags, where);
-        list(get(cp, "SchoolHolidays", where), where);
-        list(get(cp, "Dampings", where), where);
+        ucm.get_school_holiday_damping() = damping_sampling_read<FP>(get(cp, "SchoolHolidayDamping", where), flags, where);
+        for (const Value& h : list(get(cp, "SchoolHolidays", where), where).items()) {
+            ucm.get_school_holidays().emplace_back(SimulationTime(num(get(h, "Element0", where), where)),
+                                                   SimulationTime(num(get(h, "Element1", where), where)));
+        }
+        for (const Value& d : list(get(cp, "Dampings", where), where).items()) {
+            ucm.get_dampings().push_back(damping_sampling_read<FP>(d, flags, where));
+        }
     }
-    p.template get<osecir::ContactPatterns<FP>>() = grp;
     p.template get<osecir::DynamicNPIsInfectedSymptoms<FP>>() =
         dynamic_npis_read<FP>(get(par, "DynamicNPIsInfectedSymptoms", where), flags, where);
     return m;

@@ -3,9 +3,10 @@
 Reference bindings mirrored (pycode/memilio-simulation/memilio/simulation/bindings/...):
 AgeGroup, Damping, ContactMatrix, ContactMatrixGroup, UncertainContactMatrix, MobilityParameters,
 TimeSeries (get_num_time_points / get_time / get_value / get_last_value / get_last_time / as_ndarray),
-set_log_level / LogLevel and seed_random_number_generator (accepted for call compatibility; the engine
-logs nothing and draws no random numbers). The model, graph and simulation types live in
-`memilio_amd.simulation.osecir`.
+UncertainValue, DampingSampling, DynamicNPIs, the parameter distributions and
+seed_random_number_generator (numpy's generator: samples follow the reference's distributions, not its
+random values), set_log_level / LogLevel (accepted for call compatibility; nothing is logged). The model,
+graph, simulation and sampling functions live in `memilio_amd.simulation.osecir`.
 
 A script written against the reference switches with
     import memilio_amd.simulation as mio
@@ -209,26 +210,60 @@ class ParameterDistributionConstant(ParameterDistribution):
 
 
 def distribution_from_json(js):
-    """A distribution object from its JSON, or the JSON itself for the types not sampled here (LogNormal,
-    Exponential): kept so that files round-trip."""
+    """A distribution object from its JSON (parameter_distributions.h:925-958); a missing field raises KeyError,
+    an unknown Type ValueError."""
     if not isinstance(js, dict):
-        return js
-    t = js.get("Type")
+        raise ValueError("a distribution must be a JSON object")
+    t = js["Type"]
     if t == "Normal":
-        d = ParameterDistributionNormal.__new__(ParameterDistributionNormal)
-        ParameterDistribution.__init__(d)
-        d.mean, d.standard_dev = float(js["Mean"]), float(js["StandardDev"])
-        d.lower_bound, d.upper_bound = float(js["LowerBound"]), float(js["UpperBound"])
-        d.quantile = float(js["Quantile"])
+        # the five-argument constructor, as the reference's reader (it adapts mean and spread to the bounds)
+        d = ParameterDistributionNormal(float(js["LowerBound"]), float(js["UpperBound"]), float(js["Mean"]),
+                                        float(js["StandardDev"]), float(js["Quantile"]))
     elif t == "Uniform":
         d = ParameterDistributionUniform(js["LowerBound"], js["UpperBound"])
+    elif t == "LogNormal":
+        d = ParameterDistributionLogNormal(js["LogMean"], js["LogStddev"])
+    elif t == "Exponential":
+        d = ParameterDistributionExponential(js["Rate"])
     elif t == "Constant":
         d = ParameterDistributionConstant(js["Constant"])
     else:
-        return js
-    for v in js.get("PredefinedSamples", []):
+        raise ValueError(f"Type of ParameterDistribution {t} not valid.")
+    for v in js["PredefinedSamples"]:
         d.add_predefined_sample(v)
     return d
+
+
+class ParameterDistributionLogNormal(ParameterDistribution):
+    """mio::ParameterDistributionLogNormal (parameter_distributions.h:566-690): exp of a normal sample."""
+
+    type_name = "LogNormal"
+
+    def __init__(self, log_mean, log_stddev):
+        super().__init__()
+        self.log_mean, self.log_stddev = float(log_mean), float(log_stddev)
+
+    def get_rand_sample(self, gen):
+        return float(gen.lognormal(self.log_mean, self.log_stddev))
+
+    def _json_fields(self):
+        return {"LogMean": self.log_mean, "LogStddev": self.log_stddev}
+
+
+class ParameterDistributionExponential(ParameterDistribution):
+    """mio::ParameterDistributionExponential (parameter_distributions.h:693-808)."""
+
+    type_name = "Exponential"
+
+    def __init__(self, rate):
+        super().__init__()
+        self.rate = float(rate)
+
+    def get_rand_sample(self, gen):
+        return float(gen.exponential(1.0 / self.rate))
+
+    def _json_fields(self):
+        return {"Rate": self.rate}
 
 
 def distribution_to_json(d):

@@ -23,9 +23,9 @@ reference's serializers, JSON flavour (cpp/memilio/io/json_serializer.h, io.h:58
                                                                       models/ode_secir/model.h:219-225
   MobilityParameters    {"Coefficients": group, "DynamicNPIs": npis}  metapopulation_mobility_instant.h:259-264
 
-Distributions of uncertain values become ParameterDistributionNormal / Uniform / Constant objects (the types
-draw_sample uses); other types (LogNormal, Exponential) are kept as the JSON they were read from and written
-back unchanged. Reading is as strict as the reference's: a missing element is an error.
+Distributions of uncertain values become ParameterDistribution objects (Normal, Uniform, LogNormal,
+Exponential, Constant; parameter_distributions.h:925-958). Reading is as strict as the reference's: a
+missing element is an error.
 """
 import json
 import os
@@ -80,7 +80,10 @@ def uv_from_json(js, flags=IOF_None, where="UncertainValue"):
     v = 0.0 if flags & IOF_OmitValues else _number(_expect(js, "Value", where), where)
     dist = None
     if not flags & (IOF_OmitValues | IOF_OmitDistributions) and js.get("Distribution") is not None:
-        dist = distribution_from_json(js["Distribution"])
+        try:
+            dist = distribution_from_json(js["Distribution"])
+        except (KeyError, ValueError, TypeError) as e:
+            raise JsonFormatError(f"{where}: distribution: {e}") from None
     return v, dist
 
 

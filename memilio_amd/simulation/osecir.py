@@ -208,7 +208,8 @@ class Parameters:
         """Clamp parameters to admissible values as osecir::Parameters::apply_constraints
         (parameters.h:465-612): seasonality outside [0, 0.5] -> 0, negative capacities -> 0, stay times
         below 0.1 -> 0.1, RelativeTransmissionNoSymptoms < 0 -> 0, the other probabilities outside [0, 1]
-        -> 0 (MaxRiskOfInfectionFromSymptomatic is not checked). Returns True if anything was corrected."""
+        -> 0 (MaxRiskOfInfectionFromSymptomatic is not checked), DeathsPerSevere -> 0 where
+        CriticalPerSevere + DeathsPerSevere > 1. Returns True if anything was corrected."""
         corrected = False
         sc, a = self._sc, self._age
         if not 0.0 <= sc[capi.P_SEASONALITY] <= 0.5:
@@ -238,6 +239,10 @@ class Parameters:
             if bad.any():
                 a[k][bad] = 0.0
                 corrected = True
+        bad = a[capi.A_CRITICAL_PER_SEVERE] + a[capi.A_DEATHS_PER_SEVERE] > 1.0
+        if bad.any():
+            a[capi.A_DEATHS_PER_SEVERE][bad] = 0.0
+            corrected = True
         return corrected
 
 
@@ -678,8 +683,8 @@ def draw_sample(model_or_graph):
 
 def set_params_distributions_normal(model, t0, tmax, dev_rel):
     """osecir::set_params_distributions_normal (parameter_space.h:46-118): a truncated normal distribution
-    around every sampled value within (1 -+ 2.6 dev_rel) * value, and a contact damping of value 0.5 (with
-    such a distribution) at the middle of [t0, tmax] on every contact matrix. The reference passes
+    around every sampled value within (1 -+ 2.6 dev_rel) * value, and a contact damping of value 0.5 at the
+    middle of [t0, tmax] on every contact matrix. The reference passes
     dev_rel * value in the place of the quantile of ParameterDistributionNormal(lb, ub, mean, quantile) (its
     four-argument form); that is kept, so the distributions are the ones the reference samples."""
     big = np.finfo(np.float64).max
@@ -689,7 +694,8 @@ def set_params_distributions_normal(model, t0, tmax, dev_rel):
         v = uv.value
         lb = min(max(min_val, (1 - dev_rel * 2.6) * v), 0.1 * big)
         ub = min(max(min_val, (1 + dev_rel * 2.6) * v), 0.5 * big)
-        if abs(lb - ub) <= 1e-12:
+        diff = abs(lb - ub)
+        if diff <= 1e-12 or diff <= max(abs(lb), abs(ub)) * np.finfo(np.float64).tiny:
             uv.set_distribution(ParameterDistributionConstant(min(max(min_val, v), 0.3 * big)))
         else:
             uv.set_distribution(ParameterDistributionNormal(lb, ub, min(max(min_val, v), 0.3 * big),
@@ -714,7 +720,9 @@ def set_params_distributions_normal(model, t0, tmax, dev_rel):
     cp = p.ContactPatterns
     cp.dampings.append(DampingSampling(0.5, 0, 0, float(t0) + (float(tmax) - float(t0)) / 2,
                                        list(range(cp.cont_freq_mat.num_matrices)), np.ones(p.get_num_groups())))
-    set_distribution(cp.dampings[-1].value, 0.0)
+    # the reference gives the distribution to the first contact damping sampling (the new one only if there
+    # was none before)
+    set_distribution(cp.dampings[0].value, 0.0)
 
 
 class ParameterStudy:

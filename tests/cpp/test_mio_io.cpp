@@ -155,11 +155,9 @@ void compare(const Graph& a, const Graph& b)
                       "age parameter " + std::to_string(k));
             }
         }
-        for (size_t i = 0; i < ma.populations.numel(); ++i) {
-            check(ma.populations.data()[i] == mb.populations.data()[i], "population " + std::to_string(i));
-        }
-        const auto& ca = ma.parameters.get<mio::osecir::ContactPatterns<double>>();
-        const auto& cb = mb.parameters.get<mio::osecir::ContactPatterns<double>>();
+        check(ma.populations.get_compartments() == mb.populations.get_compartments(), "populations");
+        const auto& ca = ma.parameters.get<mio::osecir::ContactPatterns<double>>().get_cont_freq_mat();
+        const auto& cb = mb.parameters.get<mio::osecir::ContactPatterns<double>>().get_cont_freq_mat();
         check(ca.get_num_matrices() == cb.get_num_matrices(), "contact matrices");
         for (size_t k = 0; k < ca.get_num_matrices(); ++k) {
             check(ca[k].get_baseline() == cb[k].get_baseline() && ca[k].get_minimum() == cb[k].get_minimum(),

@@ -157,7 +157,7 @@ static mio::Graph<mio::osecir::Model<double>, mio::MobilityParameters<double>> r
                 M(size_t(i), size_t(j)) = 0.2 * U(rng);
             }
         }
-        auto& cm = p.get<mio::osecir::ContactPatterns<double>>();
+        mio::ContactMatrixGroup& cm = p.get<mio::osecir::ContactPatterns<double>>();
         cm[0]    = mio::ContactMatrix(B, M);
         cm[0].add_damping(0.3, mio::DampingLevel(0), mio::DampingType(0), mio::SimulationTime(5.0));
         cm[0].add_damping(0.2, mio::DampingLevel(1), mio::DampingType(0), mio::SimulationTime(9.0));
@@ -189,7 +189,7 @@ static Graph sample_graph(const mio::Graph<mio::osecir::Model<double>, mio::Mobi
     Graph g;
     for (const auto& node : pg.nodes()) {
         auto m   = node.property;
-        auto& cm = m.parameters.get<mio::osecir::ContactPatterns<double>>();
+        mio::ContactMatrixGroup& cm = m.parameters.get<mio::osecir::ContactPatterns<double>>();
         auto& B  = cm[0].get_baseline();
         const double f = U(rng);
         for (size_t i = 0; i < B.rows(); ++i) {
@@ -309,6 +309,36 @@ int main()
             e = std::max(e, compare_graph(g, 0.0, 0.5, 20.0));
         }
         std::printf("parameter study (16 runs): max rel err %.3e\n", e);
+        CHECK(e < 1e-6);
+    }
+    // 3b. ParameterStudy::run with the reference's sampling in create_simulation (draw_sample of a copy of the
+    //     parameter graph, make_sampled_graph_simulation): each run's result equals the oracle of its sample
+    {
+        auto pg = random_params_graph(10, 2, 13);
+        for (auto& node : pg.nodes()) {
+            mio::osecir::set_params_distributions_normal(node.property, 0.0, 20.0, 0.2);
+        }
+        mio::thread_local_rng().seed({7});
+        std::vector<double> texp;
+        mio::ParameterStudy<decltype(pg), double> study(pg, 0.0, 20.0, 0.5, 8);
+        auto results = study.run(
+            [&texp](const auto& params, double t0, double dt, size_t) {
+                auto copy    = params;
+                auto sampled = mio::osecir::draw_sample(copy);
+                texp.push_back(sampled.nodes()[0].property.parameters.template get<mio::osecir::TimeExposed<double>>()[AgeGroup(0)]);
+                return mio::make_sampled_graph_simulation<double>(sampled, t0, dt, dt);
+            },
+            [](auto&& sim, size_t) {
+                return std::move(sim).get_graph();
+            });
+        CHECK(results.size() == 8);
+        double e = 0;
+        for (auto& g : results) {
+            e = std::max(e, compare_graph(g, 0.0, 0.5, 20.0));
+        }
+        std::sort(texp.begin(), texp.end());
+        CHECK(std::adjacent_find(texp.begin(), texp.end()) == texp.end()); // every run drew its own sample
+        std::printf("sampled parameter study (8 runs): max rel err %.3e\n", e);
         CHECK(e < 1e-6);
     }
     // 4. osecir::simulate with RKIntegratorCore

@@ -139,8 +139,20 @@ def test_distribution_json():
     assert isinstance(back, mio.ParameterDistributionNormal) and back.to_json() == js
     for d in (mio.ParameterDistributionUniform(1.0, 2.0), mio.ParameterDistributionConstant(4.0)):
         assert mio.distribution_from_json(d.to_json()).to_json() == d.to_json()
-    other = {"Type": "LogNormal", "LogMean": 0.0, "LogStddev": 0.25, "PredefinedSamples": []}
-    assert mio.distribution_from_json(other) == other
+    ln = {"Type": "LogNormal", "LogMean": 0.0, "LogStddev": 0.25, "PredefinedSamples": [-5.0]}
+    d = mio.distribution_from_json(ln)
+    assert isinstance(d, mio.ParameterDistributionLogNormal) and d.to_json() == ln
+    assert d.get_sample() == -5.0 and d.get_sample() > 0
+    ex = {"Type": "Exponential", "Rate": 2.0, "PredefinedSamples": []}
+    assert mio.distribution_from_json(ex).to_json() == ex
+    with pytest.raises(ValueError, match="not valid"):
+        mio.distribution_from_json({"Type": "Poisson", "PredefinedSamples": []})
+    with pytest.raises(KeyError):
+        mio.distribution_from_json({"Type": "Uniform", "LowerBound": 0.0, "PredefinedSamples": []})
+    # the reader uses the five-argument constructor: a written spread too wide for the bounds is adapted
+    wide = {"Type": "Normal", "Mean": 5.0, "StandardDev": 10.0, "LowerBound": 1.0, "UpperBound": 10.0,
+            "Quantile": Q, "PredefinedSamples": []}
+    assert mio.distribution_from_json(wide).standard_dev == pytest.approx(4.0 / Q, rel=1e-15)
 
 
 # ----------------------------------------------------------------------------------------- uncertain values
@@ -469,3 +481,23 @@ def test_sampled_study_graph_is_reproducible():
                                       b.get_node(v).property.populations.get_compartments())
         assert a.get_node(v).property.parameters.TimeExposed[0].value == \
             b.get_node(v).property.parameters.TimeExposed[0].value
+
+
+def test_cpp_layer_sampling(tmp_path):
+    """The same rules through the C++ mio:: layer (tests/cpp/test_mio_sampling.cpp: distributions,
+    UncertainValue, UncertainContactMatrix, set_params_distributions_normal, draw_sample of a model and a
+    graph, distributions and sampling inputs in the graph files)."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(__file__), "cpp", "test_mio_sampling")
+    assert os.path.exists(exe), "tests/cpp/test_mio_sampling is built by __graft_entry__.build()"
+    r = subprocess.run([exe, str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    # the C++ files hold the distributions; the Python reader takes them as distribution objects
+    g = osecir.read_graph(str(tmp_path))
+    d = g.get_node(0).property.parameters.TimeExposed[0].get_distribution()
+    assert isinstance(d, mio.ParameterDistributionNormal) and d.get_predefined_samples() == [3.3]
+    assert isinstance(g.get_node(1).property.populations[2, S.Recovered].get_distribution(),
+                      mio.ParameterDistributionLogNormal)
+    cp = g.get_node(1).property.parameters.ContactPatterns
+    assert cp.school_holidays == [(10.0, 20.0)] and len(cp.dampings) == 1
