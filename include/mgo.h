@@ -308,6 +308,18 @@ mgo_status mgo_batch_ensemble_percentiles(mgo_context* ctx, mgo_batch_id id, int
                                           const double* p, int32_t num_p, double* out);
 mgo_status mgo_batch_sum_nodes(mgo_context* ctx, mgo_batch_id id, int32_t first_run, int32_t num_runs, double* out);
 
+/* The same statistics over results held in host memory (ensemble_mean / ensemble_percentile of
+ * cpp/memilio/data/analyze_result.h:289-343 applied to time series the caller holds, e.g. the interpolated
+ * results of a parameter study): values[run][position] with num_positions positions per run (for the
+ * reference's [run][node] time series: position = (node * num_time_points + time) * num_elements + element).
+ * The values are copied to the device and reduced there by the kernels of mgo_batch_ensemble_*; results
+ * have one run's layout: mean_out[position], out[q][position]. Percentiles: each p[q] in (0, 1), at most
+ * 8192 runs. */
+mgo_status mgo_ensemble_mean_values(mgo_context* ctx, const double* values, int32_t num_runs, int64_t num_positions,
+                                    double* mean_out);
+mgo_status mgo_ensemble_percentiles_values(mgo_context* ctx, const double* values, int32_t num_runs,
+                                           int64_t num_positions, const double* p, int32_t num_p, double* out);
+
 /* ---------------------------------------------------------------- serving loop
  * A worker thread owned by the context runs half-day cycles over the resident requests; requests are
  * admitted as HBM budget and slots free up (continuous batching), so requests submitted while others

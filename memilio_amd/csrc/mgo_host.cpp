@@ -2163,6 +2163,22 @@ static void download(Batch& b, const DevBuf& src, size_t doubles, double* dst)
     hip_check(hipStreamSynchronize(b.stream), "download");
 }
 
+// percentile ranks floor(num_runs * p) (analyze_result.h:339), checked
+static std::vector<int32_t> percentile_ranks(int32_t num_runs, const double* p, int32_t num_p)
+{
+    if (num_runs > 8192) {
+        fail(MGO_ERR_UNSUPPORTED, "percentiles over more than 8192 runs");
+    }
+    std::vector<int32_t> ranks;
+    for (int q = 0; q < num_p; ++q) {
+        if (!(p[q] > 0.0 && p[q] < 1.0)) {
+            fail(MGO_ERR_INVALID_VALUE, "percentile %g outside (0, 1)", p[q]);
+        }
+        ranks.push_back(int32_t(static_cast<size_t>(double(num_runs) * p[q])));
+    }
+    return ranks;
+}
+
 mgo_status mgo_batch_ensemble_mean(mgo_context* ctx, mgo_batch_id id, int32_t first_run, int32_t num_runs,
                                    double* mean_out)
 {
@@ -2190,16 +2206,7 @@ mgo_status mgo_batch_ensemble_percentiles(mgo_context* ctx, mgo_batch_id id, int
         if (!out || !p || num_p < 1) {
             fail(MGO_ERR_INVALID_VALUE, "percentiles or output missing");
         }
-        if (num_runs > 8192) {
-            fail(MGO_ERR_UNSUPPORTED, "percentiles over more than 8192 runs");
-        }
-        std::vector<int32_t> ranks;
-        for (int q = 0; q < num_p; ++q) {
-            if (!(p[q] > 0.0 && p[q] < 1.0)) {
-                fail(MGO_ERR_INVALID_VALUE, "percentile %g outside (0, 1)", p[q]);
-            }
-            ranks.push_back(int32_t(static_cast<size_t>(double(num_runs) * p[q])));
-        }
+        const std::vector<int32_t> ranks = percentile_ranks(num_runs, p, num_p);
         EnsembleView v;
         ensemble_view(*b, first_run, num_runs, v);
         const int64_t positions = int64_t(v.days) * v.nodes * v.G * NC;
@@ -2210,6 +2217,66 @@ mgo_status mgo_batch_ensemble_percentiles(mgo_context* ctx, mgo_batch_id id, int
                                          sel.as<double>(), b->stream),
                   "ensemble percentiles");
         download(*b, sel, size_t(positions) * size_t(num_p), out);
+    });
+}
+
+// host values [run][position] on the device, with the per-run offsets the ensemble kernels take
+static void upload_values(mgo_context* ctx, const double* values, int32_t num_runs, int64_t positions, DevBuf& dv,
+                          DevBuf& base)
+{
+    if (!values || num_runs < 1 || positions < 1) {
+        fail(MGO_ERR_INVALID_VALUE, "values missing or empty (%d runs, %lld positions)", num_runs,
+             (long long)positions);
+    }
+    const size_t n = size_t(num_runs) * size_t(positions);
+    dv.alloc(n * sizeof(double));
+    hip_check(hipMemcpyAsync(dv.p, values, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream), "upload values");
+    std::vector<int64_t> b(static_cast<size_t>(num_runs));
+    for (int32_t r = 0; r < num_runs; ++r) {
+        b[size_t(r)] = int64_t(r) * positions;
+    }
+    base.upload(b, ctx->stream);
+}
+
+static void download_values(mgo_context* ctx, const DevBuf& src, size_t doubles, double* dst)
+{
+    hip_check(hipMemcpyAsync(dst, src.p, doubles * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "download");
+    hip_check(hipStreamSynchronize(ctx->stream), "download");
+}
+
+mgo_status mgo_ensemble_mean_values(mgo_context* ctx, const double* values, int32_t num_runs, int64_t num_positions,
+                                    double* mean_out)
+{
+    return guarded(ctx, [&] {
+        if (!mean_out) {
+            fail(MGO_ERR_INVALID_VALUE, "output is NULL");
+        }
+        DevBuf dv, base, mean;
+        upload_values(ctx, values, num_runs, num_positions, dv, base);
+        mean.alloc(size_t(num_positions) * sizeof(double));
+        hip_check(launch_ensemble_mean(dv.as<double>(), base.as<int64_t>(), num_runs, num_positions,
+                                       mean.as<double>(), ctx->stream),
+                  "ensemble mean");
+        download_values(ctx, mean, size_t(num_positions), mean_out);
+    });
+}
+
+mgo_status mgo_ensemble_percentiles_values(mgo_context* ctx, const double* values, int32_t num_runs,
+                                           int64_t num_positions, const double* p, int32_t num_p, double* out)
+{
+    return guarded(ctx, [&] {
+        if (!out || !p || num_p < 1) {
+            fail(MGO_ERR_INVALID_VALUE, "percentiles or output missing");
+        }
+        const std::vector<int32_t> ranks = percentile_ranks(num_runs, p, num_p);
+        DevBuf dv, base, dranks, sel;
+        upload_values(ctx, values, num_runs, num_positions, dv, base);
+        dranks.upload(ranks, ctx->stream);
+        sel.alloc(size_t(num_positions) * size_t(num_p) * sizeof(double));
+        hip_check(launch_ensemble_select(dv.as<double>(), base.as<int64_t>(), num_runs, num_positions,
+                                         dranks.as<int32_t>(), num_p, sel.as<double>(), ctx->stream),
+                  "ensemble percentiles");
+        download_values(ctx, sel, size_t(num_positions) * size_t(num_p), out);
     });
 }
 

@@ -43,6 +43,28 @@ class Context:
         """MGO_CTX_* flags for the runs started afterwards."""
         self.check(self.lib.mgo_context_set_flags(self.ptr, flags))
 
+    def ensemble_mean_values(self, values):
+        """ensemble_mean over the first axis of host values [run, ...], computed on the device; returns the
+        shape of one run (mgo_ensemble_mean_values)."""
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        if v.ndim < 1 or v.shape[0] < 1 or v[0].size < 1:
+            raise ValueError("ensemble_mean_values needs at least one run with values")
+        out = np.empty(v.shape[1:], dtype=np.float64)
+        self.check(self.lib.mgo_ensemble_mean_values(self.ptr, capi.dptr(v), v.shape[0], v[0].size, capi.dptr(out)))
+        return out
+
+    def ensemble_percentiles_values(self, values, ps):
+        """ensemble_percentile over the first axis of host values [run, ...] for each p in ps, computed on the
+        device; returns [len(ps), ...] (mgo_ensemble_percentiles_values)."""
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        if v.ndim < 1 or v.shape[0] < 1 or v[0].size < 1:
+            raise ValueError("ensemble_percentiles_values needs at least one run with values")
+        p = np.ascontiguousarray(np.atleast_1d(ps), dtype=np.float64)
+        out = np.empty((p.size,) + v.shape[1:], dtype=np.float64)
+        self.check(self.lib.mgo_ensemble_percentiles_values(self.ptr, capi.dptr(v), v.shape[0], v[0].size,
+                                                            capi.dptr(p), p.size, capi.dptr(out)))
+        return out
+
     def set_tile(self, tile):
         """Runs per node-major tile for ensembles created afterwards (0 = autotuned)."""
         self.check(self.lib.mgo_context_set_tile(self.ptr, tile))

@@ -23,10 +23,12 @@ from ..engine import Context
 from ..graphspec import GraphSpec, NC
 from . import (AgeGroup, ContactMatrix, ContactMatrixGroup, DampingSampling, DynamicNPIs, MobilityParameters,
                ParameterDistributionConstant, ParameterDistributionNormal, TimeSeries, UncertainContactMatrix,
-               UncertainValue, apply_dampings, interpolate_simulation_result, make_mobility_damping_vector)
+               UncertainValue, apply_dampings, make_mobility_damping_vector)
+from . import interpolate_simulation_result as _interp_ts
 
 __all__ = ["InfectionState", "Model", "ModelGraph", "MobilityGraph", "MobilitySimulation", "GraphParameterStudy",
-           "ParameterStudy", "Simulation", "simulate", "interpolate_simulation_result", "ControlledStepperWrapper",
+           "ParameterStudy", "Simulation", "simulate", "interpolate_simulation_result", "interpolate_ensemble_results",
+           "ensemble_mean", "ensemble_percentile", "ControlledStepperWrapper",
            "RKIntegratorCore", "ExplicitStepperWrapper", "EulerIntegratorCore", "write_graph", "read_graph",
            "draw_sample", "set_params_distributions_normal"]
 
@@ -786,6 +788,68 @@ class GraphParameterStudy:
         for run, sg in enumerate(graphs):
             handle_result_func(sg, run)
         return None
+
+
+# ----------------------------------------------------------------------------------------- ensemble results
+def interpolate_simulation_result(result, abs_tol=1e-10):
+    """interpolate_simulation_result of a TimeSeries, or of a MobilityGraph: the node results at the integer
+    days, one TimeSeries per node (data/analyze_result.h:186-265; binding osecir.cpp:224-227)."""
+    if isinstance(result, _Graph):
+        return [_interp_ts(result.get_node(v).property.result, abs_tol) for v in range(result.num_nodes)]
+    return _interp_ts(result, abs_tol)
+
+
+def interpolate_ensemble_results(ensemble):
+    """interpolate_ensemble_results (analyze_result.h:77-86; binding osecir.cpp:228): [run][node] TimeSeries
+    of the MobilityGraphs of a study."""
+    return [interpolate_simulation_result(g) for g in ensemble]
+
+
+def _ensemble_values(ensemble):
+    """[run][node] TimeSeries as values[run][node][time][element]; the input must be uniform as
+    interpolate_ensemble_results returns it (same nodes, time points and elements in every run)."""
+    runs = len(ensemble)
+    if runs == 0 or len(ensemble[0]) == 0:
+        raise ValueError("ensemble results are empty")
+    nodes = len(ensemble[0])
+    T = ensemble[0][0].get_num_time_points()
+    E = ensemble[0][0].get_num_elements()
+    if T == 0 or E == 0:
+        raise ValueError("ensemble results have no values")
+    v = np.empty((runs, nodes, T, E))
+    for r, run in enumerate(ensemble):
+        if len(run) != nodes:
+            raise ValueError("ensemble results not uniform: runs differ in their number of nodes")
+        for n, ts in enumerate(run):
+            if ts.get_num_time_points() != T or ts.get_num_elements() != E:
+                raise ValueError("ensemble results not uniform: time points or elements differ")
+            v[r, n] = np.stack(ts.values)
+    return v
+
+
+def _series(times, values):
+    ts = TimeSeries(values.shape[-1])
+    for t, row in zip(times, values):
+        ts.add_time_point(t, row)
+    return ts
+
+
+def ensemble_mean(ensemble):
+    """ensemble_mean (analyze_result.h:289-315; binding osecir.cpp:93): per node the mean over runs, summed in
+    run order as value / num_runs, computed on the device (mgo_ensemble_mean_values); the times are the last
+    run's, as in the reference."""
+    v = _ensemble_values(ensemble)
+    m = _context().ensemble_mean_values(v)
+    return [_series(ensemble[-1][n].times, m[n]) for n in range(v.shape[1])]
+
+
+def ensemble_percentile(ensemble, p):
+    """ensemble_percentile (analyze_result.h:316-343; binding osecir.cpp:94): per node, time and element the
+    sorted run values at index floor(num_runs * p), p in (0, 1), computed on the device
+    (mgo_ensemble_percentiles_values); the times are the first run's."""
+    v = _ensemble_values(ensemble)
+    q = _context().ensemble_percentiles_values(v, [float(p)])[0]
+    return [_series(ensemble[0][n].times, q[n]) for n in range(v.shape[1])]
 
 
 # ----------------------------------------------------------------------------------------- graph files

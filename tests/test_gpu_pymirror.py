@@ -125,3 +125,22 @@ def test_mobility_simulation_edge_dampings(oracle):
     cfg = capi.sim_cfg(t0=0.0, dt_graph=0.5, dt_node=0.1)
     ref, _ = oracle.graph_run(spec, cfg, 30.0)
     assert rel_err(_results(sim.graph), ref) < 1e-6
+
+
+def test_ensemble_statistics_of_a_study(oracle):
+    """osecir.interpolate_ensemble_results, ensemble_mean and ensemble_percentile over the graphs of a study
+    (computed on the device from the host results, mgo_ensemble_*_values) equal the oracle's restatement of
+    analyze_result.h:289-343 bitwise."""
+    graph = ps.study_graph(n=4, G=2)
+    study = osecir.GraphParameterStudy(graph, t0=0, tmax=12, dt=0.5, num_runs=7)
+    graphs = study.run(sample=ps.scale_contacts)
+    ens = osecir.interpolate_ensemble_results(graphs)
+    assert len(ens) == 7 and len(ens[0]) == 4 and ens[0][0].get_num_time_points() == 13
+    vals = [np.stack([np.stack(ts.values) for ts in run]) for run in ens]   # [run] -> [node][time][element]
+    assert not np.array_equal(vals[0], vals[1])
+    mean = osecir.ensemble_mean(ens)
+    assert len(mean) == 4 and mean[2].get_time(12) == 12
+    assert np.array_equal(np.stack([np.stack(ts.values) for ts in mean]), oracle.ensemble_mean(vals))
+    for p in (0.1, 0.5, 0.9):
+        pct = osecir.ensemble_percentile(ens, p)
+        assert np.array_equal(np.stack([np.stack(ts.values) for ts in pct]), oracle.ensemble_percentile(vals, p)), p

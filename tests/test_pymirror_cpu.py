@@ -144,3 +144,19 @@ def test_mobility_dampings_descriptor(oracle):
     with pytest.raises(ValueError):
         osecir._spec_from([extra.get_node(v).property.model for v in range(3)],
                           [(e.start_node_idx, e.end_node_idx, e.property) for e in extra._edges])
+
+
+def test_ensemble_inputs_must_be_uniform():
+    """ensemble_mean / ensemble_percentile take [run][node] series of one shape (analyze_result.h:289-343
+    asserts it); other input is refused before anything reaches the device."""
+    def ts(n):
+        s = mio.TimeSeries(2)
+        for t in range(n):
+            s.add_time_point(float(t), np.array([1.0, 2.0]))
+        return s
+    with pytest.raises(ValueError, match="not uniform"):
+        osecir.ensemble_mean([[ts(3), ts(3)], [ts(3)]])
+    with pytest.raises(ValueError, match="not uniform"):
+        osecir.ensemble_percentile([[ts(3)], [ts(4)]], 0.5)
+    with pytest.raises(ValueError, match="empty"):
+        osecir.ensemble_mean([])
