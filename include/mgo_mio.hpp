@@ -23,6 +23,8 @@
 //   osecir::simulate                     cpp/models/ode_secir/model.h:385-390
 //   ParameterStudy::run / run_serial     cpp/memilio/compartments/parameter_studies.h:47-130,156-217,288-345
 //   interpolate_simulation_result        cpp/memilio/data/analyze_result.h:186-265
+//   interpolate_ensemble_results,        analyze_result.h:77-86, 289-343 (the statistics computed on the
+//   ensemble_mean, ensemble_percentile   device by mgo_ensemble_*_values)
 //   parameter sampling                   cpp/memilio/utils/uncertain_value.h, parameter_distributions.h,
 //                                        epidemiology/uncertain_matrix.h, damping_sampling.h:255-306,
 //                                        models/ode_secir/parameter_space.h:46-251
@@ -2917,6 +2919,94 @@ std::vector<TimeSeries<FP>> interpolate_simulation_result(const Graph<Simulation
         r.push_back(interpolate_simulation_result(n.property.get_result(), abs_tol));
     }
     return r;
+}
+
+// interpolate_ensemble_results (data/analyze_result.h:77-86): per run, the interpolated node results
+template <class G>
+auto interpolate_ensemble_results(const std::vector<G>& ensemble)
+{
+    using R = decltype(interpolate_simulation_result(std::declval<const G&>()));
+    std::vector<R> r;
+    r.reserve(ensemble.size());
+    for (const auto& g : ensemble) {
+        r.push_back(interpolate_simulation_result(g));
+    }
+    return r;
+}
+
+namespace detail
+{
+// [run][node] series as values[run][node][time][element]; the input must be uniform
+template <class FP>
+std::vector<double> ensemble_values(const std::vector<std::vector<TimeSeries<FP>>>& e, size_t& nodes, size_t& T,
+                                    size_t& E)
+{
+    if (e.empty() || e[0].empty() || e[0][0].get_num_time_points() == 0 || e[0][0].get_num_elements() == 0) {
+        throw Error(MGO_ERR_INVALID_VALUE, "ensemble results are empty");
+    }
+    nodes = e[0].size();
+    T     = e[0][0].get_num_time_points();
+    E     = e[0][0].get_num_elements();
+    std::vector<double> v;
+    v.reserve(e.size() * nodes * T * E);
+    for (const auto& run : e) {
+        if (run.size() != nodes) {
+            throw Error(MGO_ERR_INVALID_VALUE, "ensemble results not uniform: runs differ in their number of nodes");
+        }
+        for (const auto& ts : run) {
+            if (ts.get_num_time_points() != T || ts.get_num_elements() != E) {
+                throw Error(MGO_ERR_INVALID_VALUE, "ensemble results not uniform: time points or elements differ");
+            }
+            for (size_t t = 0; t < T; ++t) {
+                v.insert(v.end(), ts.value_data(t), ts.value_data(t) + E);
+            }
+        }
+    }
+    return v;
+}
+
+template <class FP>
+std::vector<TimeSeries<FP>> ensemble_series(const std::vector<TimeSeries<FP>>& times_of, const double* values,
+                                            size_t nodes, size_t T, size_t E)
+{
+    std::vector<TimeSeries<FP>> r(nodes, TimeSeries<FP>(E));
+    for (size_t n = 0; n < nodes; ++n) {
+        for (size_t t = 0; t < T; ++t) {
+            r[n].add_time_point(times_of[n].get_time(t), values + (n * T + t) * E);
+        }
+    }
+    return r;
+}
+} // namespace detail
+
+// ensemble_mean (data/analyze_result.h:289-315): per node, time and element the mean over runs, summed in run
+// order as value / num_runs, computed on the device (mgo_ensemble_mean_values); the times are the last run's
+template <class FP>
+std::vector<TimeSeries<FP>> ensemble_mean(const std::vector<std::vector<TimeSeries<FP>>>& ensemble_result)
+{
+    size_t nodes = 0, T = 0, E = 0;
+    const std::vector<double> v = detail::ensemble_values(ensemble_result, nodes, T, E);
+    std::vector<double> m(nodes * T * E);
+    mgo_context* ctx = default_context();
+    detail::check(mgo_ensemble_mean_values(ctx, v.data(), int32_t(ensemble_result.size()), int64_t(m.size()), m.data()),
+                  ctx);
+    return detail::ensemble_series(ensemble_result.back(), m.data(), nodes, T, E);
+}
+
+// ensemble_percentile (data/analyze_result.h:316-343): the sorted run values at index floor(num_runs * p),
+// p in (0, 1), computed on the device (mgo_ensemble_percentiles_values); the times are the first run's
+template <class FP>
+std::vector<TimeSeries<FP>> ensemble_percentile(const std::vector<std::vector<TimeSeries<FP>>>& ensemble_result, FP p)
+{
+    size_t nodes = 0, T = 0, E = 0;
+    const std::vector<double> v = detail::ensemble_values(ensemble_result, nodes, T, E);
+    std::vector<double> q(nodes * T * E);
+    const double pd  = p;
+    mgo_context* ctx = default_context();
+    detail::check(mgo_ensemble_percentiles_values(ctx, v.data(), int32_t(ensemble_result.size()), int64_t(q.size()),
+                                                  &pd, 1, q.data()),
+                  ctx);
+    return detail::ensemble_series(ensemble_result.front(), q.data(), nodes, T, E);
 }
 
 // ---------------------------------------------------------------------------------------------- study

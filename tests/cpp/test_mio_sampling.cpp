@@ -425,6 +425,35 @@ void test_distribution_files(const std::string& dir)
           "same seed, same file: same sample");
 }
 
+// ensemble_mean / ensemble_percentile refuse non-uniform [run][node] input before anything reaches the device
+void test_ensemble_input_checks()
+{
+    auto series = [](size_t points) {
+        mio::TimeSeries<double> ts(2);
+        const double v[2] = {1.0, 2.0};
+        for (size_t t = 0; t < points; ++t) {
+            ts.add_time_point(double(t), v);
+        }
+        return ts;
+    };
+    auto refused = [](auto&& f) {
+        try {
+            f();
+        }
+        catch (const mio::Error& e) {
+            return e.status == MGO_ERR_INVALID_VALUE;
+        }
+        return false;
+    };
+    std::vector<std::vector<mio::TimeSeries<double>>> nodes_differ{{series(3), series(3)}, {series(3)}};
+    std::vector<std::vector<mio::TimeSeries<double>>> points_differ{{series(3)}, {series(4)}};
+    check(refused([&] { mio::ensemble_mean(nodes_differ); }), "ensemble_mean: runs with different node counts");
+    check(refused([&] { mio::ensemble_percentile(points_differ, 0.5); }),
+          "ensemble_percentile: different time points");
+    check(refused([&] { mio::ensemble_mean(std::vector<std::vector<mio::TimeSeries<double>>>{}); }),
+          "ensemble_mean: empty");
+}
+
 } // namespace
 
 int main(int argc, char** argv)
@@ -440,6 +469,7 @@ int main(int argc, char** argv)
     test_apply_constraints();
     test_sample_graph();
     test_distribution_files(argv[1]);
+    test_ensemble_input_checks();
     if (failures) {
         std::cerr << failures << " check(s) failed\n";
         return 1;

@@ -310,6 +310,30 @@ int main()
         }
         std::printf("parameter study (16 runs): max rel err %.3e\n", e);
         CHECK(e < 1e-6);
+        // ensemble statistics of the study on the device equal the reference's arithmetic on the host:
+        // run-order sum of value / num_runs, and the sorted run values at floor(num_runs * p)
+        auto ens  = mio::interpolate_ensemble_results(results);
+        auto mean = mio::ensemble_mean(ens);
+        auto p90  = mio::ensemble_percentile(ens, 0.9);
+        CHECK(ens.size() == 16 && mean.size() == ens[0].size() && p90.size() == ens[0].size());
+        bool same = true;
+        for (size_t n = 0; n < mean.size() && same; ++n) {
+            for (size_t t = 0; t < mean[n].get_num_time_points(); ++t) {
+                for (size_t k = 0; k < mean[n].get_num_elements(); ++k) {
+                    double acc = 0.0;
+                    std::vector<double> col;
+                    for (const auto& run : ens) {
+                        acc += run[n].value_data(t)[k] / double(ens.size());
+                        col.push_back(run[n].value_data(t)[k]);
+                    }
+                    std::sort(col.begin(), col.end());
+                    same = same && mean[n].value_data(t)[k] == acc &&
+                           p90[n].value_data(t)[k] == col[static_cast<size_t>(double(col.size()) * 0.9)];
+                }
+            }
+        }
+        std::printf("ensemble mean / 90th percentile on the device: %s\n", same ? "bitwise equal" : "DIFFER");
+        CHECK(same);
     }
     // 3b. ParameterStudy::run with the reference's sampling in create_simulation (draw_sample of a copy of the
     //     parameter graph, make_sampled_graph_simulation): each run's result equals the oracle of its sample
