@@ -319,6 +319,10 @@ mgo_status mgo_ensemble_mean_values(mgo_context* ctx, const double* values, int3
                                     double* mean_out);
 mgo_status mgo_ensemble_percentiles_values(mgo_context* ctx, const double* values, int32_t num_runs,
                                            int64_t num_positions, const double* p, int32_t num_p, double* out);
+/* sum_nodes (analyze_result.h:267-287) of host values[run][day][node][element] (num_elements per node):
+ * out[run][day][element], the nodes summed in node order, on the device (the kernel of mgo_batch_sum_nodes). */
+mgo_status mgo_sum_nodes_values(mgo_context* ctx, const double* values, int32_t num_runs, int32_t num_days,
+                                int32_t num_nodes, int32_t num_elements, double* out);
 
 /* ---------------------------------------------------------------- serving loop
  * A worker thread owned by the context runs half-day cycles over the resident requests; requests are

@@ -23,8 +23,9 @@
 //   osecir::simulate                     cpp/models/ode_secir/model.h:385-390
 //   ParameterStudy::run / run_serial     cpp/memilio/compartments/parameter_studies.h:47-130,156-217,288-345
 //   interpolate_simulation_result        cpp/memilio/data/analyze_result.h:186-265
-//   interpolate_ensemble_results,        analyze_result.h:77-86, 289-343 (the statistics computed on the
-//   ensemble_mean, ensemble_percentile   device by mgo_ensemble_*_values)
+//   interpolate_ensemble_results,        analyze_result.h:77-86, 267-343 (the statistics computed on the
+//   sum_nodes, ensemble_mean,            device by mgo_sum_nodes_values, mgo_ensemble_*_values)
+//   ensemble_percentile
 //   parameter sampling                   cpp/memilio/utils/uncertain_value.h, parameter_distributions.h,
 //                                        epidemiology/uncertain_matrix.h, damping_sampling.h:255-306,
 //                                        models/ode_secir/parameter_space.h:46-251
@@ -3007,6 +3008,35 @@ std::vector<TimeSeries<FP>> ensemble_percentile(const std::vector<std::vector<Ti
                                                   &pd, 1, q.data()),
                   ctx);
     return detail::ensemble_series(ensemble_result.front(), q.data(), nodes, T, E);
+}
+
+// sum_nodes (data/analyze_result.h:267-287): per run one series, the nodes summed in node order on the device
+// (mgo_sum_nodes_values); the times are each run's first node's
+template <class FP>
+std::vector<std::vector<TimeSeries<FP>>> sum_nodes(const std::vector<std::vector<TimeSeries<FP>>>& ensemble_result)
+{
+    size_t nodes = 0, T = 0, E = 0;
+    const std::vector<double> v = detail::ensemble_values(ensemble_result, nodes, T, E); // [run][node][t][e]
+    const size_t runs = ensemble_result.size();
+    std::vector<double> w(v.size()); // [run][t][node][e], the layout of the device's daily outputs
+    for (size_t r = 0; r < runs; ++r) {
+        for (size_t n = 0; n < nodes; ++n) {
+            for (size_t t = 0; t < T; ++t) {
+                std::copy_n(v.data() + ((r * nodes + n) * T + t) * E, E, w.data() + ((r * T + t) * nodes + n) * E);
+            }
+        }
+    }
+    std::vector<double> s(runs * T * E);
+    mgo_context* ctx = default_context();
+    detail::check(mgo_sum_nodes_values(ctx, w.data(), int32_t(runs), int32_t(T), int32_t(nodes), int32_t(E), s.data()),
+                  ctx);
+    std::vector<std::vector<TimeSeries<FP>>> r(runs, std::vector<TimeSeries<FP>>(1, TimeSeries<FP>(E)));
+    for (size_t k = 0; k < runs; ++k) {
+        for (size_t t = 0; t < T; ++t) {
+            r[k][0].add_time_point(ensemble_result[k][0].get_time(t), s.data() + (k * T + t) * E);
+        }
+    }
+    return r;
 }
 
 // ---------------------------------------------------------------------------------------------- study

@@ -2280,6 +2280,28 @@ mgo_status mgo_ensemble_percentiles_values(mgo_context* ctx, const double* value
     });
 }
 
+mgo_status mgo_sum_nodes_values(mgo_context* ctx, const double* values, int32_t num_runs, int32_t num_days,
+                                int32_t num_nodes, int32_t num_elements, double* out)
+{
+    return guarded(ctx, [&] {
+        if (!out) {
+            fail(MGO_ERR_INVALID_VALUE, "output is NULL");
+        }
+        if (num_days < 1 || num_nodes < 1 || num_elements < 1) {
+            fail(MGO_ERR_INVALID_VALUE, "days, nodes and elements must be positive");
+        }
+        const int64_t per_run = int64_t(num_days) * num_nodes * num_elements;
+        DevBuf dv, base, sums;
+        upload_values(ctx, values, num_runs, per_run, dv, base);
+        const size_t n = size_t(num_runs) * size_t(num_days) * size_t(num_elements);
+        sums.alloc(n * sizeof(double));
+        hip_check(launch_sum_nodes(dv.as<double>(), base.as<int64_t>(), num_runs, num_days, num_nodes, num_elements,
+                                   sums.as<double>(), ctx->stream),
+                  "sum nodes");
+        download_values(ctx, sums, n, out);
+    });
+}
+
 mgo_status mgo_batch_sum_nodes(mgo_context* ctx, mgo_batch_id id, int32_t first_run, int32_t num_runs, double* out)
 {
     return guarded(ctx, [&] {

@@ -334,6 +334,22 @@ int main()
         }
         std::printf("ensemble mean / 90th percentile on the device: %s\n", same ? "bitwise equal" : "DIFFER");
         CHECK(same);
+        auto sums     = mio::sum_nodes(ens);
+        bool sums_ok  = sums.size() == ens.size();
+        for (size_t r = 0; r < ens.size() && sums_ok; ++r) {
+            sums_ok = sums[r].size() == 1 && sums[r][0].get_num_time_points() == ens[r][0].get_num_time_points();
+            for (size_t t = 0; t < ens[r][0].get_num_time_points() && sums_ok; ++t) {
+                for (size_t k = 0; k < ens[r][0].get_num_elements(); ++k) {
+                    double acc = 0.0;
+                    for (const auto& node : ens[r]) {
+                        acc += node.value_data(t)[k];
+                    }
+                    sums_ok = sums_ok && sums[r][0].value_data(t)[k] == acc;
+                }
+            }
+        }
+        std::printf("sum_nodes on the device: %s\n", sums_ok ? "bitwise equal" : "DIFFER");
+        CHECK(sums_ok);
     }
     // 3b. ParameterStudy::run with the reference's sampling in create_simulation (draw_sample of a copy of the
     //     parameter graph, make_sampled_graph_simulation): each run's result equals the oracle of its sample
