@@ -30,7 +30,7 @@ __all__ = ["InfectionState", "Model", "ModelGraph", "MobilityGraph", "MobilitySi
            "ParameterStudy", "Simulation", "simulate", "interpolate_simulation_result", "interpolate_ensemble_results",
            "ensemble_mean", "ensemble_percentile", "ControlledStepperWrapper",
            "RKIntegratorCore", "ExplicitStepperWrapper", "EulerIntegratorCore", "write_graph", "read_graph",
-           "draw_sample", "set_params_distributions_normal"]
+           "draw_sample", "set_params_distributions_normal", "MultiIndex_PopulationsArray"]
 
 
 class InfectionState(enum.IntEnum):
@@ -590,6 +590,14 @@ def simulate(t0, tmax, dt, model, integrator=None):
 
 # ----------------------------------------------------------------------------------------- sampling
 _S = InfectionState
+
+
+class MultiIndex_PopulationsArray(tuple):
+    """Index (AgeGroup, InfectionState) of one compartment (the binding's MultiIndex of the populations array),
+    accepted wherever populations take (group, state)."""
+
+    def __new__(cls, group, state):
+        return super().__new__(cls, (group, state))
 
 
 def _copy_uv(dst, src):

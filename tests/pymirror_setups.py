@@ -100,3 +100,51 @@ def scale_contacts(graph, run):
     for v in range(graph.num_nodes):
         cm = graph.get_node(v).property.parameters.ContactPatterns.cont_freq_mat[0]
         cm.baseline = cm.baseline * rng.uniform(0.8, 1.2)
+
+
+def parameter_study_example_graph():
+    """The set-up of the reference's pycode/examples/simulation/ode_secir_parameter_study.py: 6 age groups,
+    TransmissionProbabilityOnContact drawn from U(0.1, 0.2), four contact matrices damped at t = 7 (draw_sample
+    rebuilds them without that damping), two nodes joined both ways with 0.01 commuting except the dead."""
+    G = 6
+    S = osecir.InfectionState
+    model = osecir.Model(G)
+    for i in range(G):
+        group = mio.AgeGroup(i)
+        model.parameters.TimeExposed[group] = 3.2
+        model.parameters.TimeInfectedNoSymptoms[group] = 2.
+        model.parameters.TimeInfectedSymptoms[group] = 6
+        model.parameters.TimeInfectedSevere[group] = 12
+        model.parameters.TimeInfectedCritical[group] = 8
+        model.populations[group, S.Exposed] = 100
+        model.populations[group, S.InfectedNoSymptoms] = 50
+        model.populations[group, S.InfectedNoSymptomsConfirmed] = 0
+        model.populations[group, S.InfectedSymptoms] = 50
+        model.populations[group, S.InfectedSymptomsConfirmed] = 0
+        model.populations[group, S.InfectedSevere] = 20
+        model.populations[group, S.InfectedCritical] = 10
+        model.populations[group, S.Recovered] = 10
+        model.populations[group, S.Dead] = 0
+        model.populations.set_difference_from_group_total_AgeGroup((group, S.Susceptible), 10000)
+        model.parameters.TransmissionProbabilityOnContact[group].set_distribution(
+            mio.ParameterDistributionUniform(0.1, 0.2))
+        model.parameters.RelativeTransmissionNoSymptoms[group] = 0.67
+        model.parameters.RecoveredPerInfectedNoSymptoms[group] = 0.09
+        model.parameters.RiskOfInfectionFromSymptomatic[group] = 0.25
+        model.parameters.SeverePerInfectedSymptoms[group] = 0.2
+        model.parameters.CriticalPerSevere[group] = 0.25
+        model.parameters.DeathsPerCritical[group] = 0.3
+    model.parameters.ContactPatterns.cont_freq_mat = mio.ContactMatrixGroup(4, G)
+    for k in range(4):
+        model.parameters.ContactPatterns.cont_freq_mat[k] = mio.ContactMatrix(np.ones((G, G)) * 0.5)
+    model.parameters.ContactPatterns.cont_freq_mat.add_damping(mio.Damping(np.ones((G, G)) * 0.7, 7.0))
+    model.apply_constraints()
+    graph = osecir.ModelGraph()
+    graph.add_node(0, model)
+    graph.add_node(1, model)
+    coeffs = 0.01 * np.ones(model.populations.numel())
+    for i in range(G):
+        coeffs[model.populations.get_flat_index(osecir.MultiIndex_PopulationsArray(mio.AgeGroup(i), S.Dead))] = 0
+    graph.add_edge(0, 1, coeffs)
+    graph.add_edge(1, 0, coeffs)
+    return graph

@@ -144,3 +144,28 @@ def test_ensemble_statistics_of_a_study(oracle):
     for p in (0.1, 0.5, 0.9):
         pct = osecir.ensemble_percentile(ens, p)
         assert np.array_equal(np.stack([np.stack(ts.values) for ts in pct]), oracle.ensemble_percentile(vals, p)), p
+
+
+def test_parameter_study_example(oracle):
+    """The reference's ode_secir_parameter_study.py: every run samples TransmissionProbabilityOnContact from
+    U(0.1, 0.2) (shared by both nodes, drawn per group) and its device result equals the oracle on the
+    simulated graph."""
+    graph = ps.parameter_study_example_graph()
+    study = osecir.GraphParameterStudy(graph, t0=0, tmax=10, dt=0.5, num_runs=3)
+    seen = []
+
+    def handle_result(g, run_idx):
+        tpc = [g.get_node(0).property.model.parameters.TransmissionProbabilityOnContact[mio.AgeGroup(i)].value
+               for i in range(6)]
+        assert all(0.1 <= v <= 0.2 for v in tpc)
+        assert tpc == [g.get_node(1).property.model.parameters.TransmissionProbabilityOnContact[mio.AgeGroup(i)].value
+                       for i in range(6)]
+        assert g.get_node(0).property.result.get_last_time() == 10
+        cfg = capi.sim_cfg(t0=0.0, dt_graph=0.5, dt_node=0.5)
+        ref, _ = oracle.graph_run(_spec(g), cfg, 10.0)
+        assert rel_err(_results(g), ref) < 1e-6
+        seen.append(tuple(tpc))
+
+    mio.seed_random_number_generator(3)
+    study.run(handle_result)
+    assert len(seen) == 3 and len(set(seen)) == 3
