@@ -148,3 +148,48 @@ def parameter_study_example_graph():
     graph.add_edge(0, 1, coeffs)
     graph.add_edge(1, 0, coeffs)
     return graph
+
+
+def mobility_parameter_study_graph():
+    """The set-up of the reference's pycode/examples/simulation/mobility_parameter_study.py: one age group, a
+    damped contact matrix (draw_sample rebuilds it without the damping), an infected node of 10000 and a
+    susceptible node of 2000 joined both ways, node 0's InfectedNoSymptoms drawn from a truncated normal."""
+    S = osecir.InfectionState
+    A = mio.AgeGroup(0)
+    model = osecir.Model(1)
+    model.parameters.TimeExposed[A] = 3.2
+    model.parameters.TimeInfectedNoSymptoms[A] = 2.
+    model.parameters.TimeInfectedSymptoms[A] = 6
+    model.parameters.TimeInfectedSevere[A] = 12
+    model.parameters.TimeInfectedCritical[A] = 8
+    model.parameters.ContactPatterns.cont_freq_mat[0].baseline = np.r_[0.5]
+    model.parameters.ContactPatterns.cont_freq_mat[0].add_damping(mio.Damping(np.r_[0.3], t=0.3))
+    model.parameters.TransmissionProbabilityOnContact[A] = 1.0
+    model.parameters.RelativeTransmissionNoSymptoms[A] = 0.67
+    model.parameters.RecoveredPerInfectedNoSymptoms[A] = 0.09
+    model.parameters.RiskOfInfectionFromSymptomatic[A] = 0.25
+    model.parameters.SeverePerInfectedSymptoms[A] = 0.2
+    model.parameters.CriticalPerSevere[A] = 0.25
+    model.parameters.DeathsPerCritical[A] = 0.3
+    graph = osecir.ModelGraph()
+    for st, v in ((S.Exposed, 100), (S.InfectedNoSymptoms, 50), (S.InfectedNoSymptomsConfirmed, 0),
+                  (S.InfectedSymptoms, 50), (S.InfectedSevere, 20), (S.InfectedCritical, 10), (S.Recovered, 10),
+                  (S.Dead, 0)):
+        model.populations[A, st] = v
+    model.populations.set_difference_from_group_total_AgeGroup((A, S.Susceptible), 10000)
+    model.apply_constraints()
+    graph.add_node(id=0, model=model)
+    for st in (S.Exposed, S.InfectedNoSymptoms, S.InfectedSymptoms, S.InfectedSevere, S.InfectedCritical,
+               S.Recovered, S.Dead):
+        model.populations[A, st] = 0
+    model.populations.set_difference_from_group_total_AgeGroup((A, S.Susceptible), 2000)
+    model.apply_constraints()
+    graph.add_node(id=1, model=model)
+    coeffs = 0.1 * np.ones(model.populations.numel())
+    coeffs[S.Dead] = 0
+    mobility_params = mio.MobilityParameters(coeffs)
+    graph.add_edge(0, 1, mobility_params)
+    graph.add_edge(1, 0, mobility_params)
+    carrier_distribution = mio.ParameterDistributionNormal(50, 2000, 200, 100, 2.5758)
+    graph.get_node(0).property.populations[A, S.InfectedNoSymptoms].set_distribution(carrier_distribution)
+    return graph

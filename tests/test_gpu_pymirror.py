@@ -169,3 +169,28 @@ def test_parameter_study_example(oracle):
     mio.seed_random_number_generator(3)
     study.run(handle_result)
     assert len(seen) == 3 and len(set(seen)) == 3
+
+
+def test_mobility_parameter_study_example(oracle):
+    """The reference's mobility_parameter_study.py: node 0's initial InfectedNoSymptoms is drawn per run from
+    the truncated normal on [50, 2000], the susceptible node keeps none, the group total stays 10000; each
+    run's device result equals the oracle on its graph."""
+    graph = ps.mobility_parameter_study_graph()
+    study = osecir.GraphParameterStudy(graph, 0, 50, dt=1.0, num_runs=3)
+    carriers = []
+
+    def handle_result(g, run_idx):
+        m0, m1 = g.get_node(0).property.model, g.get_node(1).property.model
+        ins = m0.populations[mio.AgeGroup(0), osecir.InfectionState.InfectedNoSymptoms].value
+        assert 50 <= ins <= 2000
+        assert m1.populations[mio.AgeGroup(0), osecir.InfectionState.InfectedNoSymptoms].value == 0
+        assert abs(m0.populations.get_group_total_AgeGroup(0) - 10000) < 1e-9
+        assert g.get_node(1).property.result.get_last_time() == 50
+        cfg = capi.sim_cfg(t0=0.0, dt_graph=1.0, dt_node=1.0)
+        ref, _ = oracle.graph_run(_spec(g), cfg, 50.0)
+        assert rel_err(_results(g), ref) < 1e-6
+        carriers.append(ins)
+
+    mio.seed_random_number_generator(8)
+    study.run(handle_result)
+    assert len(carriers) == 3 and len(set(carriers)) == 3
