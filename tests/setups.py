@@ -162,3 +162,82 @@ def fixup_graph(seed=4, n=30, G=3, outflow=0.95):
     sp.populations[:, :, E] *= rng.uniform(1, 30, (n, G))
     sp.age_params[:, capi.A_RISK_OF_INFECTION_FROM_SYMPTOMATIC, :] = 0.05
     return sp
+
+
+# ------------------------------------------------------------------ return fix-up known answers
+def fixup_known_answer(case):
+    """Two-node graphs with one edge 0 -> 1 on which the return of the travelers underflows the destination;
+    the expected day-1 states are derived by hand from metapopulation_mobility_instant.h:568-603 (one Euler
+    step of the travelers with the destination's model at the departure state, then the fix-up of every
+    compartment j with remaining(j) = to(j) - m(j) < 0: m(argmax of remaining over j's age group, first
+    maximum) -= remaining(j), m(j) += remaining(j), with `remaining` computed once before the loop).
+
+    The model is made linear and small: transmission probability 0 (no S -> E), T_E = 0.5, T_INS = 1,
+    RecoveredPerInfectedNoSymptoms = 0, T_ISy = T_ISev = T_ICr = inf, so per age group the only flows are
+    E -> INS (rate 2) and INS -> ISy (rate 1); S, ISev, ICr, R, D keep their values. Nodes integrate with
+    Euler (euler.h:54-62) at dt 0.25, the edge returns with one Euler step of 0.5 (graph dt), so a node's
+    half-day is (I + A/4)^2 while the travelers' return is (I + A/2): the difference A^2/16 takes INS (and
+    ISy) below what the returning travelers hold. One Euler step of 0.25: E <- E/2, INS <- 3/4 INS + E/2,
+    ISy <- ISy + INS/4. Every number below is a short binary fraction, so the results are exact in FP64.
+
+    case "single" (G=1): one compartment (INS) underflows, the fix-up takes the deficit from S.
+    case "tie" (G=1): as "single" with destination R = S = 2000: the first maximum (S) gives.
+    case "groups" (G=2): group 0 has no underflow and the largest compartment of the whole vector; group 1
+    underflows in INS and ISy, and after the first fix-up S of group 1 would no longer be its largest
+    remaining value (R = 99.5 > 100 - 1.125), so a `remaining` recomputed inside the loop, a slice over the
+    whole vector or over group 0, or a group offset left out give other results.
+
+    Returns (spec, cfg, tmax, expected) with expected[node] the [G][10] state at day 1 (after the return).
+    """
+    G = 2 if case == "groups" else 1
+    spec = GraphSpec(2, G)
+    a = spec.age_params
+    a[:, capi.A_TRANSMISSION_PROBABILITY_ON_CONTACT, :] = 0.0
+    a[:, capi.A_TIME_EXPOSED, :] = 0.5
+    a[:, capi.A_TIME_INFECTED_NO_SYMPTOMS, :] = 1.0
+    for k in (capi.A_TIME_INFECTED_SYMPTOMS, capi.A_TIME_INFECTED_SEVERE, capi.A_TIME_INFECTED_CRITICAL):
+        a[:, k, :] = np.inf
+    a[:, capi.A_RECOVERED_PER_INFECTED_NO_SYMPTOMS, :] = 0.0
+    spec.contact_baseline[:] = 1.0
+    coeff = np.zeros((1, NC * G))
+    p = spec.populations
+    exp = np.zeros((2, G, NC))
+    if case in ("single", "tie"):
+        # source: S 1000, E 64. First half-day (E, INS, ISy): (64,0,0) -> (32,32,0) -> (16,40,8).
+        # destination: S 2000, INS 1, R 500 (2000 for "tie"): (0,1,0) -> (0,0.75,0.25) -> (0,0.5625,0.4375).
+        # departure at 0.5 with c_E = 1/2: m = (E 8); source (8,40,8), destination (8,0.5625,0.4375).
+        # second half-day: source (8,40,8) -> (4,34,18) -> (2,27.5,26.5);
+        #                  destination (8,0.5625,0.4375) -> (4,4.421875,0.578125) -> (2,5.31640625,1.68359375).
+        # return: m + 0.5 A m = (E 0, INS 8, ISy 0); remaining: S 2000, E 2, INS -2.68359375, ISy 1.68359375,
+        # R 500 (2000). Fix-up of INS from S (first maximum): m = (S 2.68359375, INS 5.31640625).
+        r_dst = 2000.0 if case == "tie" else 500.0
+        p[0, 0, S], p[0, 0, E] = 1000.0, 64.0
+        p[1, 0, S], p[1, 0, INS], p[1, 0, R] = 2000.0, 1.0, r_dst
+        coeff[0, E] = 0.5
+        exp[0, 0, [S, E, INS, ISY]] = [1002.68359375, 2.0, 32.81640625, 26.5]
+        exp[1, 0, [S, E, INS, ISY, R]] = [1997.31640625, 2.0, 0.0, 1.68359375, r_dst]
+    elif case == "groups":
+        # group 0: S travels (c_S = 1/4, m_S = 250) and comes back unchanged; destination R0 = 10000, the
+        # largest value of the vector and at another index of its group than the maximum of group 1.
+        # group 1 source: S 1000, E 64 -> (16,40,8) at 0.5; c_E = 1/2, c_INS = 3/4: m = (E 8, INS 30).
+        # source after departure (8,10,8) -> (4,11.5,10.5) -> (2,10.625,13.375).
+        # destination group 1: S 100, R 99.5; after departure (8,30,0) -> (4,26.5,7.5) -> (2,21.875,14.125).
+        # return: m + 0.5 A m = (E 0, INS 23, ISy 15); remaining: S 100, E 2, INS -1.125, ISy -0.875, R 99.5.
+        # Both deficits from S of group 1 (the maximum of the remaining values computed before the loop):
+        # m = (S 2, INS 21.875, ISy 14.125).
+        p[0, 0, S] = 1000.0
+        p[0, 1, S], p[0, 1, E] = 1000.0, 64.0
+        p[1, 0, R] = 10000.0
+        p[1, 1, S], p[1, 1, R] = 100.0, 99.5
+        coeff[0, S] = 0.25
+        coeff[0, NC + E], coeff[0, NC + INS] = 0.5, 0.75
+        exp[0, 0, S] = 1000.0
+        exp[0, 1, [S, E, INS, ISY]] = [1002.0, 2.0, 32.5, 27.5]
+        exp[1, 0, R] = 10000.0
+        exp[1, 1, [S, E, INS, ISY, R]] = [98.0, 2.0, 0.0, 0.0, 99.5]
+    else:
+        raise ValueError(case)
+    spec.set_edges([0], [1], coeff)
+    cfg = capi.sim_cfg(t0=0.0, dt_graph=0.5, dt_node=0.25,
+                       integrator=capi.integrator_cfg(capi.INTEGRATOR_EULER, 0.0, 0.0, 0.0, 0.0))
+    return spec, cfg, 1.0, exp

@@ -215,6 +215,30 @@ def test_graph_negative_fixups_sequential(oracle):
     assert rel_err(out[0], ref) < 1e-12
 
 
+@pytest.mark.parametrize("case", ["single", "tie", "groups"])
+def test_return_fixup_known_answers(ctx, oracle, case):
+    """The return fix-up on the hand-derived set-ups of setups.fixup_known_answer (one edge; one underflow,
+    a tie for the largest compartment, two underflows in one of two age groups), through the batched path,
+    the plain graph path and the sequential replay: each equals the oracle and the hand-derived day-1 states
+    to 1e-12 (Euler nodes, fixed step)."""
+    spec, cfg, tmax, expected = setups.fixup_known_answer(case)
+    ref, st = oracle.graph_run(spec, cfg, tmax)
+    assert st.negative_fixups == (2 if case == "groups" else 1)
+    want = expected.reshape(2, spec.G * 10)
+    out, info = ctx.ensemble_run([spec], [cfg], [tmax])
+    assert info.fixup_passes > 0
+    sim = GraphSim(ctx, spec, cfg, horizon=tmax)
+    sim.advance(tmax)
+    plain = sim.daily()
+    sim.close()
+    with Context(flags=capi.CTX_SEQUENTIAL_FIXUP) as c2:
+        seq, sinfo = c2.ensemble_run([spec], [cfg], [tmax])
+    assert sinfo.exact_replays > 0
+    for got in (out[0], plain, seq[0]):
+        assert rel_err(got, ref) < 1e-12
+        assert rel_err(got[1], want) < 1e-12
+
+
 def test_deterministic(ctx):
     """Two identical runs are bitwise identical (no atomics in the exchange)."""
     spec = small_graph(n=20, seed=17)

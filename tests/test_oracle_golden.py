@@ -386,3 +386,17 @@ def test_return_fixup_properties(oracle):
         per_node = got.sum(axis=2)
         assert np.max(np.abs(per_node - per_node[0]) / per_node[0]) < 1e-12
         assert got.min() >= -1e-9
+
+
+@pytest.mark.parametrize("case", ["single", "tie", "groups"])
+def test_return_fixup_known_answers(oracle, case):
+    """Return fix-up (metapopulation_mobility_instant.h:576-596) against states derived by hand (see
+    setups.fixup_known_answer): the deficit moves to the largest remaining compartment of the same age group,
+    the first of equal maxima, with the remaining values taken once before the loop. Exact: every value of
+    these set-ups is a short binary fraction."""
+    spec, cfg, tmax, expected = setups.fixup_known_answer(case)
+    got, st = oracle.graph_run(spec, cfg, tmax)
+    assert got.shape == (2, 2, spec.G * 10)
+    assert st.negative_fixups == (2 if case == "groups" else 1)
+    np.testing.assert_array_equal(got[0].reshape(2, spec.G, 10), spec.populations)
+    np.testing.assert_array_equal(got[1].reshape(2, spec.G, 10), expected)
