@@ -365,7 +365,10 @@ typedef struct mgo_damping_update {
 
 typedef struct mgo_result_info {
     mgo_ticket ticket;
-    mgo_status status;          /* MGO_OK or the error that ended the request */
+    mgo_status status;          /* MGO_OK or the error that ended the request. MGO_ERR_OUT_OF_MEMORY with
+                                   num_days > 0: the request ran to its end, but a scenario update that did
+                                   not fit the serving arena was dropped (mgo_serve_updates_applied lists the
+                                   updates it ran with); the daily values are those of that run */
     double submit_ms;           /* context clock at submission */
     double admit_ms;            /* context clock at admission to the device */
     double retire_ms;           /* context clock when its last day was computed and its slot freed */
@@ -382,6 +385,13 @@ mgo_status mgo_serve_update(mgo_context* ctx, mgo_ticket ticket, const mgo_dampi
 mgo_status mgo_serve_poll(mgo_context* ctx, mgo_result_info* out, int32_t capacity, int32_t* count);
 /* Daily values of a finished request, [day][node][10*G]; releases the request's host copy. */
 mgo_status mgo_serve_take(mgo_context* ctx, mgo_ticket ticket, double* daily_out, size_t capacity_doubles);
+/* Scenario updates a finished request ran with, in the order they were applied, each with the damping
+ * time it was given (t_start resolved for updates submitted with t_start < 0); *count = their number,
+ * up to capacity of them written. Call before mgo_serve_take. MGO_ERR_BUSY while the request runs. Running
+ * the same request alone with these updates (t_start as reported) gives its result bit for bit.
+ * (No counterpart in the reference: reports what the serving loop decided.) */
+mgo_status mgo_serve_updates_applied(mgo_context* ctx, mgo_ticket ticket, mgo_damping_update* out,
+                                     int32_t capacity, int32_t* count);
 /* Waits until every submitted request is finished, then stops the worker. */
 mgo_status mgo_serve_stop(mgo_context* ctx);
 

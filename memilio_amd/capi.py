@@ -263,6 +263,7 @@ SIGNATURES = {
     "mgo_serve_poll": (C.c_int32, [C.c_void_p, C.POINTER(ResultInfo), C.c_int32, _ip]),
     "mgo_serve_get_stats": (C.c_int32, [C.c_void_p, C.POINTER(ServeStats)]),
     "mgo_serve_take": (C.c_int32, [C.c_void_p, C.c_int64, _dp, C.c_size_t]),
+    "mgo_serve_updates_applied": (C.c_int32, [C.c_void_p, C.c_int64, C.POINTER(DampingUpdate), C.c_int32, _ip]),
     "mgo_serve_stop": (C.c_int32, [C.c_void_p]),
     "mgo_clock_ms": (C.c_double, [C.c_void_p]),
 }

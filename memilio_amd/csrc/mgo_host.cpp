@@ -327,7 +327,7 @@ struct ClassHost {
     std::vector<int> insts; // global instance indices
     int num_nodes = 0;
     // device buffers
-    DevBuf y, snap, ypre, rc, lcoef, cbase, cdelta, nodep, dtime, dmat, tnode, dtnode, steps, attempts, out,
+    DevBuf y, snap, ypre, rc, yint, lcoef, cbase, cdelta, nodep, dtime, dmat, tnode, dtnode, steps, attempts, out,
         ipool, epool, cvec, cvec_min, mpool, estructs, inst_ids;
     DevBuf fx_rec, fx_pool, fx_dirty;  // fix-up records of the return passes
     DevBuf node_order;                 // node-kernel lane slot -> node (empty: identity)
@@ -1176,7 +1176,9 @@ struct Batch {
         C.snap.alloc(N * G * SNAP_STRIDE * sizeof(double));
         C.ypre.alloc(y.size() * sizeof(double));
         C.rc.alloc(N * G * RC_STRIDE * sizeof(double));
-        device_bytes += C.snap.bytes + C.ypre.bytes + C.rc.bytes;
+        C.yint.alloc(N * G * YINT_STRIDE * sizeof(double));
+        hip_check(hipMemsetAsync(C.yint.p, 0xff, C.yint.bytes, stream), "clear yint"); // NaN weights: no step yet
+        device_bytes += C.snap.bytes + C.ypre.bytes + C.rc.bytes + C.yint.bytes;
         up(C.lcoef, lcoef);
         up(C.cbase, cbase);
         up(C.cdelta, cdelta);
@@ -1272,6 +1274,7 @@ struct Batch {
         d.snap      = C.snap.as<double>();
         d.ypre      = C.ypre.as<double>();
         d.rc        = C.rc.as<double>();
+        d.yint      = C.yint.as<double>();
         d.lcoef     = C.lcoef.as<double>();
         d.cbase     = C.cbase.as<double>();
         d.cdelta    = C.cdelta.as<double>();
@@ -1486,6 +1489,7 @@ struct Batch {
                       "reset dt");
             hip_check(hipMemsetAsync(C.steps.p, 0, C.steps.bytes, stream), "reset steps");
             hip_check(hipMemsetAsync(C.attempts.p, 0, C.attempts.bytes, stream), "reset attempts");
+            hip_check(hipMemsetAsync(C.yint.p, 0xff, C.yint.bytes, stream), "reset yint");
             C.clear_fixup_records(stream);
             // damping lists, caches and NPI state as built (dynamic NPIs change them during a run)
             C.npi.reset(stream);

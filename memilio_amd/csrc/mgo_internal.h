@@ -7,6 +7,7 @@
 // per (G, integrator). Within a segment:
 //   lane slot   = node * G + g   (one GPU lane per (node, age group); 64/G nodes per wavefront)
 //   y, ypre             [node][g][10]          state, pre-return state
+//   yint                [node][g][YINT_STRIDE] state before the last step of a phase ending on an output day
 //   snap                [node][g][SNAP_STRIDE] departure row: snapshot (10), ISy mobility factor, commuter
 //                                              nondetection factor at the departure time
 //   lcoef               [node][g][NUM_LC]      per-lane rate coefficients (see LaneCoef)
@@ -87,6 +88,12 @@ enum RetCoef : int {
 constexpr int SNAP_FAC    = 10;
 constexpr int SNAP_ND     = 11;
 constexpr int SNAP_STRIDE = 12;
+
+// last step of a lane's phase that ends on an output day d: the state before that step and the weight
+// (d - t_prev) / (t_end - t_prev) of interpolate_simulation_result (analyze_result.h:235-252), or -1 when the
+// step ends before d (the next phase's first step encloses d)
+constexpr int YINT_W      = 10;
+constexpr int YINT_STRIDE = 12;
 
 // node flags
 enum : int32_t {
@@ -227,6 +234,7 @@ struct SegDev {
     double* snap;
     double* ypre;
     double* rc;
+    double* yint;                  // [node][g][YINT_STRIDE] last step before a phase-end output day
     const double* lcoef;
     const double* cbase;
     const double* cdelta;

@@ -521,6 +521,39 @@ __device__ __forceinline__ double* out_ptr(const InstDev& I, int v, int age, int
     return I.out + (size_t(day) * I.num_nodes + v) * size_t(G * NC) + size_t(age) * NC;
 }
 
+// linear_interpolation (math/interpolation.h:44-48): y_a + w (y_b - y_a), rounded as the reference (no FMA)
+__device__ __forceinline__ double lerp_ref(double ya, double yb, double w)
+{
+#pragma clang fp contract(off)
+    return ya + w * (yb - ya);
+}
+
+// Daily value of the output day d = c_out_day at the end of a phase, from the lane's state y after the phase's
+// edge action (apply_mobility changes the last value of the node's TimeSeries in place), as
+// interpolate_simulation_result (analyze_result.h:235-261) takes it from the node's time points: between the
+// state before the node's last step and y when that step ends at or after d (weight 1 when it ends on d);
+// else y, the last value so far: then the next phase's first step encloses d and overwrites the day.
+template <int G>
+__device__ __forceinline__ void write_phase_end_day(const SegDev& c, const InstDev& I, size_t lane, int v, int age,
+                                                    const double (&y)[NC])
+{
+    const double* pr = c.yint + lane * YINT_STRIDE;
+    const double w   = pr[YINT_W];
+    double* op       = out_ptr(I, v, age, G, I.c_out_day);
+    if (w >= 0.0) {
+#pragma unroll
+        for (int i = 0; i < NC; ++i) {
+            op[i] = lerp_ref(pr[i], y[i], w);
+        }
+    }
+    else {
+#pragma unroll
+        for (int i = 0; i < NC; ++i) {
+            op[i] = y[i];
+        }
+    }
+}
+
 // segment node of an instance's local node v
 __device__ __forceinline__ size_t seg_node(const InstDev& I, int v)
 {
@@ -737,8 +770,10 @@ __device__ __forceinline__ void node_body(const LaunchDev& ld, InstDev* __restri
     const double eps    = 1e-12;
     // output days strictly inside this phase are interpolated here; a day at the phase end is written
     // after the edge action
+    // (the first is the day after the node's own time: a fixed-step node can end a phase just below the day
+    // the phase ended on, and then its first step here encloses that day)
     const int ndays  = I.ndays;
-    int next_day     = max(0, int(floor(I.c_t0 + 1e-10)) + 1 - I.day0);
+    int next_day     = max(0, int(floor(t)) + 1 - I.day0);
     const double tde = I.c_t1 - 1e-10;
 
     bool active  = false;
@@ -895,17 +930,28 @@ __device__ __forceinline__ void node_body(const LaunchDev& ld, InstDev* __restri
                         double* op     = out_ptr(I, L.np.local, L.age, G, next_day);
 #pragma unroll
                         for (int i = 0; i < NC; ++i) {
-                            op[i] = y[i] + w * (yo[i] - y[i]);
+                            op[i] = lerp_ref(y[i], yo[i], w);
                         }
                     }
                     ++next_day;
+                }
+                active = fabs((tmax - t) / (tmax - t0)) > eps;
+                if (!active && I.c_out_day >= 0) {
+                    // the last step of the phase (so far): the phase-end day is interpolated from it after the
+                    // edge action (write_phase_end_day)
+                    const double dd = double(I.c_out_day + I.day0);
+                    double* pr      = c.yint + (size_t(L.node) * G + L.age) * YINT_STRIDE;
+#pragma unroll
+                    for (int i = 0; i < NC; ++i) {
+                        pr[i] = y[i];
+                    }
+                    pr[YINT_W] = (t >= dd) ? (dd - ta) / (t - ta) : -1.0;
                 }
 #pragma unroll
                 for (int i = 0; i < NC; ++i) {
                     y[i] = yo[i];
                 }
                 need_k1 = true;
-                active  = fabs((tmax - t) / (tmax - t0)) > eps;
             }
         }
         if (!npi || stalled) {
@@ -954,11 +1000,7 @@ __device__ __forceinline__ void node_body(const LaunchDev& ld, InstDev* __restri
         }
     }
     else if (I.c_out_day >= 0) {
-        double* op = out_ptr(I, L.np.local, L.age, G, I.c_out_day);
-#pragma unroll
-        for (int i = 0; i < NC; ++i) {
-            op[i] = y[i];
-        }
+        write_phase_end_day<G>(c, I, size_t(L.node) * G + L.age, L.np.local, L.age, y);
     }
 }
 
@@ -1173,11 +1215,7 @@ __device__ __forceinline__ void depart_body(const LaunchDev& ld, InstDev* __rest
         yp[i] = y[i];
     }
     if (I.c_out_day >= 0) {
-        double* op = out_ptr(I, v, age, G, I.c_out_day);
-#pragma unroll
-        for (int i = 0; i < NC; ++i) {
-            op[i] = y[i];
-        }
+        write_phase_end_day<G>(c, I, lane_off, v, age, y);
     }
 }
 
@@ -1343,11 +1381,7 @@ __device__ __forceinline__ void return_body(const LaunchDev& ld, InstDev* __rest
         yp[i] = y[i];
     }
     if (I.c_out_day >= 0) {
-        double* op = out_ptr(I, v, age, G, I.c_out_day);
-#pragma unroll
-        for (int i = 0; i < NC; ++i) {
-            op[i] = y[i];
-        }
+        write_phase_end_day<G>(c, I, lane_off, v, age, y);
     }
 }
 
@@ -1439,11 +1473,12 @@ __device__ __forceinline__ void replay_body(const LaunchDev& ld, InstDev* __rest
     }
     if (I.c_out_day >= 0 && a < G) {
         for (int v = 0; v < I.num_nodes; ++v) {
-            const size_t off = (seg_node(I, v) * G + a) * NC;
-            double* op       = out_ptr(I, v, a, G, I.c_out_day);
+            const size_t lane = seg_node(I, v) * G + a;
+            double yv[NC];
             for (int i = 0; i < NC; ++i) {
-                op[i] = c.y[off + i];
+                yv[i] = c.y[lane * NC + i];
             }
+            write_phase_end_day<G>(c, I, lane, v, a, yv);
         }
     }
 }

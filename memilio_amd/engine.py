@@ -268,6 +268,17 @@ class Server:
                 time.sleep(0.0005)
         return {t: self.finished[t] for t in want}
 
+    def updates_applied(self, ticket):
+        """Scenario updates a finished request ran with: [(scale, t_start, level, type)], t_start as applied."""
+        cap = 64
+        while True:
+            arr = (capi.DampingUpdate * cap)()
+            n = C.c_int32(0)
+            self.ctx.check(self.ctx.lib.mgo_serve_updates_applied(self.ctx.ptr, ticket, arr, cap, C.byref(n)))
+            if n.value <= cap:
+                return [(arr[k].scale, arr[k].t_start, arr[k].level, arr[k].type) for k in range(n.value)]
+            cap = n.value
+
     def stats(self):
         """Counters of the serving loop (mgo_serve_get_stats) as a dict."""
         st = capi.ServeStats()

@@ -71,6 +71,33 @@ def lib():
     return _lib
 
 
+_fma_lib = None
+
+
+def graph_run_fma(spec, cfg, tmax):
+    """graph_run with liboracle_fma.so: the same restatement compiled with FMA contraction (x86-64-v3), a
+    second valid compilation of the reference arithmetic; tests use it to bound the drift between builds."""
+    global _fma_lib
+    if _fma_lib is None:
+        path = os.path.join(HERE, "liboracle_fma.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"oracle library {path} missing; run __graft_entry__.build()")
+        L = C.CDLL(path)
+        L.orc_graph_run.restype = C.c_int32
+        L.orc_graph_run.argtypes = [C.POINTER(capi.GraphDesc), C.POINTER(capi.SimCfg), C.c_double, C.c_int32, _dp,
+                                    _ip, C.POINTER(OrcStats)]
+        _fma_lib = L
+    nd = spec.num_days(cfg.t0, tmax)
+    out = np.zeros((nd, spec.num_nodes, spec.state_size()))
+    got = C.c_int32(0)
+    st = OrcStats()
+    s = _fma_lib.orc_graph_run(C.byref(spec.desc()), C.byref(cfg), tmax, nd, capi.dptr(out), C.byref(got),
+                               C.byref(st))
+    if s != 0:
+        raise RuntimeError(f"orc_graph_run (FMA build) failed: {s}")
+    return out[:got.value], st
+
+
 def edge_npi_calls(n, threshold, base, duration, value, weights, times, inf_rel):
     """Number of coefficient dampings after each apply_mobility NPI check (see orc_edge_npi_calls)."""
     w = np.ascontiguousarray(weights, dtype=np.float64)

@@ -14,7 +14,7 @@ import pytest
 
 from memilio_amd import synthetic
 from memilio_amd.engine import Batch, Context, GraphSim
-from tests.test_oracle_math import closed_form_daily, no_transmission_graph
+from tests.test_oracle_math import closed_form_daily, compartment_scale_err, no_transmission_graph
 
 pytestmark = pytest.mark.gpu
 
@@ -95,3 +95,34 @@ def test_closed_form_on_device(ctx, cfg, tol):
     sim.close()
     ref = closed_form_daily(spec, np.arange(got.shape[0], dtype=np.float64))
     assert rel_err(got, ref) < tol
+
+
+@pytest.mark.parametrize("kind", ["fixed", "adaptive"])
+def test_plain_graph_path_full_size(ctx, oracle, kind):
+    """The plain single-graph path (mgo_graph_create / mgo_graph_advance, make_mobility_sim(...).advance) on
+    the 400-node, 6-group graph with Poisson(50) out-degree over 100 days, every compartment of every node every
+    day. Adaptive Cash-Karp 1e-10: 1e-6 relative. Fixed step 0.1: 1e-12 on the scale of each compartment's
+    course (its largest value over the run) and 1e-9 relative to the current value; at this horizon two
+    compilations of the reference arithmetic itself (with and without FMA contraction) differ by more than 1e-12
+    relative in the compartments that have decayed by orders of magnitude (test_build_drift_bounds_the_per_day_bar).
+    Advancing in two calls (47.5, then 100) gives the same values as one call."""
+    spec = synthetic.graph(num_nodes=400, G=6, seed=1)
+    cfg = synthetic.fixed_cfg(0.1) if kind == "fixed" else synthetic.ck_cfg(1e-10)
+    sim = GraphSim(ctx, spec, cfg, horizon=100.0)
+    sim.advance(100.0)
+    got = sim.daily()
+    sim.close()
+    assert got.shape == (101, 400, 60)
+    ref, st = oracle.graph_run(spec, cfg, 100.0)
+    assert st.returns > 0
+    if kind == "fixed":
+        assert compartment_scale_err(got, ref) < 1e-12
+        assert rel_err(got, ref) < 1e-9
+    else:
+        assert rel_err(got, ref) < 1e-6
+    sim = GraphSim(ctx, spec, cfg, horizon=100.0)
+    sim.advance(47.5)
+    sim.advance(100.0)
+    split = sim.daily()
+    sim.close()
+    assert np.array_equal(split, got)

@@ -194,3 +194,93 @@ def test_serve_errors():
             Server(ctx)   # already serving
         assert e.value.status == capi.ERR_INVALID_VALUE
         srv.stop()
+
+
+def test_serve_poisson_load_with_updates(oracle):
+    """Serving under load (C4 shape): Poisson arrivals at 200 requests/s for 1.5 s of solves drawn from the
+    uneven-batch distribution (1..2000 nodes, 1..8 age groups, 30..180 days, rel tol 1e-6/1e-8/1e-10), 10 % of
+    the arrivals being contact-scaling updates U[0.3, 1] (from the next day) on a resident request. Every
+    result finishes OK and keeps each node's total every day; every result equals, bit for bit, the same
+    request served alone with the updates the loop reports it ran with (mgo_serve_updates_applied); the four
+    smallest results with updates, and four without, equal the oracle run with those updates as dampings of
+    every node's contact matrix (graph_simulation.h:101-120, contact_matrix.h:190-194)."""
+    import threading
+    import time
+    pool = synthetic.uneven(48, seed=3)
+    rate, seconds = 200.0, 1.5
+    rng = np.random.default_rng([7, 200])
+    n_arr = int(rate * seconds)
+    gaps = rng.exponential(1.0 / rate, n_arr)
+    is_update = rng.random(n_arr) < 0.1
+    picks = rng.integers(0, len(pool), n_arr)
+    scales = rng.uniform(0.3, 1.0, n_arr)
+    with Context() as ctx:
+        srv = Server(ctx)
+        tickets, live, req_of = [], [], {}
+        t_next = time.perf_counter()
+        for k in range(n_arr):
+            t_next += gaps[k]
+            while time.perf_counter() < t_next:
+                time.sleep(0.0005)
+            for r in srv.poll():
+                srv.finished[r.ticket] = r   # Server.wait reads them there
+            if is_update[k]:
+                live = [tk for tk in live if tk not in srv.finished]
+                if live:
+                    tk = live[int(rng.integers(0, len(live)))]
+                    try:
+                        srv.update(tk, float(scales[k]))
+                    except MgoError as e:
+                        assert e.status == capi.ERR_OUT_OF_RANGE   # finished meanwhile
+                continue
+            spec, horizon, tol = pool[picks[k]]
+            cfg = synthetic.ck_cfg(tol)
+            tk = srv.submit(spec, cfg, horizon)
+            tickets.append(tk)
+            live.append(tk)
+            req_of[tk] = (spec, cfg, horizon)
+        infos = srv.wait(tickets, timeout_s=120.0)
+        applied = {tk: srv.updates_applied(tk) for tk in tickets}
+        served = {tk: srv.take(tk) for tk in tickets}
+        # each request again, alone, with the updates it ran with
+        alone = {}
+        for tk in tickets:
+            spec, cfg, horizon = req_of[tk]
+            t2 = srv.submit(spec, cfg, horizon)
+            for (s, ts, lev, typ) in applied[tk]:
+                srv.update(t2, s, ts, lev, typ)
+            srv.wait([t2], timeout_s=60.0)
+            assert srv.updates_applied(t2) == applied[tk]
+            alone[tk] = srv.take(t2)
+        srv.stop()
+    n_updates = sum(len(v) for v in applied.values())
+    assert len(tickets) > 200 and n_updates > 10
+    for tk in tickets:
+        assert infos[tk].status == capi.OK, tk
+        got = served[tk]
+        spec, cfg, horizon = req_of[tk]
+        assert got.shape == (spec.num_days(cfg.t0, horizon), spec.num_nodes, spec.state_size())
+        tot = got.sum(axis=2)
+        assert np.max(np.abs(tot - tot[0]) / tot[0]) < 1e-10, tk
+        assert np.array_equal(got, alone[tk]), f"request {tk} differs served in the mix vs alone"
+        for (s, ts, lev, typ) in applied[tk]:
+            assert 0.3 <= s <= 1.0 and cfg.t0 + 1.0 <= ts <= horizon + 1.0
+    by_size = sorted(tickets, key=lambda tk: req_of[tk][0].num_nodes * req_of[tk][2])
+    sample = [tk for tk in by_size if applied[tk]][:4] + [tk for tk in by_size if not applied[tk]][:4]
+    for tk in sample:
+        spec, cfg, horizon = req_of[tk]
+        if applied[tk]:
+            spec = copy_spec(spec)
+            spec.set_dampings([[(ts, lev, typ, 1.0 - s) for (s, ts, lev, typ) in applied[tk]]] * spec.num_nodes)
+        assert rel_err(served[tk], _oracle_daily(oracle, spec, cfg, horizon)) < 1e-6, tk
+
+
+def copy_spec(spec):
+    """A GraphSpec with its own copies of every array (dampings are set on the copy)."""
+    import copy
+    c = copy.copy(spec)
+    for name, v in vars(spec).items():
+        if isinstance(v, np.ndarray):
+            setattr(c, name, v.copy())
+    c._desc = None
+    return c
