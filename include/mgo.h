@@ -319,6 +319,18 @@ mgo_status mgo_ensemble_mean_values(mgo_context* ctx, const double* values, int3
                                     double* mean_out);
 mgo_status mgo_ensemble_percentiles_values(mgo_context* ctx, const double* values, int32_t num_runs,
                                            int64_t num_positions, const double* p, int32_t num_p, double* out);
+
+/* Allocation-free form of mgo_ensemble_percentiles_values for values already in device memory (no counterpart
+ * in the reference; for callers that cannot allocate: code under HIP graph capture, tight serving loops).
+ * values_dev[run][position] and out_dev[q][position] are device pointers of the caller; the per-run offsets and
+ * ranks the kernels read live in a fixed device scratch of the library (one per GPU, held until the process
+ * ends): at most 8192 runs and 64 percentiles, else MGO_ERR_OUT_OF_RANGE; no device or host memory is
+ * allocated. stream (a hipStream_t) = NULL: the call runs on the context's stream and returns when out_dev is
+ * complete; else it is enqueued on that stream and returns at once. Calls overlapping on one GPU (several
+ * streams, contexts or threads) share the scratch and race: order them on one stream or synchronise. */
+mgo_status mgo_ensemble_percentiles_noalloc(mgo_context* ctx, const double* values_dev, int32_t num_runs,
+                                            int64_t num_positions, const double* p, int32_t num_p, double* out_dev,
+                                            void* stream);
 /* sum_nodes (analyze_result.h:267-287) of host values[run][day][node][element] (num_elements per node):
  * out[run][day][element], the nodes summed in node order, on the device (the kernel of mgo_batch_sum_nodes). */
 mgo_status mgo_sum_nodes_values(mgo_context* ctx, const double* values, int32_t num_runs, int32_t num_days,

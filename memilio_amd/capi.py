@@ -256,6 +256,8 @@ SIGNATURES = {
     "mgo_batch_sum_nodes": (C.c_int32, [C.c_void_p, C.c_int64, C.c_int32, C.c_int32, _dp]),
     "mgo_ensemble_mean_values": (C.c_int32, [C.c_void_p, _dp, C.c_int32, C.c_int64, _dp]),
     "mgo_ensemble_percentiles_values": (C.c_int32, [C.c_void_p, _dp, C.c_int32, C.c_int64, _dp, C.c_int32, _dp]),
+    "mgo_ensemble_percentiles_noalloc": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int64, _dp, C.c_int32,
+                                                     C.c_void_p, C.c_void_p]),
     "mgo_sum_nodes_values": (C.c_int32, [C.c_void_p, _dp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _dp]),
     "mgo_serve_start": (C.c_int32, [C.c_void_p, C.POINTER(ServeCfg)]),
     "mgo_serve_submit": (C.c_int32, [C.c_void_p, C.POINTER(Request), C.POINTER(C.c_int64)]),

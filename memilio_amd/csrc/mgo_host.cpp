@@ -1644,6 +1644,55 @@ struct Batch {
 
 } // namespace
 
+// ------------------------------------------------------------------------------------------ scratch
+// Device scratch of the allocation-free entry point (mgo_ensemble_percentiles_noalloc): per-run offsets of
+// the values and the percentile ranks the select kernel reads. One copy per GPU, held until the process
+// ends; host code reaches it only through the address hipGetSymbolAddress returns for the current device.
+constexpr int kScratchRuns  = 8192;
+constexpr int kScratchRanks = 64;
+constexpr size_t kScratchBytes = size_t(kScratchRuns) * sizeof(int64_t) + size_t(kScratchRanks) * sizeof(int32_t);
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static void* g_scratch_dev[kMaxDevices] = {};
+static std::mutex g_scratch_mu;
+
+// address of this device's scratch, looked up once per device with that device current
+static hipError_t scratch_setup(int device)
+{
+    if (device < 0 || device >= kMaxDevices) {
+        return hipErrorInvalidDevice;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_dev[device]) {
+        return hipSuccess;
+    }
+    void* ptr        = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e == hipSuccess) {
+        g_scratch_dev[device] = ptr;
+    }
+    return e;
+}
+
+struct SelectRanks {
+    int32_t r[kScratchRanks];
+};
+
+// per-run offsets run * positions and the ranks into the scratch (a kernel, so the call records no memset
+// or copy node when captured)
+__global__ void noalloc_select_setup(int64_t* base, int runs, int64_t positions, SelectRanks ranks, int num_ranks,
+                                     int32_t* dranks)
+{
+    const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < runs) {
+        base[i] = int64_t(i) * positions;
+    }
+    if (i < num_ranks) {
+        dranks[i] = ranks.r[i];
+    }
+}
+
 // ------------------------------------------------------------------------------------------ context
 struct ServeState;
 
@@ -1740,6 +1789,11 @@ mgo_status mgo_context_create(const mgo_device_cfg* cfg, mgo_context** out)
     }
     if (hipSetDevice(ctx->device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         tl_error = "cannot create HIP stream";
+        return MGO_ERR_DEVICE;
+    }
+    if (scratch_setup(ctx->device) != hipSuccess) {
+        (void)hipStreamDestroy(ctx->stream);
+        tl_error = "cannot resolve the device scratch";
         return MGO_ERR_DEVICE;
     }
     size_t free_b = 0, total_b = 0;
@@ -2281,6 +2335,44 @@ mgo_status mgo_ensemble_percentiles_values(mgo_context* ctx, const double* value
                                          dranks.as<int32_t>(), num_p, sel.as<double>(), ctx->stream),
                   "ensemble percentiles");
         download_values(ctx, sel, size_t(num_positions) * size_t(num_p), out);
+    });
+}
+
+mgo_status mgo_ensemble_percentiles_noalloc(mgo_context* ctx, const double* values_dev, int32_t num_runs,
+                                            int64_t num_positions, const double* p, int32_t num_p, double* out_dev,
+                                            void* stream)
+{
+    return guarded(ctx, [&] {
+        if (!values_dev || !out_dev || !p || num_p < 1 || num_runs < 1 || num_positions < 1) {
+            fail(MGO_ERR_INVALID_VALUE, "values, percentiles or output missing or empty");
+        }
+        if (num_runs > kScratchRuns || num_p > kScratchRanks) {
+            fail(MGO_ERR_OUT_OF_RANGE, "%d runs / %d percentiles exceed the scratch of %d runs / %d percentiles",
+                 num_runs, num_p, kScratchRuns, kScratchRanks);
+        }
+        SelectRanks rk{};
+        for (int q = 0; q < num_p; ++q) {
+            if (!(p[q] > 0.0 && p[q] < 1.0)) {
+                fail(MGO_ERR_INVALID_VALUE, "percentile %g outside (0, 1)", p[q]);
+            }
+            rk.r[q] = int32_t(static_cast<size_t>(double(num_runs) * p[q])); // analyze_result.h:339
+        }
+        void* scr = g_scratch_dev[ctx->device];
+        if (!scr) {
+            fail(MGO_ERR_DEVICE, "device scratch not resolved");
+        }
+        auto* base     = static_cast<int64_t*>(scr);
+        auto* dranks   = reinterpret_cast<int32_t*>(static_cast<unsigned char*>(scr) + kScratchRuns * sizeof(int64_t));
+        hipStream_t st = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+        const int n    = std::max(num_runs, num_p);
+        hipLaunchKernelGGL(noalloc_select_setup, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, base, num_runs,
+                           num_positions, rk, num_p, dranks);
+        hip_check(hipGetLastError(), "scratch setup");
+        hip_check(launch_ensemble_select(values_dev, base, num_runs, num_positions, dranks, num_p, out_dev, st),
+                  "ensemble percentiles");
+        if (!stream) {
+            hip_check(hipStreamSynchronize(st), "ensemble percentiles");
+        }
     });
 }
 
