@@ -88,6 +88,19 @@ def ensemble_bench(args, ctx, specs, cfgs, tmaxs):
             f"{info.edge_kernel_ms:.1f} ms, steps {info.steps}, attempts {info.attempts}, fix-up passes "
             f"{info.fixup_passes}, replays {info.exact_replays}")
     ctx.set_flags(args.base_flags)
+    out["download_value"] = None
+    if not args.no_download:
+        # the same run followed by the download of every run's daily values to host memory
+        batch.reset()
+        t0 = time.perf_counter()
+        batch.run()
+        daily = batch.read_daily()
+        dl = time.perf_counter() - t0
+        nbytes = sum(d.nbytes for d in daily)
+        del daily
+        out["download_value"] = node_days / dl
+        log(f"run + download of {nbytes / 1e9:.1f} GB of daily values: {dl * 1e3:.1f} ms -> "
+            f"{out['download_value']:.3e} node-days/s")
     total = sum(times)
     out["value"] = node_days * args.steps / total
     out["ms_per_step"] = total / args.steps * 1e3
@@ -261,8 +274,10 @@ def main():
     ap.add_argument("--nodes", type=int, default=400)
     ap.add_argument("--groups", type=int, default=6)
     ap.add_argument("--days", type=float, default=100.0)
-    ap.add_argument("--cpu-samples", type=int, default=8)
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-samples", type=int, default=64)
+    ap.add_argument("--cpu-threads", type=int, default=16,
+                    help="OpenMP threads of the CPU baseline (capped at the host's core count)")
+    ap.add_argument("--no-download", action="store_true", help="skip the run timed with its result download")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
@@ -324,27 +339,54 @@ def main():
         f"({job_roof['bound']}), fraction {job_roof['frac']:.3f}")
 
     plain_value = None if args.no_plain else plain_bench(ctx, base, cfg, days)
+    single_value = None
+    if not args.no_plain:
+        # one G=6 model over 90 days (osecir::simulate) on the device
+        one = synthetic.single_model(seed=0, G=G)
+        ctx.simulate_single(one, cfg, 90.0)
+        t0 = time.perf_counter()
+        ctx.simulate_single(one, cfg, 90.0)
+        single_value = 90.0 / (time.perf_counter() - t0)
+        log(f"single model on the device: {single_value:.3e} node-days/s")
     uneven = uneven_bench(args, ctx) if args.uneven > 0 else None
     serving = None if args.no_serve else serving_bench(args, ctx)
     ctx.close()
 
     cpu = None
     max_rel = None
+    cpu_other = None
     if not args.no_cpu:
         from oracle import oracle_py
+        nproc = os.cpu_count() or 1
+        threads = max(1, min(args.cpu_threads, nproc))
         cpu_n = min(args.cpu_samples, S)
         sub = specs[:cpu_n]
         t0 = time.perf_counter()
-        ref, st = oracle_py.ensemble_run(sub, cfgs[:cpu_n], tmaxs[:cpu_n], threads=args.cpu_threads)
+        ref, st = oracle_py.ensemble_run(sub, cfgs[:cpu_n], tmaxs[:cpu_n], threads=threads, fma=True)
         cpu_s = time.perf_counter() - t0
-        cpu = {"value": cpu_n * N * days / cpu_s, "unit": "node-days/s", "cores": args.cpu_threads,
+        cpu = {"value": cpu_n * N * days / cpu_s, "unit": "node-days/s", "cores": threads, "host_nproc": nproc,
                "kind": "port",
-               "sample": f"{cpu_n} of the {S} samples ({cpu_n * N * days:.0f} node-days), oracle = reference "
-                         f"algorithm restated in C++ (-O3, no FMA contraction), OpenMP over samples, "
-                         f"{cpu_s:.1f} s"}
+               "sample": f"{cpu_n} of the {S} samples ({cpu_n * N * days:.0f} node-days, extrapolated linearly to "
+                         f"the ensemble), the reference algorithm restated in C++ (oracle/oracle.cpp) built -O3 "
+                         f"-march=x86-64-v3 with FMA contraction, OpenMP over samples on {threads} of {nproc} "
+                         f"host threads, {cpu_s:.1f} s"}
         max_rel = max(float(np.max(np.abs(g - r) / np.maximum(np.abs(r), 1e-6)))
-                      for g, r in zip(ens["sample"], ref))
-        log(f"cpu baseline: {cpu['value']:.3e} node-days/s on {args.cpu_threads} threads; max rel err {max_rel:.2e}")
+                      for g, r in zip(ens["sample"], ref[:len(ens["sample"])]))
+        log(f"cpu baseline: {cpu['value']:.3e} node-days/s on {threads} of {nproc} threads; max rel err of the GPU "
+            f"samples vs this build {max_rel:.2e}")
+        # the other configurations on the CPU: one model on one core, the single graph (serial), as the
+        # reference runs them (no OpenMP inside one simulation)
+        one = synthetic.single_model(seed=0, G=G)
+        t0 = time.perf_counter()
+        oracle_py.simulate_daily(one, cfg, 90.0)
+        c0_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        oracle_py.graph_run(base, cfg, days)
+        c1_s = time.perf_counter() - t0
+        cpu_other = {"single_model_node_days_per_s": 90.0 / c0_s, "single_graph_node_days_per_s": N * days / c1_s,
+                     "note": "oracle (-O3, no FMA contraction), one core: one G=6 model over 90 days; the "
+                             f"{N}-node graph over {days:g} days"}
+        log(f"cpu: one model {90.0 / c0_s:.3e} node-days/s, single graph {N * days / c1_s:.3e} node-days/s (one core)")
 
     # HBM bytes per node-kernel launch of this workload, measured by rocprofv3 PMC passes (FETCH_SIZE,
     # WRITE_SIZE) and scaled by the counters' reading for a copy of known size (profiles/)
@@ -375,9 +417,13 @@ def main():
                    "days": days, "edges": int(base.num_edges), "tile": ens["tile"]},
         "graph_replay_node_days_per_s": ens["graph_value"],
         "plain_gpu_node_days_per_s": plain_value,
+        "with_download_node_days_per_s": ens["download_value"],
+        "single_model_gpu_node_days_per_s": single_value,
+        "cpu_other": cpu_other,
         "batched_vs_plain": value / plain_value if plain_value else None,
         "max_rel_err_vs_oracle": max_rel,
-        "roofline": {"bound": "mfma", "roof": "FP64 vector ALU (78.6 TFLOP/s = FP64 peak of MI355X)",
+        "roofline": {"bound": "fp64-valu", "roof": "FP64 vector ALU (78.6 TFLOP/s = FP64 peak of MI355X; no MFMA "
+                                                   "is used), latency-limited at one wavefront per SIMD",
                      "kernel": f"node_phase<G={G},CashKarp54>", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
                      "traffic_source": traffic_src, "avg_launch_ms": node_ms, "launches": launches,

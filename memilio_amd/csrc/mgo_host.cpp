@@ -2002,7 +2002,7 @@ static void set_run_tmax(Batch& b, const std::vector<double>& tmax, hipStream_t 
 
 // Tile autotuner. Takes the largest group of runs that share a graph; if it has at least four runs, each
 // candidate tile is timed on a trial batch of up to 256 of those runs over two simulated days (direct
-// launches, one warm-up run, then the timed run) and the fastest is kept for this shape (number of age
+// launches, one warm-up run, then the faster of two timed runs) and the fastest is kept for this shape (number of age
 // groups, integrator, nodes, edges, group size bucket) for the life of the context.
 static int autotune_tile(mgo_context* ctx, const std::vector<RunSpec>& runs)
 {
@@ -2059,10 +2059,13 @@ static int autotune_tile(mgo_context* ctx, const std::vector<RunSpec>& runs)
         tb.use_graph = false;
         tb.build(trial, ctx->stream, c);
         set_run_tmax(tb, tmax, ctx->stream);
-        double ms = 0;
+        double ms = 0, ms2 = 0;
         tb.run(&ms); // warm-up
         tb.reset();
         tb.run(&ms);
+        tb.reset();
+        tb.run(&ms2);
+        ms = std::min(ms, ms2); // the faster of two timed runs: one slow run does not decide
         if (ms < best_ms) {
             best_ms   = ms;
             best_tile = c;

@@ -74,9 +74,10 @@ def lib():
 _fma_lib = None
 
 
-def graph_run_fma(spec, cfg, tmax):
-    """graph_run with liboracle_fma.so: the same restatement compiled with FMA contraction (x86-64-v3), a
-    second valid compilation of the reference arithmetic; tests use it to bound the drift between builds."""
+def fma_lib():
+    """liboracle_fma.so: the same restatement compiled for x86-64-v3 with FMA contraction, a second valid
+    compilation of the reference arithmetic (tests bound the drift between builds; bench.py times it as the
+    optimised CPU build)."""
     global _fma_lib
     if _fma_lib is None:
         path = os.path.join(HERE, "liboracle_fma.so")
@@ -86,7 +87,18 @@ def graph_run_fma(spec, cfg, tmax):
         L.orc_graph_run.restype = C.c_int32
         L.orc_graph_run.argtypes = [C.POINTER(capi.GraphDesc), C.POINTER(capi.SimCfg), C.c_double, C.c_int32, _dp,
                                     _ip, C.POINTER(OrcStats)]
+        L.orc_ensemble_run.restype = C.c_int32
+        L.orc_ensemble_run.argtypes = [C.POINTER(capi.EnsembleDesc), C.c_int32, _dp, C.POINTER(OrcStats)]
+        L.orc_simulate_daily.restype = C.c_int32
+        L.orc_simulate_daily.argtypes = [C.POINTER(capi.GraphDesc), C.POINTER(capi.SimCfg), C.c_double, C.c_int32,
+                                         _dp, C.POINTER(OrcStats)]
         _fma_lib = L
+    return _fma_lib
+
+
+def graph_run_fma(spec, cfg, tmax):
+    """graph_run with the FMA build (fma_lib)."""
+    fma_lib()
     nd = spec.num_days(cfg.t0, tmax)
     out = np.zeros((nd, spec.num_nodes, spec.state_size()))
     got = C.c_int32(0)
@@ -212,8 +224,9 @@ def graph_node_series(spec, cfg, tmax, node, capacity=400000):
     return times[:np_].copy(), vals[:np_].copy()
 
 
-def ensemble_run(specs, cfgs, tmaxs, threads=0, want_output=True):
-    """ParameterStudy over pre-sampled graphs; returns (list of [day][node][10G] arrays or None, stats)."""
+def ensemble_run(specs, cfgs, tmaxs, threads=0, want_output=True, fma=False):
+    """ParameterStudy over pre-sampled graphs; returns (list of [day][node][10G] arrays or None, stats).
+    threads: OpenMP threads over runs (0 = all); fma: run the FMA build (fma_lib)."""
     nr = len(specs)
     graphs = (capi.GraphDesc * nr)(*[s.desc() for s in specs])
     cfg_arr = (capi.SimCfg * nr)(*cfgs)
@@ -222,7 +235,8 @@ def ensemble_run(specs, cfgs, tmaxs, threads=0, want_output=True):
     sizes = [s.num_days(c.t0, t) * s.num_nodes * s.state_size() for s, c, t in zip(specs, cfgs, tm)]
     out = np.zeros(sum(sizes)) if want_output else None
     st = OrcStats()
-    s = lib().orc_ensemble_run(C.byref(desc), threads, capi.dptr(out) if want_output else None, C.byref(st))
+    L = fma_lib() if fma else lib()
+    s = L.orc_ensemble_run(C.byref(desc), threads, capi.dptr(out) if want_output else None, C.byref(st))
     if s != 0:
         raise RuntimeError(f"orc_ensemble_run failed: {s}")
     if not want_output:
