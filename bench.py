@@ -422,6 +422,7 @@ def main():
         "cpu_other": cpu_other,
         "batched_vs_plain": value / plain_value if plain_value else None,
         "max_rel_err_vs_oracle": max_rel,
+        "accuracy_ok": None if max_rel is None else bool(max_rel <= 1e-6),
         "roofline": {"bound": "fp64-valu", "roof": "FP64 vector ALU (78.6 TFLOP/s = FP64 peak of MI355X; no MFMA "
                                                    "is used), latency-limited at one wavefront per SIMD",
                      "kernel": f"node_phase<G={G},CashKarp54>", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,

@@ -169,6 +169,10 @@ struct DevBuf {
     DevBuf& operator=(const DevBuf&) = delete;
     ~DevBuf()
     {
+        release();
+    }
+    void release()
+    {
         if (p) {
             if (arena) {
                 arena->free(p, bytes);
@@ -177,9 +181,13 @@ struct DevBuf {
                 (void)hipFree(p);
             }
         }
+        p     = nullptr;
+        bytes = 0;
+        arena = nullptr;
     }
     void alloc(size_t b)
     {
+        release(); // a second alloc / upload on the same buffer replaces it instead of leaking it
         bytes = std::max<size_t>(b, 8);
         arena = tl_arena;
         if (arena) {
