@@ -318,3 +318,25 @@ def test_build_drift_bounds_the_per_day_bar():
     assert compartment_scale_err(b, a) < 1e-13
     totals = a.sum(axis=2)
     assert float(np.max(np.abs(b.sum(axis=2) - totals) / totals)) < 1e-14
+
+
+def test_error_norm_selection_is_ulp_level():
+    """The device takes odeint's error norm max_i n_i / d_i (default_error_checker, n_i = |err_i|, d_i = abs +
+    rel (|y_i| + dt |k1_i|)) by picking the index of the largest quotient with cross-multiplied comparisons and
+    dividing once; odeint divides every element and takes the maximum. On near-tied quotients (the only case
+    where the two can pick different elements) the results differ by at most one unit in the last place, so the
+    step-size decision differs only for norms within an ulp of the accept threshold 1."""
+    rng = np.random.default_rng(5)
+    worst = 0.0
+    for _ in range(2000):
+        d = rng.uniform(1e-3, 1e3, 10)
+        q = rng.uniform(0.5, 2.0)
+        n = q * d * (1.0 + rng.integers(-2, 3, 10) * np.finfo(float).eps)   # quotients within a few ulp
+        en, ed = 0.0, 1.0
+        for ni, di in zip(n, d):
+            if ni * ed > en * di:
+                en, ed = ni, di
+        e_cross = en / ed
+        e_max = float(np.max(n / d))
+        worst = max(worst, abs(e_cross - e_max) / np.spacing(e_max))
+    assert worst <= 1.0
