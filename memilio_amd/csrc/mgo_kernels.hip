@@ -94,6 +94,13 @@ __device__ __forceinline__ bool group_all(bool v, int base)
     return m == ((1ull << G) - 1);
 }
 
+// x^p for the step-size controllers' factors (x > 0, |p| < 1): exp2(p log2 x) within a few ulp of pow, at
+// about half of its cost (pow carries log(x) in extended precision for arbitrary exponents)
+__device__ __forceinline__ double step_pow(double x, double p)
+{
+    return exp2(p * log2(x));
+}
+
 // Per-lane context of one node lane, loaded once per kernel.
 template <int G>
 struct Lane {
@@ -857,7 +864,7 @@ __device__ __forceinline__ void node_body(const LaunchDev& ld, InstDev* __restri
                     double e = en / ed;
                     e = group_max<G>(e, L.base);
                     if (e > 1.0) {
-                        dt *= fmax(0.9 * pow(e, -1.0 / 3.0), 0.2);
+                        dt *= fmax(0.9 * step_pow(e, -1.0 / 3.0), 0.2);
                         dt   = fmin(dt, dt_max);
                         fail = true;
                     }
@@ -865,7 +872,7 @@ __device__ __forceinline__ void node_body(const LaunchDev& ld, InstDev* __restri
                         t += dt;
                         if (e < 0.5) {
                             e = fmax(1.0 / 3125.0, e);
-                            dt *= 0.9 * pow(e, -0.2);
+                            dt *= 0.9 * step_pow(e, -0.2);
                             dt = fmin(dt, dt_max);
                         }
                         fail = false;
@@ -898,7 +905,7 @@ __device__ __forceinline__ void node_body(const LaunchDev& ld, InstDev* __restri
                 if (conv || !valid) {
                     t += dt;
                 }
-                double dt_new = dt * pow(q, 1.0 / 5.0);
+                double dt_new = dt * step_pow(q, 1.0 / 5.0);
                 dt_new *= 0.9;
                 dt   = fmin(dt_new, dt_max);
                 done = conv || !valid;
