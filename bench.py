@@ -383,10 +383,16 @@ def main():
         t0 = time.perf_counter()
         oracle_py.graph_run(base, cfg, days)
         c1_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        oracle_py.graph_run_threads(base, cfg, days, threads, fma=True)
+        c1p_s = time.perf_counter() - t0
         cpu_other = {"single_model_node_days_per_s": 90.0 / c0_s, "single_graph_node_days_per_s": N * days / c1_s,
+                     "single_graph_omp_node_days_per_s": N * days / c1p_s, "single_graph_omp_threads": threads,
                      "note": "oracle (-O3, no FMA contraction), one core: one G=6 model over 90 days; the "
-                             f"{N}-node graph over {days:g} days"}
-        log(f"cpu: one model {90.0 / c0_s:.3e} node-days/s, single graph {N * days / c1_s:.3e} node-days/s (one core)")
+                             f"{N}-node graph over {days:g} days; and that graph with its node phase on {threads} "
+                             "OpenMP threads (x86-64-v3 FMA build, edges sequential)"}
+        log(f"cpu: one model {90.0 / c0_s:.3e} node-days/s, single graph {N * days / c1_s:.3e} node-days/s (one "
+            f"core), {N * days / c1p_s:.3e} node-days/s (node phase on {threads} threads)")
 
     # HBM bytes per node-kernel launch of this workload, measured by rocprofv3 PMC passes (FETCH_SIZE,
     # WRITE_SIZE) and scaled by the counters' reading for a copy of known size (profiles/)

@@ -1295,6 +1295,7 @@ struct GraphSim {
     std::vector<SimulationNode> nodes;
     std::vector<EdgeEntry> edges; // sorted by (start, end) as Graph::add_edge keeps them (graph.h:231-241)
     orc_stats* st = nullptr;
+    int node_threads = 1; // > 1: the node phase of each half-day runs on that many OpenMP threads
 
     void add_edge(int s, int e, const Vec& c)
     {
@@ -1322,8 +1323,18 @@ struct GraphSim {
             if (t + step > t_max) {
                 step = t_max - t;
             }
-            for (auto& n : nodes) {
-                n.advance(t, step);
+            if (node_threads > 1) {
+                // nodes advance independently (each owns its model, integrator and TimeSeries); the edges
+                // below stay sequential in the reference's order
+#pragma omp parallel for schedule(dynamic, 4) num_threads(node_threads)
+                for (int k = 0; k < int(nodes.size()); ++k) {
+                    nodes[size_t(k)].advance(t, step);
+                }
+            }
+            else {
+                for (auto& n : nodes) {
+                    n.advance(t, step);
+                }
             }
             t += step;
             for (auto& e : edges) {
@@ -1518,14 +1529,15 @@ static void build_graph(const mgo_graph_desc* g, const mgo_sim_cfg* cfg, GraphSi
 }
 
 static int32_t run_graph(const mgo_graph_desc* g, const mgo_sim_cfg* cfg, double tmax, int32_t capacity_days,
-                         double* daily_out, int32_t* num_days, orc_stats* stats)
+                         double* daily_out, int32_t* num_days, orc_stats* stats, int node_threads = 1)
 {
     if (!valid_desc(g) || !cfg || !make_core(cfg->integrator)) {
         return MGO_ERR_INVALID_VALUE;
     }
     GraphSim sim;
     orc_stats local{};
-    sim.st = &local;
+    sim.st           = &local;
+    sim.node_threads = node_threads;
     build_graph(g, cfg, sim);
     sim.advance(tmax);
     const size_t n = size_t(g->num_groups) * NC;
@@ -1614,6 +1626,12 @@ int32_t orc_graph_run(const mgo_graph_desc* graph, const mgo_sim_cfg* cfg, doubl
                       double* daily_out, int32_t* num_days, orc_stats* stats)
 {
     return run_graph(graph, cfg, tmax, capacity_days, daily_out, num_days, stats);
+}
+
+int32_t orc_graph_run_threads(const mgo_graph_desc* graph, const mgo_sim_cfg* cfg, double tmax, int32_t node_threads,
+                              int32_t capacity_days, double* daily_out, int32_t* num_days, orc_stats* stats)
+{
+    return run_graph(graph, cfg, tmax, capacity_days, daily_out, num_days, stats, node_threads < 1 ? 1 : node_threads);
 }
 
 int32_t orc_graph_run_node_series(const mgo_graph_desc* graph, const mgo_sim_cfg* cfg, double tmax, int32_t node,

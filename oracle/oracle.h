@@ -42,6 +42,11 @@ int32_t orc_simulate_daily(const mgo_graph_desc* model, const mgo_sim_cfg* cfg, 
 int32_t orc_graph_run(const mgo_graph_desc* graph, const mgo_sim_cfg* cfg, double tmax, int32_t capacity_days,
                       double* daily_out, int32_t* num_days, orc_stats* stats);
 
+/* orc_graph_run with the node phase of every half-day on node_threads OpenMP threads (the CPU timing of a
+ * single graph parallel over nodes; results equal orc_graph_run bit for bit). */
+int32_t orc_graph_run_threads(const mgo_graph_desc* graph, const mgo_sim_cfg* cfg, double tmax, int32_t node_threads,
+                              int32_t capacity_days, double* daily_out, int32_t* num_days, orc_stats* stats);
+
 /* Same as orc_graph_run but also returns the full TimeSeries of one node (result of the reference's
  * get_result()); returns the number of points of that node (or negative status). */
 int32_t orc_graph_run_node_series(const mgo_graph_desc* graph, const mgo_sim_cfg* cfg, double tmax, int32_t node,

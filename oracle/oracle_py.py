@@ -43,6 +43,9 @@ def lib():
         L.orc_graph_run.restype = C.c_int32
         L.orc_graph_run.argtypes = [C.POINTER(capi.GraphDesc), C.POINTER(capi.SimCfg), C.c_double, C.c_int32, _dp,
                                     _ip, C.POINTER(OrcStats)]
+        L.orc_graph_run_threads.restype = C.c_int32
+        L.orc_graph_run_threads.argtypes = [C.POINTER(capi.GraphDesc), C.POINTER(capi.SimCfg), C.c_double, C.c_int32,
+                                            C.c_int32, _dp, _ip, C.POINTER(OrcStats)]
         L.orc_graph_run_node_series.restype = C.c_int32
         L.orc_graph_run_node_series.argtypes = [C.POINTER(capi.GraphDesc), C.POINTER(capi.SimCfg), C.c_double,
                                                 C.c_int32, C.c_int32, _dp, _dp]
@@ -208,6 +211,26 @@ def graph_run(spec, cfg, tmax):
     s = lib().orc_graph_run(C.byref(spec.desc()), C.byref(cfg), tmax, nd, capi.dptr(out), C.byref(got), C.byref(st))
     if s != 0:
         raise RuntimeError(f"orc_graph_run failed: {s}")
+    return out[:got.value], st
+
+
+def graph_run_threads(spec, cfg, tmax, node_threads, fma=False):
+    """graph_run with the node phase on node_threads OpenMP threads (bitwise equal to graph_run); fma: the FMA
+    build (fma_lib)."""
+    L = fma_lib() if fma else lib()
+    if fma and not getattr(L, "_threads_bound", False):
+        L.orc_graph_run_threads.restype = C.c_int32
+        L.orc_graph_run_threads.argtypes = [C.POINTER(capi.GraphDesc), C.POINTER(capi.SimCfg), C.c_double,
+                                            C.c_int32, C.c_int32, _dp, _ip, C.POINTER(OrcStats)]
+        L._threads_bound = True
+    nd = spec.num_days(cfg.t0, tmax)
+    out = np.zeros((nd, spec.num_nodes, spec.state_size()))
+    got = C.c_int32(0)
+    st = OrcStats()
+    s = L.orc_graph_run_threads(C.byref(spec.desc()), C.byref(cfg), tmax, node_threads, nd, capi.dptr(out),
+                                C.byref(got), C.byref(st))
+    if s != 0:
+        raise RuntimeError(f"orc_graph_run_threads failed: {s}")
     return out[:got.value], st
 
 

@@ -400,3 +400,14 @@ def test_return_fixup_known_answers(oracle, case):
     assert st.negative_fixups == (2 if case == "groups" else 1)
     np.testing.assert_array_equal(got[0].reshape(2, spec.G, 10), spec.populations)
     np.testing.assert_array_equal(got[1].reshape(2, spec.G, 10), expected)
+
+
+def test_node_parallel_graph_run_is_bitwise_serial(oracle):
+    """The oracle's graph run with its node phase on several OpenMP threads (the CPU timing of one graph
+    parallel over nodes) equals the serial run bit for bit: nodes advance independently, edges stay in order."""
+    spec = setups.fixup_graph()
+    cfg = synthetic.ck_cfg(1e-10)
+    a, sa = oracle.graph_run(spec, cfg, 12.0)
+    b, sb = oracle.graph_run_threads(spec, cfg, 12.0, 4)
+    assert np.array_equal(a, b)
+    assert (sa.steps, sa.negative_fixups) == (sb.steps, sb.negative_fixups)
