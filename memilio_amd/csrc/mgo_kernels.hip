@@ -101,6 +101,17 @@ __device__ __forceinline__ double step_pow(double x, double p)
     return exp2(p * log2(x));
 }
 
+// 1/x for a population x >= 1e-12: the hardware reciprocal refined by two Newton steps (within an ulp of the
+// correctly rounded quotient, half the instructions of the IEEE division sequence)
+__device__ __forceinline__ double recip_pop(double x)
+{
+    double r       = __builtin_amdgcn_rcp(x);
+    double e       = fma(-x, r, 1.0);
+    r              = fma(r, e, r);
+    e              = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
 // Per-lane context of one node lane, loaded once per kernel.
 template <int G>
 struct Lane {
@@ -194,7 +205,7 @@ __device__ __forceinline__ double force_of_infection(const SegDev& c, Lane<G>& L
         risk             = smoother_cosine(ttr, L.np.ttc, L.np.ttc_hi, L.c[LC_RISK], L.c[LC_MAXRISK]);
     }
     const double Nj   = pop[S] + pop[E] + pop[INS] + pop[ISY] + pop[ISEV] + pop[ICR] + pop[R];
-    const double divN = (Nj < 1e-12) ? 0.0 : 1.0 / Nj;
+    const double divN = (Nj < 1e-12) ? 0.0 : recip_pop(Nj);
     const double w    = divN * (L.c[LC_REL] * pop[INS] + risk * pop[ISY]);
     if (L.np.flags & NF_DAMPED) {
         contact_row<G>(c, L, t);
