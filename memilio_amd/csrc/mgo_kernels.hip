@@ -290,7 +290,7 @@ constexpr double E1 = 37.0 / 378.0 - 2825.0 / 27648.0, E3 = 250.0 / 621.0 - 1857
 template <int G>
 __device__ __forceinline__ void cash_karp_stages(const SegDev& c, Lane<G>& L, const double (&y)[NC],
                                                  const double (&k1)[NC], double t, double h, double (&yo)[NC],
-                                                 double (&er)[NC], bool want_err)
+                                                 double (&er)[NC], bool want_err, double& season_end)
 {
     const StageSeasons<G> ss = stage_seasons<G>(L, t, h, ck::C2, ck::C3, ck::C4, ck::C5, ck::C6);
     double k2[NC], k3[NC], k4[NC], k5[NC], yt[NC];
@@ -313,7 +313,8 @@ __device__ __forceinline__ void cash_karp_stages(const SegDev& c, Lane<G>& L, co
     for (int i = 0; i < NC; ++i) {
         yt[i] = y[i] + (ck::A51 * h) * k1[i] + (ck::A52 * h) * k2[i] + (ck::A53 * h) * k3[i] + (ck::A54 * h) * k4[i];
     }
-    rhs<G>(c, L, yt, t + ck::C5 * h, stage_season<G, 3>(L, ss), k5);
+    season_end = stage_season<G, 3>(L, ss); // C5 = 1: the season at t + h, the next step's k1 time
+    rhs<G>(c, L, yt, t + ck::C5 * h, season_end, k5);
 #pragma unroll
     for (int i = 0; i < NC; ++i) {
         yt[i] = y[i] + (ck::A61 * h) * k1[i] + (ck::A62 * h) * k2[i] + (ck::A63 * h) * k3[i] + (ck::A64 * h) * k4[i] +
@@ -352,7 +353,7 @@ constexpr double D1 = BH1 - BL1, D3 = BH3 - BL3, D4 = BH4 - BL4, D5 = BH5 - BL5,
 template <int G>
 __device__ __forceinline__ void rkf45_stages(const SegDev& c, Lane<G>& L, const double (&y)[NC],
                                              const double (&k1)[NC], double t, double h, double (&yo)[NC],
-                                             double (&er)[NC])
+                                             double (&er)[NC], double& season_end)
 {
     const StageSeasons<G> ss = stage_seasons<G>(L, t, h, rkf::C2, rkf::C3, rkf::C4, rkf::C5, rkf::C6);
     double k2[NC], k3[NC], k4[NC], k5[NC], k6[NC], yt[NC];
@@ -375,7 +376,8 @@ __device__ __forceinline__ void rkf45_stages(const SegDev& c, Lane<G>& L, const 
     for (int i = 0; i < NC; ++i) {
         yt[i] = y[i] + (h * rkf::A51) * k1[i] + (h * rkf::A52) * k2[i] + (h * rkf::A53) * k3[i] + (h * rkf::A54) * k4[i];
     }
-    rhs<G>(c, L, yt, t + rkf::C5 * h, stage_season<G, 3>(L, ss), k5);
+    season_end = stage_season<G, 3>(L, ss); // C5 = 1: the season at t + h, the next step's k1 time
+    rhs<G>(c, L, yt, t + rkf::C5 * h, season_end, k5);
 #pragma unroll
     for (int i = 0; i < NC; ++i) {
         yt[i] = y[i] + (h * rkf::A61) * k1[i] + (h * rkf::A62) * k2[i] + (h * rkf::A63) * k3[i] +
@@ -796,6 +798,9 @@ __device__ __forceinline__ void node_body(const LaunchDev& ld, InstDev* __restri
 
     bool active  = false;
     bool need_k1 = true;
+    // season of the next k1: an accepted step ends at t + h, where its fifth stage evaluated the season already
+    bool have_season   = false;
+    double season_next = 0.0, season_end = 0.0;
     bool fail = true, valid = true, first = true, stalled = false;
     uint32_t nsteps = 0, natt = 0;
     double k1[NC];
@@ -837,7 +842,8 @@ __device__ __forceinline__ void node_body(const LaunchDev& ld, InstDev* __restri
                     dt         = tmax - t;
                     dt_min     = fmin(tmax - t, dt_min);
                 }
-                rhs<G>(c, L, y, t, season_at<G>(L, t), k1);
+                rhs<G>(c, L, y, t, have_season ? season_next : season_at<G>(L, t), k1);
+                have_season = false;
                 need_k1 = false;
                 fail    = true;
                 valid   = true;
@@ -859,7 +865,7 @@ __device__ __forceinline__ void node_body(const LaunchDev& ld, InstDev* __restri
                     fail = true;
                 }
                 else {
-                    cash_karp_stages<G>(c, L, y, k1, t, dt, yo, er, true);
+                    cash_karp_stages<G>(c, L, y, k1, t, dt, yo, er, true, season_end);
                     // max_i |err_i| / (abs + rel (|y_i| + dt |k1_i|)) (odeint default_error_checker): the index of
                     // the largest quotient by cross-multiplied comparison, then one division
                     double en = 0.0, ed = 1.0;
@@ -881,6 +887,8 @@ __device__ __forceinline__ void node_body(const LaunchDev& ld, InstDev* __restri
                     }
                     else {
                         t += dt;
+                        have_season = true;
+                        season_next = season_end;
                         if (e < 0.5) {
                             e = fmax(1.0 / 3125.0, e);
                             dt *= 0.9 * step_pow(e, -0.2);
@@ -902,7 +910,7 @@ __device__ __forceinline__ void node_body(const LaunchDev& ld, InstDev* __restri
                     valid = false;
                     dt    = dt_min;
                 }
-                rkf45_stages<G>(c, L, y, k1, t, dt, yo, er);
+                rkf45_stages<G>(c, L, y, k1, t, dt, yo, er, season_end);
                 bool conv = true;
                 double q  = INFINITY;
 #pragma unroll
@@ -915,6 +923,8 @@ __device__ __forceinline__ void node_body(const LaunchDev& ld, InstDev* __restri
                 q    = group_min<G>(q, L.base);
                 if (conv || !valid) {
                     t += dt;
+                    have_season = true;
+                    season_next = season_end;
                 }
                 double dt_new = dt * step_pow(q, 1.0 / 5.0);
                 dt_new *= 0.9;
@@ -925,8 +935,10 @@ __device__ __forceinline__ void node_body(const LaunchDev& ld, InstDev* __restri
                 }
             }
             else if (K == MGO_INTEGRATOR_CASH_KARP54_FIXED) {
-                cash_karp_stages<G>(c, L, y, k1, t, dt, yo, er, false);
+                cash_karp_stages<G>(c, L, y, k1, t, dt, yo, er, false, season_end);
                 t += dt;
+                have_season = true;
+                season_next = season_end;
                 done = true;
             }
             else { // Euler
