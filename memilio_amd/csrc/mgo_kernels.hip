@@ -121,6 +121,7 @@ struct Lane {
     int node;
     int age;
     int base;         // first lane of the node's group
+    double* wx;       // LDS: the wavefront's exchange slots, one double per lane (force of infection)
 };
 
 // Effective contact row C(t)_age,. = B - D(t) o (B - M) (contact_matrix.h:190-194, damping.h:381-396)
@@ -210,11 +211,16 @@ __device__ __forceinline__ double force_of_infection(const SegDev& c, Lane<G>& L
     if (L.np.flags & NF_DAMPED) {
         contact_row<G>(c, L, t);
     }
+    // w of the node's G lanes through the wavefront's LDS slots: one write, G reads (a wavefront's LDS
+    // instructions complete in order; the lanes of a node are active together)
+    L.wx[threadIdx.x % WAVE] = w;
+    __builtin_amdgcn_wave_barrier();
     double lam = 0.0;
 #pragma unroll
     for (int j = 0; j < G; ++j) {
-        lam += L.crow[j] * __shfl(w, L.base + j);
+        lam += L.crow[j] * L.wx[L.base + j];
     }
+    __builtin_amdgcn_wave_barrier();
     c_icr = L.c[LC_ISEV_ICR];
     c_d   = L.c[LC_ISEV_D];
     if (L.np.flags & NF_ICU_ACTIVE) {
@@ -395,8 +401,9 @@ __device__ __forceinline__ void rkf45_stages(const SegDev& c, Lane<G>& L, const 
 }
 
 template <int G>
-__device__ __forceinline__ const SegDev* load_lane(const LaunchDev& ld, Lane<G>& L)
+__device__ __forceinline__ const SegDev* load_lane(const LaunchDev& ld, Lane<G>& L, double* wx)
 {
+    L.wx = wx + (threadIdx.x / WAVE) * WAVE;
     constexpr int NPW = WAVE / G;
     const int lb      = xcd_block();
     if (lb >= ld.num_blocks) {
@@ -757,10 +764,10 @@ __device__ void out_damping(const SegDev& c, Lane<G>& L, InstDev& I, const doubl
 constexpr uint32_t kMaxAttemptsPerPhase = 1u << 20;
 
 template <int G, int K, bool NPI = false>
-__device__ __forceinline__ void node_body(const LaunchDev& ld, InstDev* __restrict__ inst)
+__device__ __forceinline__ void node_body(const LaunchDev& ld, InstDev* __restrict__ inst, double* wx)
 {
     Lane<G> L;
-    const SegDev* cp = load_lane<G>(ld, L);
+    const SegDev* cp = load_lane<G>(ld, L, wx);
     if (!cp) {
         return;
     }
@@ -1543,13 +1550,15 @@ __device__ __forceinline__ int block_groups(const LaunchDev& ld)
 template <int G, int K, bool NPI>
 __global__ void __launch_bounds__(BLOCK) node_phase(LaunchDev ld, InstDev* __restrict__ inst)
 {
-    node_body<G, K, NPI>(ld, inst);
+    __shared__ double wx[BLOCK];
+    node_body<G, K, NPI>(ld, inst, wx);
 }
 
 template <int K>
 __global__ void __launch_bounds__(BLOCK) node_phase_mixed(LaunchDev ld, InstDev* __restrict__ inst)
 {
-    MGO_DISPATCH_G(block_groups(ld), (node_body<GG, K>(ld, inst)))
+    __shared__ double wx[BLOCK];
+    MGO_DISPATCH_G(block_groups(ld), (node_body<GG, K>(ld, inst, wx)))
 }
 
 template <int G>
