@@ -121,7 +121,8 @@ struct Lane {
     int node;
     int age;
     int base;         // first lane of the node's group
-    double* wx;       // LDS: the wavefront's exchange slots, one double per lane (force of infection)
+    double* wx;       // LDS: the wavefront's exchange slots, one double per lane (force of infection, error norm)
+    double* ws;       // LDS: the wavefront's stage-season slots, (5 + G - 1) / G planes of BLOCK doubles
 };
 
 // Effective contact row C(t)_age,. = B - D(t) o (B - M) (contact_matrix.h:190-194, damping.h:381-396)
@@ -183,14 +184,16 @@ __device__ __forceinline__ StageSeasons<G> stage_seasons(const Lane<G>& L, doubl
         const int j    = L.age + q * G;
         const double c = (j == 0) ? c2 : (j == 1) ? c3 : (j == 2) ? c4 : (j == 3) ? c5 : c6;
         s.v[q]         = (j < 5) ? season_at<G>(L, t + c * h) : 1.0;
+        L.ws[q * BLOCK + threadIdx.x % WAVE] = s.v[q]; // read by the node's lanes from the wavefront's slots
     }
+    __builtin_amdgcn_wave_barrier();
     return s;
 }
 
 template <int G, int J>
-__device__ __forceinline__ double stage_season(const Lane<G>& L, const StageSeasons<G>& s)
+__device__ __forceinline__ double stage_season(const Lane<G>& L, const StageSeasons<G>&)
 {
-    return __shfl(s.v[J / G], L.base + J % G);
+    return L.ws[(J / G) * BLOCK + L.base + J % G];
 }
 
 // Force-of-infection coefficient of the lane's age group: season * tp * sum_j C_gj(t) w_j(pop),
@@ -404,6 +407,7 @@ template <int G>
 __device__ __forceinline__ const SegDev* load_lane(const LaunchDev& ld, Lane<G>& L, double* wx)
 {
     L.wx = wx + (threadIdx.x / WAVE) * WAVE;
+    L.ws = wx + BLOCK + (threadIdx.x / WAVE) * WAVE;
     constexpr int NPW = WAVE / G;
     const int lb      = xcd_block();
     if (lb >= ld.num_blocks) {
@@ -886,7 +890,15 @@ __device__ __forceinline__ void node_body(const LaunchDev& ld, InstDev* __restri
                         }
                     }
                     double e = en / ed;
-                    e = group_max<G>(e, L.base);
+                    // maximum over the node's lanes through the wavefront's slots (the same fmax order)
+                    L.wx[threadIdx.x % WAVE] = e;
+                    __builtin_amdgcn_wave_barrier();
+                    e = L.wx[L.base];
+#pragma unroll
+                    for (int j = 1; j < G; ++j) {
+                        e = fmax(e, L.wx[L.base + j]);
+                    }
+                    __builtin_amdgcn_wave_barrier();
                     if (e > 1.0) {
                         dt *= fmax(0.9 * step_pow(e, -1.0 / 3.0), 0.2);
                         dt   = fmin(dt, dt_max);
@@ -1550,14 +1562,14 @@ __device__ __forceinline__ int block_groups(const LaunchDev& ld)
 template <int G, int K, bool NPI>
 __global__ void __launch_bounds__(BLOCK) node_phase(LaunchDev ld, InstDev* __restrict__ inst)
 {
-    __shared__ double wx[BLOCK];
+    __shared__ double wx[BLOCK * (1 + (5 + G - 1) / G)]; // exchange slots, then the stage-season planes
     node_body<G, K, NPI>(ld, inst, wx);
 }
 
 template <int K>
 __global__ void __launch_bounds__(BLOCK) node_phase_mixed(LaunchDev ld, InstDev* __restrict__ inst)
 {
-    __shared__ double wx[BLOCK];
+    __shared__ double wx[BLOCK * 6]; // exchange slots and up to five stage-season planes (G = 1)
     MGO_DISPATCH_G(block_groups(ld), (node_body<GG, K>(ld, inst, wx)))
 }
 
